@@ -1,0 +1,290 @@
+/*
+ * spmspv_amd.h -- C-ABI of the MI355X (gfx950) adaptive SpMV/SpMSpV engine.
+ *
+ * Drop-in boundary for the reference's operator surface, the class
+ * SpmspvHandle<int, unsigned int, float>
+ * (hice-spmspv/hice/la/include/spmspv/csc-spmspv/spmspv.h:88-344; below
+ * `spmspv.h` abbreviates that path). Every entry point names the reference
+ * method it replaces. Plain pointers and sizes only: device pointers are
+ * marked `d_`, host pointers are not.
+ *
+ * Contract (the reference's behaviour, with its defects fixed):
+ *  - The Input and set_ calls borrow the caller's device arrays; the handle only
+ *    frees what it allocated itself (spmspv.h:2863-2876 freed borrowed CSC).
+ *  - Every kernel sets *y_nnz when y_nnz is non-NULL (the reference's
+ *    _keyval kernels never did, spmspv.h:1881).
+ *  - Sparse outputs are sorted ascending and carry the STRUCTURAL support:
+ *    row i is present iff some column of x's support has a stored entry in
+ *    row i, even when the sum is exactly 0 (the reference's dense2sparse_inner
+ *    dropped exact zeros, spmspv.h:1882-1928, while its sort kernels kept
+ *    them). All kernels therefore return identical index sets.
+ *  - alpha is accepted and ignored, as in the reference (csr5_spmspv.h:34).
+ *  - No allocation after spmspv_alloc_pre_buffer(); no host exit(); errors
+ *    come back as SPMSPV_* codes and spmspv_last_error() text.
+ *  - All work is queued on the handle's stream (spmspv_set_stream). Calls
+ *    that return a host scalar (y_nnz, bin_len, features) synchronise that
+ *    stream, as the reference's blocking D2H reads did.
+ */
+#ifndef SPMSPV_AMD_H_
+#define SPMSPV_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: the reference's SPMSPV_* values
+ * (la/include/spmspv/csc-spmspv/detail/common.h:11-30) plus new ones. */
+#define SPMSPV_SUCCESS 0
+#define SPMSPV_UNKOWN_FORMAT -1
+#define SPMSPV_UNSUPPORTED_CSR_SPMV -2
+#define SPMSPV_UNSUPPORTED_VALUE_TYPE -3
+#define SPMSPV_UNSUPPORTED_CSR_SPMSPV -4
+#define SPMSPV_UNSUPPORTED_COO_SPMSPV -5
+#define SPMSPV_FORMAT_ERR_HOLA_SPMV -6
+#define SPMSPV_MAT_FORMAT_ERR -13
+#define SPMSPV_VEC_FORMAT_ERR -14
+#define SPMSPV_INVALID_ARGUMENT -20
+#define SPMSPV_NOT_RESERVED -21
+#define SPMSPV_DEVICE_ERROR -22
+#define SPMSPV_OUT_OF_MEMORY -23
+#define SPMSPV_NOT_IMPLEMENTED -24
+
+/* Kernel slots: the reference's 3-factor dispatch
+ * (la/test/new_test_app.cu:404-465; solution x workload x write-back). */
+#define SPMSPV_K_COL_NAIVE_ATOMIC 0 /* CscBasedNoSortNaiveSpmspv[_keyval]  spmspv.h:2117-2169 */
+#define SPMSPV_K_COL_NAIVE_SORT 1   /* CscBasedSortNaiveSpmspv            spmspv.h:2171-2201 */
+#define SPMSPV_K_COL_LB_ATOMIC 2    /* CscBasedNoSortMergeSpmspv[_keyval]  spmspv.h:2035-2088 */
+#define SPMSPV_K_COL_LB_SORT 3      /* CscBasedSortMergeSpmspv            spmspv.h:2090-2115 */
+#define SPMSPV_K_ROW_NAIVE 4        /* naivespmspv                        spmspv.h:1700-1706 */
+#define SPMSPV_K_ROW_LB 5           /* spmspv -> holaspmspv / csr5spmspv   spmspv.h:1812-1821 */
+#define SPMSPV_K_SPMV_NAIVE 6       /* naivespmv                          spmspv.h:1676-1683 */
+#define SPMSPV_K_SPMV_LB 7          /* spmv -> holaspmv / csr5spmv         spmspv.h:1789-1798 */
+#define SPMSPV_K_COL_BUCKET 8       /* CscBasedBucketSpmspv (dormant)      csc_bucket_spmspv.cuh:766-1086 */
+#define SPMSPV_NUM_SLOTS 8          /* slots the selector chooses among */
+#define SPMSPV_NUM_KERNELS 9
+
+/* Vector types, set_vector_type (spmspv.h:977-982): 0 sparse, 1 dense,
+ * 2 bit array. */
+#define SPMSPV_VEC_SPARSE 0
+#define SPMSPV_VEC_DENSE 1
+#define SPMSPV_VEC_BITARRAY 2
+
+#define SPMSPV_NUM_FEATURES 16
+
+typedef struct spmspv_handle_s* spmspv_handle_t;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* SpmspvHandle(m, n, nnz), spmspv.h:90-141. */
+int spmspv_create(spmspv_handle_t* h, int m, int n, int nnz);
+/* Destroy, spmspv.h:2863-2884 (frees only handle-owned memory). */
+int spmspv_destroy(spmspv_handle_t h);
+/* No reference counterpart: the reference used the default stream only. */
+int spmspv_set_stream(spmspv_handle_t h, void* hip_stream);
+/* warmup, spmspv.h:347-351. */
+int spmspv_warmup(spmspv_handle_t h);
+/* Text of the last error on this handle (thread-local if h is NULL). */
+const char* spmspv_last_error(spmspv_handle_t h);
+
+/* ---- matrix input ------------------------------------------------------- */
+/* InputCSR, spmspv.h:353-364 (borrows d_row_ptr[m+1], d_col[nnz], d_val[nnz]). */
+int spmspv_input_csr(spmspv_handle_t h, int nnz, int* d_row_ptr, int* d_col, float* d_val);
+/* InputCSC, spmspv.h:366-377 (borrows d_row[nnz], d_col_ptr[n+1], d_val[nnz]). */
+int spmspv_input_csc(spmspv_handle_t h, int nnz, int* d_row, int* d_col_ptr, float* d_val);
+/* ToCSC, spmspv.h:383-419: device CSR->CSC transpose into handle-owned
+ * arrays (replaces cusparseScsr2csc); rows ascending inside each column. */
+int spmspv_to_csc(spmspv_handle_t h);
+/* allocPreBuffer, spmspv.h:1722-1763: every per-call scratch buffer, sized
+ * from m, n, nnz; also the nnz-balanced row tiling (holaPreprocess,
+ * spmspv.h:1653-1665) when CSR is present. */
+int spmspv_alloc_pre_buffer(spmspv_handle_t h);
+/* deallocPreBuffer, spmspv.h:2849-2861. */
+int spmspv_dealloc_pre_buffer(spmspv_handle_t h);
+/* holaPreprocess / holaPostprocess, spmspv.h:1653-1674 (row tiling). */
+int spmspv_hola_preprocess(spmspv_handle_t h);
+int spmspv_hola_postprocess(spmspv_handle_t h);
+/* CSR5Preprocess / CSR5Postprocess / setSigma, spmspv.h:1395-1582. */
+int spmspv_csr5_preprocess(spmspv_handle_t h);
+int spmspv_csr5_postprocess(spmspv_handle_t h);
+int spmspv_set_sigma(spmspv_handle_t h, int sigma);
+/* set_spmv_type, spmspv.h:1766-1771: 0 Hola-style, 1 CSR5 for K5/K7. */
+int spmspv_set_spmv_type(spmspv_handle_t h, int code);
+
+/* ---- x input ------------------------------------------------------------ */
+/* set_vector_type, spmspv.h:977-982. */
+int spmspv_set_vector_type(spmspv_handle_t h, int code);
+/* set_sparse_x, spmspv.h:991-1002 (borrows d_key[xnnz], d_val[xnnz];
+ * keys distinct, in [0,n), any order). */
+int spmspv_set_sparse_x(spmspv_handle_t h, int* d_key, float* d_val, int xnnz);
+/* set_x, spmspv.h:984-989 (borrows d_x[n]). */
+int spmspv_set_x(spmspv_handle_t h, float* d_x);
+/* set_bitvector, spmspv.h:1004-1011 (borrows d_bits[num_ints]). */
+int spmspv_set_bitvector(spmspv_handle_t h, int* d_bits, int num_ints);
+/* set_sparse_x_dense_x, spmspv.h:1013-1025. */
+int spmspv_set_sparse_x_dense_x(spmspv_handle_t h, int* d_key, float* d_val, float* d_x, int xnnz);
+
+/* ---- vector format conversions (spmspv.h:1121-1390, 1880-1998) ---------- */
+int spmspv_sparse2dense(spmspv_handle_t h);     /* sparse2dense    :1239-1268 */
+int spmspv_sparse2bitarray(spmspv_handle_t h);  /* sparse2bitarray :1270-1307 */
+int spmspv_dense2sparse(spmspv_handle_t h);     /* dense2sparse    :1176-1223 */
+int spmspv_dense2bitarray(spmspv_handle_t h);   /* dense2bitarray  :1371-1389 */
+int spmspv_bitarray2sparse(spmspv_handle_t h);  /* bitarray2sparse :1310-1352 */
+/* dense2sparse_outer, spmspv.h:1950-1998 (keeps entries != 0; *ynnz set). */
+int spmspv_dense2sparse_outer(spmspv_handle_t h, int m, float* d_dense, int* ynnz, int* d_key,
+                              float* d_val);
+/* sparse2dense_outer, spmspv.h:1930-1948. */
+int spmspv_sparse2dense_outer(spmspv_handle_t h, int ynnz, int* d_key, float* d_val, int m,
+                              float* d_dense);
+
+/* ---- features (spmspv.h:2259-2753) -------------------------------------- */
+/* computeBinlenVer2 / computeBinlen, spmspv.h:2363-2402, 2702-2721. */
+int spmspv_compute_binlen(spmspv_handle_t h, long long* bin_len);
+/* computeVecFeture_serial, spmspv.h:2337-2361, on the device. */
+int spmspv_vec_features(spmspv_handle_t h, long long* bin_len, int* max_len, int* min_len);
+/* computeMatFeture_serial, spmspv.h:2274-2334, on the device, from the CSR
+ * row pointer: out[7] = {max, min, avg, range, std, edge_equality, gini}. */
+int spmspv_mat_features(spmspv_handle_t h, float out[7]);
+/* extractMatFeatureParallel, spmspv.h:920-976: stdRow with int mean. */
+int spmspv_extract_mat_feature(spmspv_handle_t h, float* std_row);
+/* computeGM1/2/3, spmspv.h:2724-2753 (needs bin_len). out[3]. */
+int spmspv_compute_gm(spmspv_handle_t h, double out[3]);
+/* The selector's 16-entry feature vector for the current x (no reference
+ * counterpart; built from the features above). */
+int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]);
+
+/* ---- kernels ------------------------------------------------------------ */
+/* Runs kernel slot `kernel_id` on the current x:
+ *   d_y_dense (m floats, nullable) receives the dense result;
+ *   d_y_key/d_y_val (capacity m, nullable) receive the sorted sparse result;
+ *   y_nnz (host, nullable) receives the support size (synchronises).
+ * The needed x conversions (sparse->dense, sparse->bit array) are done first
+ * when x was set as sparse, exactly as the reference harness did before the
+ * row kernels (la/test/new_test_app.cu:318-330). */
+int spmspv_run(spmspv_handle_t h, int kernel_id, float alpha, float* d_y_dense, int* d_y_key,
+               float* d_y_val, int* y_nnz);
+/* Device address of the last result's support size (int), for callers that
+ * do not want the host synchronisation of y_nnz. */
+int* spmspv_ynnz_device(spmspv_handle_t h);
+
+/* Reference method names, each = spmspv_run with the matching slot. */
+int spmspv_csc_nosort_naive(spmspv_handle_t h, float alpha, int* ynnz, float* d_y);
+int spmspv_csc_nosort_naive_keyval(spmspv_handle_t h, float alpha, int* ynnz, float* d_y,
+                                   int* d_y_key, float* d_y_val);
+int spmspv_csc_sort_naive(spmspv_handle_t h, float alpha, int* ynnz, int* d_y_key, float* d_y_val,
+                          int method);
+int spmspv_csc_nosort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
+                            float* d_y, int* d_y_key, float* d_y_val);
+int spmspv_csc_nosort_merge_keyval(spmspv_handle_t h, int is_binlen_computed, float alpha,
+                                   int* ynnz, float* d_y, int* d_y_key, float* d_y_val);
+int spmspv_csc_sort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
+                          float* d_y, int* d_y_key, float* d_y_val);
+int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* d_y_key, float* d_y_val,
+                      int buckets);
+int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_holaspmspv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_csr5spmspv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_spmspv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_naivespmv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_holaspmv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_csr5spmv(spmspv_handle_t h, float alpha, float* d_y);
+int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y);
+
+/* ---- learned selector (no reference counterpart in C++: the reference's
+ * predictors are stubs returning 0, la/test/new_test_app.cu:121-131) ------ */
+/* MLP 16 -> 64 -> 64 -> 8, FP32. Parameters packed W1[64*16] b1[64]
+ * W2[64*64] b2[64] W3[8*64] b3[8] (row-major, out x in), followed by the
+ * 16 feature means and 16 inverse standard deviations used to normalise
+ * the input. */
+#define SPMSPV_MLP_F 16
+#define SPMSPV_MLP_H1 64
+#define SPMSPV_MLP_H2 64
+#define SPMSPV_MLP_C 8
+#define SPMSPV_MLP_NPARAM (64 * 16 + 64 + 64 * 64 + 64 + 8 * 64 + 8)
+#define SPMSPV_MLP_NWEIGHTS (SPMSPV_MLP_NPARAM + 2 * SPMSPV_MLP_F)
+int spmspv_selector_set_weights(spmspv_handle_t h, const float* w, int count);
+/* mode: 0 = MLP (needs weights), 1 = decision-tree fallback, 2 = fixed slot
+ * `fixed_kernel`. */
+int spmspv_selector_mode(spmspv_handle_t h, int mode, int fixed_kernel);
+/* Features + MLP inference in one device launch; returns the chosen slot. */
+int spmspv_select(spmspv_handle_t h, int* kernel_id);
+/* spmspv_select then spmspv_run of the chosen slot. */
+int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_key,
+                    float* d_y_val, int* y_nnz, int* kernel_used);
+
+/* Training on the GPU with the library's own forward/backward kernels.
+ * X[B*16] raw features (host), targets[B*8] soft labels (host; rows sum to
+ * 1). Trains in place on params[NPARAM] (host in/out) with Adam for
+ * `epochs` passes of minibatches of `batch`; returns the final mean loss in
+ * *loss. The normaliser (means, inverse stds) is computed from X and
+ * appended, so the output array holds SPMSPV_MLP_NWEIGHTS floats. */
+int spmspv_mlp_train(const float* X, const float* targets, int B, int epochs, int batch,
+                     float lr, unsigned seed, float* weights_out, float* loss);
+/* One forward+backward on the GPU for the gradient check: X[B*16] already
+ * normalised, targets[B*8]; writes loss and grad[NPARAM]. */
+int spmspv_mlp_loss_grad(const float* params, const float* X, const float* targets, int B,
+                         float* loss, float* grad);
+/* Inference on the GPU: logits[B*8] for normalised inputs X[B*16]. */
+int spmspv_mlp_forward(const float* params, const float* X, int B, float* logits);
+
+/* ---- BFS driver (the reference only replays recorded frontiers,
+ * la/test/test_application.cu:207-261; this is the loop it lacked) ------- */
+/* Level-synchronous BFS over the CSC pattern from `src`. d_levels[n]
+ * receives int32 levels (-1 unreached). policy: -1 = learned per-level
+ * selection, 0..7 = fixed slot. Levels are captured as hipGraphs when
+ * use_graphs != 0. stats (nullable) gets {levels, edges_traversed,
+ * seconds, reached}. */
+int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
+               double stats[4]);
+
+/* ---- batched multi-vector SpMSpV (no reference counterpart) ------------- */
+/* nvec sparse vectors in CSR-of-vectors form: d_vptr[nvec+1], d_vkey,
+ * d_vval; results in d_yptr[nvec+1] (capacity-based offsets given by the
+ * caller), d_ykey, d_yval, d_ynnz[nvec]. One persistent work-queue launch. */
+int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int* d_vkey,
+                       const float* d_vval, const long long* d_yoff, int* d_ykey, float* d_yval,
+                       int* d_ynnz);
+
+/* ---- utilities (the reference harness called the CUDA runtime directly) - */
+int spmspv_device_count(int* count);
+int spmspv_malloc(void** d_ptr, size_t bytes);
+int spmspv_free(void* d_ptr);
+int spmspv_memcpy_h2d(void* d_dst, const void* src, size_t bytes);
+int spmspv_memcpy_d2h(void* dst, const void* d_src, size_t bytes);
+int spmspv_memset(void* d_ptr, int value, size_t bytes);
+int spmspv_synchronize(void);
+/* Stream timing: events recorded on the handle's stream. */
+int spmspv_timer_start(spmspv_handle_t h);
+int spmspv_timer_stop(spmspv_handle_t h, float* ms);
+/* Writes `bytes` through a scratch buffer to evict the 256 MB MALL. */
+int spmspv_flush_cache(spmspv_handle_t h, size_t bytes);
+
+/* ---- synthetic inputs (host; the reference read .mtx files) ------------- */
+/* Square n x n CSC with exactly k distinct rows per column (Floyd sampling,
+ * rows ascending), values U[-1,1]. Arrays: col_ptr[n+1], row[n*k], val[n*k]. */
+int spmspv_gen_uniform_csc(int n, int k, uint64_t seed, int* col_ptr, int* row, float* val);
+/* R-MAT (Graph500 quadrant recursion) with 2^scale vertices and
+ * edge_factor*2^scale generated edges, symmetrised, deduplicated, self-loops
+ * dropped; values U(0,1] equal on (i,j) and (j,i). Writes CSR (== CSC by
+ * symmetry). Call with row_ptr == NULL to get the exact nnz in *nnz first. */
+int spmspv_gen_rmat_csr(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                        long long* nnz, int* row_ptr, int* col, float* val);
+/* k distinct sorted keys in [0,n) and values U[lo,hi] (for x). */
+int spmspv_gen_sparse_vector(int n, int k, uint64_t seed, float lo, float hi, int* key,
+                             float* val);
+/* Banded n x n CSR with half-bandwidth bw, values U(0,1]. Call with
+ * row_ptr == NULL to get nnz. */
+int spmspv_gen_banded_csr(int n, int bw, uint64_t seed, long long* nnz, int* row_ptr, int* col,
+                          float* val);
+/* Block-diagonal n x n CSR with dense bs x bs blocks, values U(0,1]. */
+int spmspv_gen_blockdiag_csr(int n, int bs, uint64_t seed, long long* nnz, int* row_ptr, int* col,
+                             float* val);
+/* Uniform random CSR with mean k per row (each row: Binomial-like count,
+ * distinct sorted columns). Call with row_ptr == NULL to get nnz. */
+int spmspv_gen_random_csr(int m, int n, double mean_per_row, uint64_t seed, long long* nnz,
+                          int* row_ptr, int* col, float* val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPMSPV_AMD_H_ */

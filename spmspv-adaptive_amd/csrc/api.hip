@@ -1,0 +1,889 @@
+// C-ABI of the engine: lifecycle, matrix and x input, conversions,
+// features and the eight kernel slots. Every entry point cites the
+// reference method it replaces in include/spmspv_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "handle.hpp"
+#include "selector.hpp"
+
+using namespace spmspv;
+
+namespace {
+thread_local std::string g_err;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    off = align_up(off, 256);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += sizeof(T) * std::max<size_t>(count, 1);
+    return p;
+  }
+};
+
+int pow2_team(double avg) {
+  int t = 1;
+  while (t < avg && t < 64) t <<= 1;
+  return t;
+}
+}  // namespace
+
+namespace spmspv {
+
+int set_error(spmspv_handle_t h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(h, expr)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(h, SPMSPV_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static int check_launch(spmspv_handle_t h, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(h, SPMSPV_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+  return SPMSPV_SUCCESS;
+}
+
+// Lay out (or measure, when base == nullptr) every buffer of the arena.
+static size_t carve(spmspv_handle_t h, char* base) {
+  Carver c{base};
+  const size_t m = h->m, n = h->n, nnz = h->nnz;
+  const size_t lb_tiles = (n + nnz + kLbTile - 1) / kLbTile + 2;
+  const size_t prep_tiles = (n + kPrepTile - 1) / kPrepTile + 2;
+  const size_t mwords = (m + 31) / 32;
+  const size_t nwords = (n + 31) / 32;
+  const size_t cmp_tiles = (std::max(mwords, nwords) + kCmpTile - 1) / kCmpTile + 2;
+  const size_t sort_tiles = (nnz + kSortTile - 1) / kSortTile + 2;
+  const size_t red_tiles = (nnz + kRedTile - 1) / kRedTile + 2;
+  const size_t row_tiles = (m + nnz + kRowTile - 1) / kRowTile + 2;
+  ;
+  Ctl* ctl = c.take<Ctl>(1);
+  MatFeat* mat = c.take<MatFeat>(1);
+  double* mp = c.take<double>(256 * 5);
+  int* ptr = c.take<int>(n + 1);
+  int* xbeg = c.take<int>(n);
+  int* split = c.take<int>(lb_tiles);
+  int* tmm = c.take<int>(2 * prep_tiles);
+  unsigned long long* ss = c.take<unsigned long long>(std::max(prep_tiles, cmp_tiles));
+  float* acc = c.take<float>(m);
+  unsigned* bm = c.take<unsigned>(mwords + 2);
+  SortBufs sb;
+  sb.keyA = c.take<unsigned>(nnz);
+  sb.valA = c.take<float>(nnz);
+  sb.keyB = c.take<unsigned>(nnz);
+  sb.valB = c.take<float>(nnz);
+  sb.hist = c.take<unsigned>(4 * 256);
+  sb.digoff = c.take<unsigned>(4 * 256);
+  sb.state = c.take<unsigned long long>(sort_tiles * 256);
+  sb.red_state = c.take<unsigned long long>(red_tiles);
+  sb.carry_val = c.take<float>(red_tiles);
+  sb.carry_idx = c.take<int>(red_tiles);
+  int* ykt = c.take<int>(m);
+  float* yvt = c.take<float>(m);
+  float* xdi = c.take<float>(n);
+  unsigned* bi = c.take<unsigned>(nwords + 2);
+  int* xki = c.take<int>(n);
+  float* xvi = c.take<float>(n);
+  int* rsplit = c.take<int>(row_tiles + 1);
+  float* rcv = c.take<float>(row_tiles);
+  int* rcr = c.take<int>(row_tiles);
+  int* rch = c.take<int>(row_tiles);
+  long long* ndev = c.take<long long>(4);
+  float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
+  if (base) {
+    h->ctl = ctl;
+    h->mat = mat;
+    h->mat_partials = mp;
+    h->ptr = ptr;
+    h->xbeg = xbeg;
+    h->split = split;
+    h->tile_mm = tmm;
+    h->scan_state = ss;
+    h->acc = acc;
+    h->bitmap = bm;
+    h->sb = sb;
+    h->yk_tmp = ykt;
+    h->yv_tmp = yvt;
+    h->x_dense_int = xdi;
+    h->bits_int = bi;
+    h->xk_int = xki;
+    h->xv_int = xvi;
+    h->row_split = rsplit;
+    h->row_carry_val = rcv;
+    h->row_carry_row = rcr;
+    h->row_carry_hit = rch;
+    h->n_dev = ndev;
+    h->weights_dev = w;
+  }
+  return c.off;
+}
+
+int ensure_arena(spmspv_handle_t h) {
+  if (h->reserved) return SPMSPV_SUCCESS;
+  const size_t bytes = carve(h, nullptr);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return set_error(h, SPMSPV_OUT_OF_MEMORY,
+                     "arena of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+  h->arena = p;
+  h->arena_bytes = bytes;
+  carve(h, static_cast<char*>(p));
+  // Everything starts at zero: control block, look-back states, the
+  // accumulator and bitmaps whose all-zero invariant the kernels keep.
+  HIP_TRY(h, hipMemsetAsync(p, 0, bytes, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->reserved = true;
+  return SPMSPV_SUCCESS;
+}
+
+int ensure_row_tiles(spmspv_handle_t h) {
+  if (h->row_ready) return SPMSPV_SUCCESS;
+  if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "row kernels need CSR input");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  h->n_row_tiles = (int)(((long long)h->m + h->nnz + kRowTile - 1) / kRowTile);
+  launch_row_split(h->stream, h->csr_row, h->m, h->nnz, h->row_split, h->n_row_tiles);
+  rc = check_launch(h, "row split");
+  if (rc) return rc;
+  h->row_ready = true;
+  return SPMSPV_SUCCESS;
+}
+
+int ensure_mat_features(spmspv_handle_t h) {
+  if (h->mat_ready) return SPMSPV_SUCCESS;
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  // Row degrees come from CSR; with CSC-only input the column pointer is
+  // used (identical for the symmetric matrices of the benchmarks).
+  const int* rp = h->csr_row ? h->csr_row : h->csc_colptr;
+  const int rows = h->csr_row ? h->m : h->n;
+  const int cols = h->csr_row ? h->n : h->m;
+  if (!rp) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "no matrix input");
+  // degree histogram needs max degree + 1 <= cols + 1 slots
+  int* hist = cols <= h->n ? h->ptr : h->yk_tmp;
+  launch_mat_features(h->stream, rp, rows, cols, h->nnz, hist, h->mat_partials, h->mat, h->ctl);
+  rc = check_launch(h, "mat features");
+  if (rc) return rc;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->mat_ready = true;
+  return SPMSPV_SUCCESS;
+}
+
+static int need_csc(spmspv_handle_t h) {
+  if (!h->csc_colptr) {
+    if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "no matrix input");
+    int rc = spmspv_to_csc(h);
+    if (rc) return rc;
+  }
+  return SPMSPV_SUCCESS;
+}
+
+// Makes x available as sparse keys/values (from dense when needed).
+static int need_sparse_x(spmspv_handle_t h) {
+  if (h->sparse_valid) return SPMSPV_SUCCESS;
+  if (h->vec_type == SPMSPV_VEC_DENSE || h->vec_type == SPMSPV_VEC_BITARRAY)
+    return spmspv_dense2sparse(h);
+  return set_error(h, SPMSPV_VEC_FORMAT_ERR, "x is not set");
+}
+
+static int need_dense_x(spmspv_handle_t h) {
+  if (h->dense_valid) return SPMSPV_SUCCESS;
+  if (!h->sparse_valid) return set_error(h, SPMSPV_VEC_FORMAT_ERR, "x is not set");
+  return spmspv_sparse2dense(h);
+}
+
+static int need_bits(spmspv_handle_t h) {
+  if (h->bits_valid) return SPMSPV_SUCCESS;
+  if (h->sparse_valid) {
+    unsigned* b = h->bits();
+    launch_fill_i32(h->stream, reinterpret_cast<int*>(b), (h->n + 31) / 32, 0);
+    launch_set_bits(h->stream, h->x_key, h->xnnz, nullptr, h->n, b);
+    h->bits_valid = true;
+    return check_launch(h, "set bits");
+  }
+  if (h->dense_valid) return spmspv_dense2bitarray(h);
+  return set_error(h, SPMSPV_VEC_FORMAT_ERR, "x is not set");
+}
+
+int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel) {
+  launch_prep(h->stream, h->csc_colptr, h->x_key, h->n, h->xnnz, nullptr, h->ptr, h->xbeg,
+              with_split ? h->split : nullptr, h->scan_state, h->tile_mm, h->ctl, sel, 0);
+  h->launches++;
+  int rc = check_launch(h, "prep");
+  if (rc) return rc;
+  if (with_split) h->prep_valid = true;
+  return SPMSPV_SUCCESS;
+}
+
+static long long lb_tiles_hint(spmspv_handle_t h) {
+  // tiles of the merged (x entries + products) sequence; bin_len <= nnz
+  const long long bl = h->binlen_host >= 0 ? h->binlen_host : -1;
+  if (bl < 0) return -1;
+  return (h->xnnz + bl + kLbTile - 1) / kLbTile;
+}
+
+int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val,
+               bool want_nnz) {
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  hipStream_t s = h->stream;
+  const bool want_sparse = y_key || y_val || want_nnz;
+  const int m = h->m;
+  switch (kid) {
+    case SPMSPV_K_COL_NAIVE_ATOMIC:
+    case SPMSPV_K_COL_LB_ATOMIC: {
+      if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+      float* acc = y_dense ? y_dense : h->acc;
+      if (y_dense) launch_fill_f32(s, y_dense, m, 0.f);
+      if (kid == SPMSPV_K_COL_NAIVE_ATOMIC) {
+        launch_naive_col(s, 0, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, nullptr,
+                         h->n, h->xnnz, nullptr, h->team_col, acc, h->bitmap, nullptr, nullptr);
+      } else {
+        if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
+        launch_lb_gather(s, 0, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
+                         h->xnnz, nullptr, acc, h->bitmap, nullptr, nullptr, lb_tiles_hint(h));
+      }
+      launch_compact(s, h->bitmap, m, acc, y_dense ? 0 : 1, y_key, y_val, h->scan_state, h->ctl);
+      h->launches += 2;
+      return check_launch(h, "column atomic kernel");
+    }
+    case SPMSPV_K_COL_NAIVE_SORT:
+    case SPMSPV_K_COL_LB_SORT: {
+      if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+      const bool lb = kid == SPMSPV_K_COL_LB_SORT;
+      if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
+      int* bk = reinterpret_cast<int*>(h->sb.keyA);
+      float* bv = h->sb.valA;
+      if (lb)
+        launch_lb_gather(s, 1, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
+                         h->xnnz, nullptr, nullptr, nullptr, bk, bv, lb_tiles_hint(h));
+      else
+        launch_naive_col(s, 1, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, h->ptr,
+                         h->n, h->xnnz, nullptr, h->team_col, nullptr, nullptr, bk, bv);
+      int* ok = y_key ? y_key : h->yk_tmp;
+      float* ov = y_val ? y_val : h->yv_tmp;
+      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, h->binlen_host);
+      h->launches += 6;
+      if (y_dense) {
+        launch_fill_f32(s, y_dense, m, 0.f);
+        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
+      }
+      return check_launch(h, "column sort kernel");
+    }
+    case SPMSPV_K_ROW_NAIVE:
+    case SPMSPV_K_ROW_LB:
+    case SPMSPV_K_SPMV_NAIVE:
+    case SPMSPV_K_SPMV_LB: {
+      if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "row kernels need CSR input");
+      const bool masked = kid == SPMSPV_K_ROW_NAIVE || kid == SPMSPV_K_ROW_LB;
+      const bool lb = kid == SPMSPV_K_ROW_LB || kid == SPMSPV_K_SPMV_LB;
+      float* yout = y_dense ? y_dense : h->acc;
+      // writing into the internal accumulator always tracks hits so that
+      // the compaction can restore its all-zero state
+      const bool hits = want_sparse || !y_dense;
+      if ((rc = need_dense_x(h))) return rc;
+      if ((masked || hits) && (rc = need_bits(h))) return rc;
+      if (lb) {
+        if ((rc = ensure_row_tiles(h))) return rc;
+        launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
+                         m, h->nnz, h->row_split, h->n_row_tiles, yout, h->bitmap, h->row_carry_val,
+                         h->row_carry_row, h->row_carry_hit);
+      } else {
+        launch_row_naive(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
+                         m, h->team_row, yout, h->bitmap);
+      }
+      if (hits)
+        launch_compact(s, h->bitmap, m, yout, yout == h->acc ? 1 : 0, y_key, y_val, h->scan_state,
+                       h->ctl);
+      h->launches += 2;
+      return check_launch(h, "row kernel");
+    }
+    default:
+      return set_error(h, SPMSPV_INVALID_ARGUMENT, "kernel id " + std::to_string(kid));
+  }
+}
+
+int readback_ynnz(spmspv_handle_t h, int* y_nnz) {
+  HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->ynnz, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  *y_nnz = h->h_pin[0];
+  return SPMSPV_SUCCESS;
+}
+
+}  // namespace spmspv
+
+// =================================================================== C-ABI
+extern "C" {
+
+const char* spmspv_last_error(spmspv_handle_t h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+int spmspv_create(spmspv_handle_t* out, int m, int n, int nnz) {
+  if (!out) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle pointer");
+  if (m <= 0 || n <= 0 || nnz < 0)
+    return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "bad dimensions");
+  auto* h = new spmspv_handle_s();
+  h->m = m;
+  h->n = n;
+  h->nnz = nnz;
+  h->team_col = std::min(64, std::max(4, pow2_team((double)nnz / n)));
+  h->team_row = std::min(64, std::max(2, pow2_team((double)nnz / m)));
+  hipError_t e = hipHostMalloc((void**)&h->h_pin, 64 * sizeof(int), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete h;
+    return set_error(nullptr, SPMSPV_DEVICE_ERROR, std::string("pinned buffer: ") + hipGetErrorString(e));
+  }
+  hipEventCreate(&h->ev0);
+  hipEventCreate(&h->ev1);
+  *out = h;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_bfs_release(spmspv_handle_t h);
+
+int spmspv_destroy(spmspv_handle_t h) {
+  if (!h) return SPMSPV_SUCCESS;
+  hipStreamSynchronize(h->stream);
+  spmspv_bfs_release(h);
+  if (h->arena) hipFree(h->arena);
+  if (h->csc_alloc) hipFree(h->csc_alloc);
+  if (h->flush_buf) hipFree(h->flush_buf);
+  if (h->h_pin) hipHostFree(h->h_pin);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  delete h;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_stream(spmspv_handle_t h, void* stream) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->stream = static_cast<hipStream_t>(stream);
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_warmup(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  HIP_TRY(h, hipFree(nullptr));
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* val) {
+  if (!h || !row_ptr || (nnz > 0 && (!col || !val)))
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSR: null argument");
+  if (nnz != h->nnz) return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSR: nnz differs from handle");
+  h->csr_row = row_ptr;
+  h->csr_col = col;
+  h->csr_val = val;
+  h->row_ready = false;
+  h->mat_ready = false;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_input_csc(spmspv_handle_t h, int nnz, int* row, int* col_ptr, float* val) {
+  if (!h || !col_ptr || (nnz > 0 && (!row || !val)))
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSC: null argument");
+  if (nnz != h->nnz) return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSC: nnz differs from handle");
+  if (h->csc_alloc) {
+    hipFree(h->csc_alloc);
+    h->csc_alloc = nullptr;
+  }
+  h->csc_row = row;
+  h->csc_colptr = col_ptr;
+  h->csc_val = val;
+  h->csc_owned = false;
+  h->mat_ready = false;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_to_csc(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "ToCSC needs CSR input");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  if (!h->csc_alloc) {
+    const size_t a = align_up(sizeof(int) * (size_t)(h->n + 1), 256);
+    const size_t b = align_up(sizeof(int) * (size_t)std::max(h->nnz, 1), 256);
+    const size_t c = sizeof(float) * (size_t)std::max(h->nnz, 1);
+    HIP_TRY(h, hipMalloc(&h->csc_alloc, a + b + c));
+    char* p = static_cast<char*>(h->csc_alloc);
+    h->csc_colptr = reinterpret_cast<int*>(p);
+    h->csc_row = reinterpret_cast<int*>(p + a);
+    h->csc_val = reinterpret_cast<float*>(p + a + b);
+    h->csc_owned = true;
+  }
+  launch_csr_to_csc_sorted(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->n, h->nnz,
+                           h->csc_colptr, h->csc_row, h->csc_val, h->sb, h->n_dev, h->ctl);
+  h->launches += 6;
+  rc = check_launch(h, "ToCSC");
+  if (rc) return rc;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  if (h->csr_row && (rc = ensure_row_tiles(h))) return rc;
+  if ((h->csr_row || h->csc_colptr) && (rc = ensure_mat_features(h))) return rc;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_dealloc_pre_buffer(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  hipStreamSynchronize(h->stream);
+  if (h->arena) hipFree(h->arena);
+  h->arena = nullptr;
+  h->reserved = false;
+  h->row_ready = false;
+  h->mat_ready = false;
+  h->prep_valid = false;
+  h->have_weights = false;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_hola_preprocess(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!h->csr_row) return set_error(h, SPMSPV_FORMAT_ERR_HOLA_SPMV, "holaPreprocess needs CSR");
+  int rc = ensure_row_tiles(h);
+  if (rc) return rc;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return SPMSPV_SUCCESS;
+}
+int spmspv_hola_postprocess(spmspv_handle_t h) { return h ? SPMSPV_SUCCESS : SPMSPV_INVALID_ARGUMENT; }
+int spmspv_csr5_preprocess(spmspv_handle_t h) { return spmspv_hola_preprocess(h); }
+int spmspv_csr5_postprocess(spmspv_handle_t h) { return h ? SPMSPV_SUCCESS : SPMSPV_INVALID_ARGUMENT; }
+int spmspv_set_sigma(spmspv_handle_t h, int sigma) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->sigma = sigma;
+  return SPMSPV_SUCCESS;
+}
+int spmspv_set_spmv_type(spmspv_handle_t h, int code) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (code != 0 && code != 1) return set_error(h, SPMSPV_INVALID_ARGUMENT, "spmv type must be 0 or 1");
+  h->spmv_type = code;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_vector_type(spmspv_handle_t h, int code) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (code < 0 || code > 2) return set_error(h, SPMSPV_INVALID_ARGUMENT, "vector type must be 0..2");
+  h->vec_type = code;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_sparse_x(spmspv_handle_t h, int* key, float* val, int xnnz) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (xnnz < 0 || xnnz > h->n) return set_error(h, SPMSPV_INVALID_ARGUMENT, "xnnz out of range");
+  if (xnnz > 0 && (!key || !val)) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null x arrays");
+  h->x_key = key;
+  h->x_val = val;
+  h->xnnz = xnnz;
+  h->vec_type = SPMSPV_VEC_SPARSE;
+  h->sparse_valid = true;
+  h->dense_valid = false;
+  h->bits_valid = false;
+  h->prep_valid = false;
+  h->binlen_host = -1;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_x(spmspv_handle_t h, float* x) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->x_dense_user = x;
+  // a dense x given on its own becomes the current x (vector type 1)
+  if (h->vec_type == SPMSPV_VEC_DENSE) {
+    h->dense_valid = x != nullptr;
+    h->sparse_valid = false;
+    h->bits_valid = false;
+    h->prep_valid = false;
+    h->binlen_host = -1;
+  } else {
+    h->dense_valid = false;
+  }
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_bitvector(spmspv_handle_t h, int* bits, int num_ints) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (bits && num_ints < (h->n + 31) / 32)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "bit array shorter than ceil(n/32) words");
+  h->bits_user = reinterpret_cast<unsigned*>(bits);
+  h->num_ints_user = num_ints;
+  h->bits_valid = false;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_sparse_x_dense_x(spmspv_handle_t h, int* key, float* val, float* x, int xnnz) {
+  int rc = spmspv_set_sparse_x(h, key, val, xnnz);
+  if (rc) return rc;
+  h->x_dense_user = x;
+  h->dense_valid = false;  // the caller's dense copy is filled on demand
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_sparse2dense(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!h->sparse_valid) return set_error(h, SPMSPV_VEC_FORMAT_ERR, "sparse2dense: x is not sparse");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  float* d = h->dense_x();
+  launch_fill_f32(h->stream, d, h->n, 0.f);
+  launch_scatter(h->stream, h->x_key, h->x_val, h->xnnz, nullptr, h->n, d);
+  h->dense_valid = true;
+  return check_launch(h, "sparse2dense");
+}
+
+int spmspv_sparse2bitarray(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!h->sparse_valid) return set_error(h, SPMSPV_VEC_FORMAT_ERR, "sparse2bitarray: x is not sparse");
+  int rc = spmspv_sparse2dense(h);
+  if (rc) return rc;
+  h->bits_valid = false;
+  rc = need_bits(h);
+  if (rc) return rc;
+  h->vec_type = SPMSPV_VEC_BITARRAY;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_dense2bitarray(spmspv_handle_t h) {
+  // bits where the dense x is nonzero (the reference's dense2bitarray_ker
+  // tested `!val`, spmspv.h:1354-1369, i.e. marked the zeros)
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!h->dense_valid) return set_error(h, SPMSPV_VEC_FORMAT_ERR, "dense2bitarray: no dense x");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  launch_bitmap_from_dense(h->stream, h->dense_x(), h->n, h->bits());
+  h->bits_valid = true;
+  h->vec_type = SPMSPV_VEC_BITARRAY;
+  return check_launch(h, "dense2bitarray");
+}
+
+int spmspv_dense2sparse(spmspv_handle_t h) {
+  // keeps entries != 0 (spmspv.h:1176-1223); writes the handle's own key/val
+  // buffers unless the caller set sparse buffers with set_sparse_x
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!h->dense_valid) return set_error(h, SPMSPV_VEC_FORMAT_ERR, "dense2sparse: no dense x");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  int* k = h->x_key ? h->x_key : h->xk_int;
+  float* v = h->x_val ? h->x_val : h->xv_int;
+  unsigned* b = h->bits();
+  launch_bitmap_from_dense(h->stream, h->dense_x(), h->n, b);
+  launch_compact(h->stream, b, h->n, h->dense_x(), 0, k, v, h->scan_state, h->ctl);
+  h->launches++;
+  rc = check_launch(h, "dense2sparse");
+  if (rc) return rc;
+  // compaction cleared the bit array; read the count back
+  int cnt = 0;
+  rc = readback_ynnz(h, &cnt);
+  if (rc) return rc;
+  h->x_key = k;
+  h->x_val = v;
+  h->xnnz = cnt;
+  h->sparse_valid = true;
+  h->bits_valid = false;
+  h->prep_valid = false;
+  h->binlen_host = -1;
+  h->vec_type = SPMSPV_VEC_SPARSE;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_bitarray2sparse(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (h->vec_type != SPMSPV_VEC_BITARRAY)
+    return set_error(h, SPMSPV_VEC_FORMAT_ERR, "bitarray2sparse: input is not a bit array");
+  return spmspv_dense2sparse(h);
+}
+
+int spmspv_dense2sparse_outer(spmspv_handle_t h, int m, float* d_dense, int* ynnz, int* d_key,
+                              float* d_val) {
+  if (!h || !d_dense || m <= 0 || m > std::max(h->m, h->n))
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "dense2sparse_outer: bad arguments");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  // use the m-sized bitmap region when it fits, else the n-sized one
+  unsigned* b = m <= h->m ? h->bitmap : h->bits_int;
+  launch_bitmap_from_dense(h->stream, d_dense, m, b);
+  launch_compact(h->stream, b, m, d_dense, 0, d_key, d_val, h->scan_state, h->ctl);
+  h->launches++;
+  rc = check_launch(h, "dense2sparse_outer");
+  if (rc) return rc;
+  if (ynnz) return readback_ynnz(h, ynnz);
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_sparse2dense_outer(spmspv_handle_t h, int ynnz, int* d_key, float* d_val, int m,
+                              float* d_dense) {
+  if (!h || !d_dense || m <= 0 || ynnz < 0 || ynnz > m || (ynnz > 0 && (!d_key || !d_val)))
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "sparse2dense_outer: bad arguments");
+  launch_fill_f32(h->stream, d_dense, m, 0.f);
+  launch_scatter(h->stream, d_key, d_val, ynnz, nullptr, m, d_dense);
+  return check_launch(h, "sparse2dense_outer");
+}
+
+// ------------------------------------------------------------- features
+int spmspv_vec_features(spmspv_handle_t h, long long* bin_len, int* max_len, int* min_len) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  int rc;
+  if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+  if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
+  HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->bin_len, 16, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  long long bl;
+  std::memcpy(&bl, h->h_pin, sizeof(bl));
+  h->binlen_host = bl;
+  if (bin_len) *bin_len = bl;
+  if (max_len) *max_len = h->h_pin[2];
+  if (min_len) *min_len = h->h_pin[3];
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_compute_binlen(spmspv_handle_t h, long long* bin_len) {
+  return spmspv_vec_features(h, bin_len, nullptr, nullptr);
+}
+
+int spmspv_mat_features(spmspv_handle_t h, float out[7]) {
+  if (!h || !out) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  int rc = ensure_mat_features(h);
+  if (rc) return rc;
+  MatFeat mf;
+  HIP_TRY(h, hipMemcpy(&mf, h->mat, sizeof(mf), hipMemcpyDeviceToHost));
+  out[0] = (float)mf.max_row;
+  out[1] = (float)mf.min_row;
+  out[2] = (float)mf.avg_row;
+  out[3] = (float)mf.range;
+  out[4] = (float)mf.std_row;
+  out[5] = (float)mf.equality;
+  out[6] = (float)mf.gini;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_extract_mat_feature(spmspv_handle_t h, float* std_row) {
+  if (!h || !std_row) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  int rc = ensure_mat_features(h);
+  if (rc) return rc;
+  MatFeat mf;
+  HIP_TRY(h, hipMemcpy(&mf, h->mat, sizeof(mf), hipMemcpyDeviceToHost));
+  *std_row = (float)mf.std_trunc;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_compute_gm(spmspv_handle_t h, double out[3]) {
+  if (!h || !out) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  long long bl = h->binlen_host;
+  if (bl < 0) {
+    int rc = spmspv_compute_binlen(h, &bl);
+    if (rc) return rc;
+  }
+  const double m = h->m, nnz = h->nnz, x = h->xnnz, b = (double)bl;
+  out[0] = 4.0 * (3.0 * x + 3.0 * b);
+  out[1] = 4.0 * ((m + 1) + 2.0 * b + nnz + m);
+  out[2] = 4.0 * ((m + 1) + 3.0 * nnz + m);
+  return SPMSPV_SUCCESS;
+}
+
+// ------------------------------------------------------------- kernels
+int spmspv_run(spmspv_handle_t h, int kid, float alpha, float* d_y_dense, int* d_y_key,
+               float* d_y_val, int* y_nnz) {
+  (void)alpha;
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  int rc = run_kernel(h, kid, d_y_dense, d_y_key, d_y_val, y_nnz != nullptr);
+  if (rc) return rc;
+  if (y_nnz) return readback_ynnz(h, y_nnz);
+  return SPMSPV_SUCCESS;
+}
+
+int* spmspv_ynnz_device(spmspv_handle_t h) {
+  if (!h || !h->reserved) return nullptr;
+  return &h->ctl->ynnz;
+}
+
+int spmspv_csc_nosort_naive(spmspv_handle_t h, float alpha, int* ynnz, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_COL_NAIVE_ATOMIC, alpha, d_y, nullptr, nullptr, ynnz);
+}
+int spmspv_csc_nosort_naive_keyval(spmspv_handle_t h, float alpha, int* ynnz, float* d_y, int* k,
+                                   float* v) {
+  return spmspv_run(h, SPMSPV_K_COL_NAIVE_ATOMIC, alpha, d_y, k, v, ynnz);
+}
+int spmspv_csc_sort_naive(spmspv_handle_t h, float alpha, int* ynnz, int* k, float* v, int method) {
+  (void)method;
+  return spmspv_run(h, SPMSPV_K_COL_NAIVE_SORT, alpha, nullptr, k, v, ynnz);
+}
+int spmspv_csc_nosort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
+                            float* d_y, int* k, float* v) {
+  (void)is_binlen_computed;
+  return spmspv_run(h, SPMSPV_K_COL_LB_ATOMIC, alpha, d_y, k, v, ynnz);
+}
+int spmspv_csc_nosort_merge_keyval(spmspv_handle_t h, int is_binlen_computed, float alpha,
+                                   int* ynnz, float* d_y, int* k, float* v) {
+  (void)is_binlen_computed;
+  return spmspv_run(h, SPMSPV_K_COL_LB_ATOMIC, alpha, d_y, k, v, ynnz);
+}
+int spmspv_csc_sort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
+                          float* d_y, int* k, float* v) {
+  (void)is_binlen_computed;
+  return spmspv_run(h, SPMSPV_K_COL_LB_SORT, alpha, d_y, k, v, ynnz);
+}
+int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* k, float* v, int buckets) {
+  (void)buckets;
+  return spmspv_run(h, SPMSPV_K_COL_BUCKET, alpha, nullptr, k, v, ynnz);
+}
+int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_ROW_NAIVE, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_holaspmspv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_ROW_LB, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_csr5spmspv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_ROW_LB, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_spmspv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_ROW_LB, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_naivespmv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_SPMV_NAIVE, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_holaspmv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_SPMV_LB, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_csr5spmv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_SPMV_LB, alpha, d_y, nullptr, nullptr, nullptr);
+}
+int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y) {
+  return spmspv_run(h, SPMSPV_K_SPMV_LB, alpha, d_y, nullptr, nullptr, nullptr);
+}
+
+// ------------------------------------------------------------- selector
+int spmspv_selector_set_weights(spmspv_handle_t h, const float* w, int count) {
+  if (!h || !w || count != SPMSPV_MLP_NWEIGHTS)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "selector weights: expected SPMSPV_MLP_NWEIGHTS floats");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpy(h->weights_dev, w, sizeof(float) * count, hipMemcpyHostToDevice));
+  h->have_weights = true;
+  h->sel_mode = 0;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_selector_mode(spmspv_handle_t h, int mode, int fixed_kernel) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (mode < 0 || mode > 2 || fixed_kernel < 0 || fixed_kernel >= SPMSPV_NUM_SLOTS)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "selector mode");
+  if (mode == 0 && !h->have_weights)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "selector mode 0 needs weights");
+  h->sel_mode = mode;
+  h->sel_fixed = fixed_kernel;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_select(spmspv_handle_t h, int* kernel_id) {
+  if (!h || !kernel_id) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  int rc;
+  if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = ensure_mat_features(h)) ||
+      (rc = need_sparse_x(h)))
+    return rc;
+  SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed};
+  if ((rc = run_prep(h, true, &sel))) return rc;
+  HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->bin_len, 24, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(h->h_pin + 8, &h->ctl->kernel_choice, 4, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  long long bl;
+  std::memcpy(&bl, h->h_pin, sizeof(bl));
+  h->binlen_host = bl;
+  *kernel_id = h->h_pin[8];
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]) {
+  if (!h || !out) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  int kid;
+  int rc = spmspv_select(h, &kid);
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpy(out, h->ctl->feat, sizeof(float) * SPMSPV_NUM_FEATURES, hipMemcpyDeviceToHost));
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_key, float* d_y_val,
+                    int* y_nnz, int* kernel_used) {
+  int kid = 0;
+  int rc = spmspv_select(h, &kid);
+  if (rc) return rc;
+  if (kernel_used) *kernel_used = kid;
+  return spmspv_run(h, kid, alpha, d_y_dense, d_y_key, d_y_val, y_nnz);
+}
+
+// ------------------------------------------------------------- utilities
+int spmspv_device_count(int* count) {
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+  }
+  return SPMSPV_SUCCESS;
+}
+int spmspv_malloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_OUT_OF_MEMORY, hipGetErrorString(e));
+}
+int spmspv_free(void* p) {
+  hipError_t e = hipFree(p);
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+}
+int spmspv_memcpy_h2d(void* d, const void* s, size_t bytes) {
+  hipError_t e = hipMemcpy(d, s, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+}
+int spmspv_memcpy_d2h(void* d, const void* s, size_t bytes) {
+  hipError_t e = hipMemcpy(d, s, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+}
+int spmspv_memset(void* d, int v, size_t bytes) {
+  hipError_t e = hipMemset(d, v, bytes);
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+}
+int spmspv_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+}
+int spmspv_timer_start(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
+  return SPMSPV_SUCCESS;
+}
+int spmspv_timer_stop(spmspv_handle_t h, float* ms) {
+  if (!h || !ms) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(h, hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(h, hipEventSynchronize(h->ev1));
+  HIP_TRY(h, hipEventElapsedTime(ms, h->ev0, h->ev1));
+  return SPMSPV_SUCCESS;
+}
+int spmspv_flush_cache(spmspv_handle_t h, size_t bytes) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (h->flush_bytes < bytes) {
+    if (h->flush_buf) hipFree(h->flush_buf);
+    h->flush_buf = nullptr;
+    HIP_TRY(h, hipMalloc(&h->flush_buf, bytes));
+    h->flush_bytes = bytes;
+  }
+  launch_fill_f32(h->stream, h->flush_buf, (long long)(bytes / 4), 1.0f);
+  return check_launch(h, "flush");
+}
+
+}  // extern "C"

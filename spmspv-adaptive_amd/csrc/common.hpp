@@ -1,0 +1,257 @@
+// Shared device primitives for the gfx950 SpMSpV engine: wave64 scans and
+// reductions, the single-pass decoupled look-back tile prefix used by every
+// scan-like kernel, and the per-handle control block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spmspv {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- control
+// One per handle, in device memory. Persistent kernels take tiles from
+// `ticket[slot]`; the last workgroup to finish (counted in `done[slot]`)
+// resets both and advances `epoch`, so a kernel needs no memset before it
+// (which keeps every launch capturable in a hipGraph).
+constexpr int kSlots = 16;
+struct Ctl {
+  unsigned int ticket[kSlots];
+  unsigned int done[kSlots];
+  unsigned int epoch;
+  unsigned int pad0;
+  long long bin_len;   // sum of column lengths over x's support
+  int max_len;         // max column length over x's support
+  int min_len;         // min column length over x's support
+  int xnnz;            // current x support size (device copy)
+  int ynnz;            // support size of the last result
+  int sort_small;      // 1 when the last sort took the single-workgroup path
+  int kernel_choice;   // last selector output
+  long long n_tiles;   // tiles of the last load-balanced gather
+  int frontier;        // BFS: size of the next frontier
+  int level;           // BFS: current level
+  float feat[16];      // last selector feature vector
+  int pad[6];
+};
+
+// Look-back status word: [63:62] flag (1 aggregate, 2 inclusive),
+// [61:38] epoch (24 bits), [37:0] value.
+constexpr int kEpochBits = 24;
+constexpr unsigned long long kValMask = (1ull << 38) - 1;
+__device__ __forceinline__ unsigned long long pack_state(unsigned flag, unsigned epoch,
+                                                         unsigned long long v) {
+  return ((unsigned long long)flag << 62) |
+         ((unsigned long long)(epoch & ((1u << kEpochBits) - 1)) << 38) | (v & kValMask);
+}
+
+__device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- wave ops
+__device__ __forceinline__ int lane() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_sum(T v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    T t = __shfl_up(v, o, kWave);
+    if (lane() >= o) v += t;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T t = __shfl_xor(v, o, kWave);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T t = __shfl_xor(v, o, kWave);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+
+// Block-wide exclusive sum of one value per thread; returns the exclusive
+// prefix and writes the block total to *total. `lds` needs NT/64 slots.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_sum(T v, T* lds, T* total) {
+  constexpr int NW = NT / kWave;
+  const int w = threadIdx.x / kWave;
+  T inc = wave_incl_sum(v);
+  if (lane() == kWave - 1) lds[w] = inc;
+  __syncthreads();
+  T wpre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    T s = lds[i];
+    if (i < w) wpre += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wpre + inc - v;
+}
+
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds) {
+  constexpr int NW = NT / kWave;
+  v = wave_sum(v);
+  if (lane() == 0) lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) tot += lds[i];
+  __syncthreads();
+  return tot;
+}
+
+template <int NT, typename T>
+__device__ __forceinline__ T block_max(T v, T* lds) {
+  constexpr int NW = NT / kWave;
+  v = wave_max(v);
+  if (lane() == 0) lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T r = lds[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) r = lds[i] > r ? lds[i] : r;
+  __syncthreads();
+  return r;
+}
+
+template <int NT, typename T>
+__device__ __forceinline__ T block_min(T v, T* lds) {
+  constexpr int NW = NT / kWave;
+  v = wave_min(v);
+  if (lane() == 0) lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T r = lds[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) r = lds[i] < r ? lds[i] : r;
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------- tickets
+// Persistent-tile helpers. Every workgroup loops: t = next_tile(); if t >=
+// ntiles break. The ticket order guarantees that when a workgroup holds tile
+// t, every tile < t is held by a running (or finished) workgroup, so the
+// look-back below always terminates.
+__device__ __forceinline__ unsigned next_tile(Ctl* ctl, int slot, unsigned* s_tile) {
+  if (threadIdx.x == 0) *s_tile = atomicAdd(&ctl->ticket[slot], 1u);
+  __syncthreads();
+  unsigned t = *s_tile;
+  __syncthreads();
+  return t;
+}
+
+// Called once per workgroup after its last tile. The last workgroup resets
+// the slot and bumps the epoch. Returns true in the last workgroup (all
+// threads), after every other workgroup's writes are visible.
+__device__ __forceinline__ bool finish_persistent(Ctl* ctl, int slot, unsigned* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned d = atomicAdd(&ctl->done[slot], 1u);
+    unsigned last = (d == gridDim.x - 1) ? 1u : 0u;
+    if (last) {
+      __threadfence();
+      atomicExch(&ctl->ticket[slot], 0u);
+      atomicExch(&ctl->done[slot], 0u);
+      atomicAdd(&ctl->epoch, 1u);
+    }
+    *s_flag = last;
+  }
+  __syncthreads();
+  bool last = *s_flag != 0;
+  __syncthreads();
+  return last;
+}
+
+// Decoupled look-back: given this tile's aggregate (thread 0's argument is
+// used), returns the exclusive prefix of all earlier tiles, to every thread.
+// `state` holds one word per tile. Wave 0 does the look-back, 64 tiles at a
+// time. Must be called by all threads of the block.
+__device__ __forceinline__ unsigned long long tile_prefix(unsigned long long* state, unsigned tile,
+                                                          unsigned epoch, unsigned long long agg,
+                                                          unsigned long long* s_pre) {
+  const unsigned ep = epoch & ((1u << kEpochBits) - 1);
+  if (threadIdx.x == 0) {
+    if (tile == 0) st_relaxed(&state[0], pack_state(2, ep, agg));
+    else st_relaxed(&state[tile], pack_state(1, ep, agg));
+  }
+  if (threadIdx.x < kWave) {
+    unsigned long long excl = 0;
+    if (tile > 0) {
+      long long base = (long long)tile - 1;  // highest tile index of the window
+      while (true) {
+        long long idx = base - lane();
+        unsigned long long s = 0;
+        unsigned flag = 0;
+        if (idx >= 0) {
+          do {
+            s = ld_relaxed(&state[idx]);
+            flag = (unsigned)(s >> 62);
+            if (((s >> 38) & ((1u << kEpochBits) - 1)) != ep) flag = 0;
+          } while (flag == 0);
+        } else {
+          flag = 2;  // before tile 0: acts as an inclusive zero
+          s = 0;
+        }
+        unsigned long long v = (idx >= 0) ? (s & kValMask) : 0ull;
+        // lanes ordered from nearest (lane 0) to farthest; stop at the first
+        // inclusive one.
+        uint64_t incl = __ballot(flag == 2);
+        int stop = incl ? __builtin_ctzll(incl) : kWave;  // first inclusive lane
+        unsigned long long contrib = (lane() <= stop) ? v : 0ull;
+        contrib = wave_sum(contrib);
+        excl += contrib;
+        if (incl) break;
+        base -= kWave;
+      }
+      if (threadIdx.x == 0) st_relaxed(&state[tile], pack_state(2, ep, excl + agg));
+    }
+    if (threadIdx.x == 0) *s_pre = excl;
+  }
+  __syncthreads();
+  unsigned long long r = *s_pre;
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------- misc
+__device__ __forceinline__ int upper_bound_lds(const int* a, int lo, int hi, int key) {
+  // first index in [lo,hi) with a[i] > key
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void atomic_add_f32(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace spmspv

@@ -1,0 +1,106 @@
+// The handle behind the C-ABI: the matrix (borrowed or owned), the current
+// x, and one device arena carved at set-up time so that no call after it
+// allocates (the reference's allocPreBuffer arena, spmspv.h:1722-1763, plus
+// the hidden per-call allocations it made, here sized up front).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/spmspv_amd.h"
+#include "kernels.hpp"
+
+struct BfsState;
+
+struct spmspv_handle_s {
+  int m = 0, n = 0, nnz = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // matrix
+  int* csr_row = nullptr;
+  int* csr_col = nullptr;
+  float* csr_val = nullptr;
+  int* csc_colptr = nullptr;
+  int* csc_row = nullptr;
+  float* csc_val = nullptr;
+  bool csc_owned = false;
+  void* csc_alloc = nullptr;
+
+  // x
+  int* x_key = nullptr;
+  float* x_val = nullptr;
+  int xnnz = 0;
+  float* x_dense_user = nullptr;
+  unsigned* bits_user = nullptr;
+  int num_ints_user = 0;
+  int vec_type = -1;
+  bool sparse_valid = false;  // x_key/x_val describe the current x
+  bool dense_valid = false;   // dense x holds the current x
+  bool bits_valid = false;    // bit array holds the current x's support
+  bool prep_valid = false;    // ptr/xbeg/split/bin_len computed for the current x
+  long long binlen_host = -1; // bin_len read back for the current x (or -1)
+  int spmv_type = 0;          // 0 merge (Hola role), 1 CSR5
+  int sigma = -1;
+
+  // arena
+  bool reserved = false;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  spmspv::Ctl* ctl = nullptr;
+  spmspv::MatFeat* mat = nullptr;
+  double* mat_partials = nullptr;
+  int* ptr = nullptr;        // n+1 (x scan); doubles as the degree histogram
+  int* xbeg = nullptr;       // n
+  int* split = nullptr;      // LB tiles + 2
+  int* tile_mm = nullptr;    // 2 per prep tile
+  unsigned long long* scan_state = nullptr;
+  float* acc = nullptr;      // m, all-zero between calls
+  unsigned* bitmap = nullptr;  // ceil(m/32), all-zero between calls
+  spmspv::SortBufs sb{};
+  int* yk_tmp = nullptr;     // m
+  float* yv_tmp = nullptr;   // m
+  float* x_dense_int = nullptr;  // n
+  unsigned* bits_int = nullptr;  // ceil(n/32)
+  int* xk_int = nullptr;     // n (x converted from dense)
+  float* xv_int = nullptr;   // n
+  int* row_split = nullptr;
+  int n_row_tiles = 0;
+  float* row_carry_val = nullptr;
+  int* row_carry_row = nullptr;
+  int* row_carry_hit = nullptr;
+  long long* n_dev = nullptr;
+  float* weights_dev = nullptr;
+  float* flush_buf = nullptr;
+  size_t flush_bytes = 0;
+  bool mat_ready = false;
+  bool row_ready = false;
+
+  // selector
+  bool have_weights = false;
+  int sel_mode = 1;
+  int sel_fixed = 0;
+  int team_col = 16;
+  int team_row = 8;
+
+  // host mirrors
+  int* h_pin = nullptr;       // small pinned readback area (64 ints)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  long long launches = 0;
+
+  BfsState* bfs = nullptr;
+
+  float* dense_x() { return x_dense_user ? x_dense_user : x_dense_int; }
+  unsigned* bits() { return bits_user ? bits_user : bits_int; }
+};
+
+namespace spmspv {
+int set_error(spmspv_handle_t h, int code, const std::string& msg);
+int ensure_arena(spmspv_handle_t h);
+int ensure_row_tiles(spmspv_handle_t h);
+int ensure_mat_features(spmspv_handle_t h);
+int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
+int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);
+int readback_ynnz(spmspv_handle_t h, int* y_nnz);
+}  // namespace spmspv

@@ -1,0 +1,501 @@
+// In-house LSD radix sort of (uint32 key, 32-bit value) pairs and the
+// segmented reduce-by-key that follows it: the sort half of the reference's
+// CSC sort kernels (thrust::sort_by_key + thrust::reduce_by_key,
+// la/include/spmspv/csc-spmspv/detail/device/csc_sort_spmspv.cuh:339-437,
+// 1030-1152), written for wave64:
+//  * keys are ranked inside a tile with 8 wave ballots per key (peers with
+//    the same 8-bit digit) and per-wave digit counters in LDS (stable);
+//  * one global histogram pass gives every pass's digit offsets; each pass
+//    is then a single kernel whose tiles find their per-digit global offset
+//    by decoupled look-back (the "onesweep" structure);
+//  * tiles are re-ordered in LDS before the store so that runs of equal
+//    digits leave as contiguous, coalesced writes;
+//  * n <= one tile takes a single-workgroup path that sorts all passes in
+//    LDS and reduces in the same launch.
+// The reduce is deterministic: fixed-order segmented scans inside a tile
+// and a fixed-order carry fix-up across tiles.
+#include "kernels.hpp"
+
+namespace spmspv {
+
+// ---------------------------------------------------------------- ranking
+// Items of a tile are laid out per wave: wave w owns positions
+// [w*64*IPT, (w+1)*64*IPT); item j of lane l sits at w*64*IPT + j*64 + l.
+// On return cnt (this wave's 256 counters) holds the per-digit counts of
+// the wave's chunk and rank[j] the stable rank of item j among equal digits
+// inside the chunk.
+template <int IPT>
+__device__ __forceinline__ void wave_rank(const unsigned (&key)[IPT], const bool (&valid)[IPT],
+                                          int shift, unsigned* cnt, int (&rank)[IPT],
+                                          unsigned (&dig)[IPT]) {
+  for (int k = lane(); k < 256; k += kWave) cnt[k] = 0u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const uint64_t lt = lane() == 0 ? 0ull : (~0ull >> (kWave - lane()));
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const unsigned d = valid[j] ? ((key[j] >> shift) & 255u) : 0u;
+    uint64_t peers = __ballot(valid[j]);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    unsigned base = 0;
+    if (valid[j]) base = cnt[d];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (valid[j]) {
+      rank[j] = (int)base + __popcll(peers & lt);
+      if ((peers >> lane()) == 1ull) cnt[d] = base + (unsigned)__popcll(peers);
+    } else {
+      rank[j] = 0;
+    }
+    dig[j] = d;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Converts the per-wave counts into exclusive-over-waves offsets (in place)
+// and returns, for digit d = threadIdx.x (NT == 256), the tile total.
+template <int NW>
+__device__ __forceinline__ unsigned digit_wave_offsets(unsigned* s_cnt) {
+  const int d = threadIdx.x;
+  unsigned tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const unsigned c = s_cnt[w * 256 + d];
+    s_cnt[w * 256 + d] = tot;
+    tot += c;
+  }
+  return tot;
+}
+
+// -------------------------------------------------------- segmented scan
+// Exclusive segmented sum over the threads of a block: the combine of
+// (f1,v1) then (f2,v2) is (f1|f2, f2 ? v2 : v1 + v2). Returns for each
+// thread the combined value of all earlier threads back to the nearest one
+// with f set (inclusive). Also returns the block-inclusive value of the last
+// thread in *last_incl (valid in all threads).
+template <int NT, typename V>
+__device__ __forceinline__ V block_seg_excl(bool f, V v, int* s_f, V* s_v, V* last_incl) {
+  constexpr int NW = NT / kWave;
+  const int w = threadIdx.x / kWave;
+  int fi = f ? 1 : 0;
+  V vi = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int fo = __shfl_up(fi, o, kWave);
+    const V vo = __shfl_up(vi, o, kWave);
+    if (lane() >= o) {
+      if (!fi) vi = vo + vi;
+      fi |= fo;
+    }
+  }
+  if (lane() == kWave - 1) {
+    s_f[w] = fi;
+    s_v[w] = vi;
+  }
+  __syncthreads();
+  V pv = V(0);
+  for (int k = 0; k < w; ++k) pv = s_f[k] ? s_v[k] : pv + s_v[k];
+  V tv = V(0);
+  for (int k = 0; k < NW; ++k) tv = s_f[k] ? s_v[k] : tv + s_v[k];
+  const V incl = fi ? vi : pv + vi;
+  V ex = __shfl_up(incl, 1, kWave);
+  if (lane() == 0) ex = pv;
+  __syncthreads();
+  *last_incl = tv;
+  return ex;
+}
+
+// ---------------------------------------------------------- tile reduce
+// Reduces the sorted keys s_k[0..cnt) / values s_v of one tile. `prev_key`
+// is the key just before the tile (ignored when global_start). get_prefix
+// (called by all threads with the tile's head count) returns the number of
+// heads of all earlier tiles. Segment ends whose head lies in an earlier
+// tile go to carry_val/carry_idx[tile] for the fix-up kernel.
+template <int NT, int IPT, typename PrefixFn>
+__device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const float* s_v, int cnt,
+                                                 unsigned prev_key, bool global_start,
+                                                 PrefixFn get_prefix, int* ykey, float* yval,
+                                                 float* carry_val, int* carry_idx, long long tile,
+                                                 int* s_i, int* s_f, float* s_fv) {
+  const int i0 = threadIdx.x * IPT;
+  bool head[IPT];
+  int nh = 0;
+  float tail = 0.f;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = i0 + j;
+    head[j] = false;
+    if (i < cnt) {
+      const unsigned pk = (i == 0) ? prev_key : s_k[i - 1];
+      head[j] = (i == 0 && global_start) || s_k[i] != pk;
+      if (head[j]) tail = s_v[i];
+      else tail += s_v[i];
+    }
+    nh += head[j] ? 1 : 0;
+  }
+  int htot;
+  const int hex = block_excl_sum<NT, int>(nh, s_i, &htot);
+  const long long gpre = get_prefix(htot);
+  float last_incl;
+  const float cin = block_seg_excl<NT, float>(nh > 0, tail, s_f, s_fv, &last_incl);
+  float acc = cin;
+  int hc = hex;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = i0 + j;
+    if (i >= cnt) break;
+    if (head[j]) {
+      ++hc;
+      acc = s_v[i];
+      ykey[gpre + hc - 1] = (int)s_k[i];
+    } else {
+      acc += s_v[i];
+    }
+    const bool end = (i == cnt - 1) || (s_k[i + 1] != s_k[i]);
+    if (end) {
+      if (hc > 0) {
+        yval[gpre + hc - 1] = acc;
+      } else {
+        carry_val[tile] = acc;
+        carry_idx[tile] = (int)(gpre - 1);
+      }
+    }
+  }
+  if (threadIdx.x == 0 && carry_idx && (cnt == 0 || head[0])) carry_idx[tile] = -1;
+  return gpre + htot;
+}
+
+// ------------------------------------------------------------- histogram
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sort_hist(const unsigned* __restrict__ keys,
+                                                  const long long* __restrict__ n_dev, int passes,
+                                                  unsigned* hist, unsigned* digoff, Ctl* ctl,
+                                                  int small_cap) {
+  __shared__ unsigned s_h[4 * 256];
+  __shared__ unsigned s_flag;
+  __shared__ unsigned s_t[NT / kWave];
+  const long long n = *n_dev;
+  if (n <= small_cap) return;
+  for (int k = threadIdx.x; k < 4 * 256; k += NT) s_h[k] = 0u;
+  __syncthreads();
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+    const unsigned k = keys[i];
+    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < passes * 256; k += NT)
+    if (s_h[k]) atomicAdd(&hist[k], s_h[k]);
+  const bool last = finish_persistent(ctl, kSlotHist, &s_flag);
+  if (!last) return;
+  for (int p = 0; p < passes; ++p) {
+    const unsigned c = __hip_atomic_load(&hist[p * 256 + threadIdx.x], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    unsigned tot;
+    const unsigned ex = block_excl_sum<NT, unsigned>(c, s_t, &tot);
+    digoff[p * 256 + threadIdx.x] = ex;
+    hist[p * 256 + threadIdx.x] = 0u;
+  }
+}
+
+// ------------------------------------------------------------ one pass
+template <int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ kin,
+                                                  const float* __restrict__ vin,
+                                                  unsigned* __restrict__ kout,
+                                                  float* __restrict__ vout,
+                                                  const long long* __restrict__ n_dev, int shift,
+                                                  const unsigned* __restrict__ digoff,
+                                                  unsigned long long* state, Ctl* ctl,
+                                                  int small_cap) {
+  constexpr int TILE = NT * IPT;
+  constexpr int NW = NT / kWave;
+  static_assert(NT == 256, "one thread per digit");
+  __shared__ unsigned s_k[TILE];
+  __shared__ float s_v[TILE];
+  __shared__ unsigned s_cnt[NW * 256];
+  __shared__ unsigned s_dstart[256];
+  __shared__ unsigned s_gbase[256];
+  __shared__ unsigned s_t[NW];
+  __shared__ unsigned s_tile, s_flag;
+  const long long n = *n_dev;
+  if (n <= small_cap) return;
+  const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
+  const unsigned ep = ctl->epoch & ((1u << kEpochBits) - 1);
+  const int w = threadIdx.x / kWave;
+  while (true) {
+    const unsigned t = next_tile(ctl, kSlotSort, &s_tile);
+    if (t >= ntiles) break;
+    const long long base = (long long)t * TILE;
+    unsigned key[IPT];
+    float val[IPT];
+    bool valid[IPT];
+    int rank[IPT];
+    unsigned dig[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const long long gi = base + w * kWave * IPT + j * kWave + lane();
+      valid[j] = gi < n;
+      key[j] = valid[j] ? kin[gi] : 0u;
+      val[j] = valid[j] ? vin[gi] : 0.f;
+    }
+    wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
+    __syncthreads();
+    const int d = threadIdx.x;
+    const unsigned tot = digit_wave_offsets<NW>(s_cnt);
+    unsigned dtot;
+    const unsigned dex = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
+    s_dstart[d] = dex;
+    // per-digit decoupled look-back
+    unsigned long long* st = state + (size_t)t * 256 + d;
+    st_relaxed(st, pack_state(t == 0 ? 2u : 1u, ep, tot));
+    unsigned long long excl = 0;
+    if (t > 0) {
+      long long k = (long long)t - 1;
+      while (true) {
+        unsigned long long s;
+        unsigned f;
+        do {
+          s = ld_relaxed(state + (size_t)k * 256 + d);
+          f = (unsigned)(s >> 62);
+          if (((s >> 38) & ((1u << kEpochBits) - 1)) != ep) f = 0;
+        } while (f == 0);
+        excl += s & kValMask;
+        if (f == 2) break;
+        --k;
+      }
+      st_relaxed(st, pack_state(2u, ep, excl + tot));
+    }
+    s_gbase[d] = digoff[d] + (unsigned)excl;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (valid[j]) {
+        const unsigned pos = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
+        s_k[pos] = key[j];
+        s_v[pos] = val[j];
+      }
+    }
+    __syncthreads();
+    const int cnt = (int)min((long long)TILE, n - base);
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      const unsigned k = s_k[i];
+      const unsigned dd = (k >> shift) & 255u;
+      const unsigned g = s_gbase[dd] + (i - s_dstart[dd]);
+      kout[g] = k;
+      vout[g] = s_v[i];
+    }
+    __syncthreads();
+  }
+  finish_persistent(ctl, kSlotSort, &s_flag);
+}
+
+// --------------------------------------------------- single-workgroup path
+// n <= TILE: all passes in LDS. mode 0: reduce and write (ykey, yval,
+// ctl->ynnz). mode 1: write the sorted pairs to (kout, vout).
+template <int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ kin,
+                                                   const float* __restrict__ vin,
+                                                   const long long* __restrict__ n_dev, int passes,
+                                                   int mode, Ctl* ctl, int* ykey, float* yval,
+                                                   unsigned* kout, float* vout) {
+  constexpr int TILE = NT * IPT;
+  constexpr int NW = NT / kWave;
+  __shared__ unsigned s_k[TILE];
+  __shared__ float s_v[TILE];
+  __shared__ unsigned s_cnt[NW * 256];
+  __shared__ unsigned s_dstart[256];
+  __shared__ unsigned s_t[NW];
+  __shared__ int s_i[NW];
+  __shared__ int s_f[NW];
+  __shared__ float s_fv[NW];
+  const long long n = *n_dev;
+  if (n > TILE) return;
+  const int w = threadIdx.x / kWave;
+  unsigned key[IPT];
+  float val[IPT];
+  bool valid[IPT];
+  int rank[IPT];
+  unsigned dig[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int gi = w * kWave * IPT + j * kWave + lane();
+    valid[j] = gi < n;
+    key[j] = valid[j] ? kin[gi] : 0u;
+    val[j] = valid[j] ? vin[gi] : 0.f;
+  }
+  for (int p = 0; p < passes; ++p) {
+    wave_rank<IPT>(key, valid, 8 * p, s_cnt + w * 256, rank, dig);
+    __syncthreads();
+    const unsigned tot = digit_wave_offsets<NW>(s_cnt);
+    unsigned dtot;
+    const unsigned dex = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
+    s_dstart[threadIdx.x] = dex;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (valid[j]) {
+        const unsigned pos = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
+        s_k[pos] = key[j];
+        s_v[pos] = val[j];
+      }
+    }
+    __syncthreads();
+    if (p + 1 < passes) {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int gi = w * kWave * IPT + j * kWave + lane();
+        if (valid[j]) {
+          key[j] = s_k[gi];
+          val[j] = s_v[gi];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int cnt = (int)n;
+  if (mode == 1) {
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      kout[i] = s_k[i];
+      vout[i] = s_v[i];
+    }
+    return;
+  }
+  auto no_prefix = [](int) -> long long { return 0; };
+  const long long tot = tile_reduce<NT, IPT>(s_k, s_v, cnt, 0u, true, no_prefix, ykey, yval,
+                                             nullptr, nullptr, 0, s_i, s_f, s_fv);
+  if (threadIdx.x == 0) {
+    ctl->ynnz = (int)tot;
+    ctl->sort_small = 1;
+  }
+}
+
+// ------------------------------------------------------- reduce by key
+template <int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys,
+                                               const float* __restrict__ vals,
+                                               const long long* __restrict__ n_dev, Ctl* ctl,
+                                               int* ykey, float* yval, float* carry_val,
+                                               int* carry_idx, unsigned long long* state,
+                                               int small_cap) {
+  constexpr int TILE = NT * IPT;
+  constexpr int NW = NT / kWave;
+  __shared__ unsigned s_k[TILE];
+  __shared__ float s_v[TILE];
+  __shared__ int s_i[NW];
+  __shared__ int s_f[NW];
+  __shared__ float s_fv[NW];
+  __shared__ unsigned s_tile, s_flag;
+  __shared__ unsigned long long s_pre;
+  const long long n = *n_dev;
+  if (n <= small_cap) return;
+  const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
+  const unsigned epoch = ctl->epoch;
+  while (true) {
+    const unsigned t = next_tile(ctl, kSlotReduce, &s_tile);
+    if (t >= ntiles) break;
+    const long long base = (long long)t * TILE;
+    const int cnt = (int)min((long long)TILE, n - base);
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      s_k[i] = keys[base + i];
+      s_v[i] = vals[base + i];
+    }
+    const unsigned prev = base > 0 ? keys[base - 1] : 0u;
+    __syncthreads();
+    auto prefix = [&](int htot) -> long long {
+      return (long long)tile_prefix(state, t, epoch, (unsigned long long)htot, &s_pre);
+    };
+    const long long end = tile_reduce<NT, IPT>(s_k, s_v, cnt, prev, base == 0, prefix, ykey, yval,
+                                               carry_val, carry_idx, t, s_i, s_f, s_fv);
+    if (t == ntiles - 1 && threadIdx.x == 0) {
+      ctl->ynnz = (int)end;
+      ctl->sort_small = 0;
+    }
+    __syncthreads();
+  }
+  finish_persistent(ctl, kSlotReduce, &s_flag);
+}
+
+__global__ void k_carry_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_idx,
+                              const long long* __restrict__ n_dev, int tile, int small_cap,
+                              float* yval) {
+  const long long n = *n_dev;
+  if (n <= small_cap) return;
+  const long long ntiles = (n + tile - 1) / tile;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntiles;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int idx = carry_idx[t];
+    if (idx < 0) continue;
+    if (t > 0 && carry_idx[t - 1] == idx) continue;
+    float s = 0.f;
+    for (long long u = t; u < ntiles && carry_idx[u] == idx; ++u) s += carry_val[u];
+    yval[idx] += s;
+  }
+}
+
+static int key_passes(int range) {
+  int bits = 1;
+  while (bits < 32 && (1u << bits) < (unsigned)range) ++bits;
+  return (bits + 7) / 8;
+}
+
+// Sorts (keyA,valA)[*n_dev] by key < range; returns 0 when the result is in
+// A, 1 when in B. Used by the SpMSpV sort path (mode reduce) and by the
+// CSR->CSC transpose (mode sorted).
+int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
+                      long long n_hint, bool skip_small) {
+  const int passes = key_passes(range);
+  const int final_b = passes & 1;
+  const bool small_known = n_hint >= 0 && n_hint <= kSmallSortCap;
+  const bool big_known = n_hint > kSmallSortCap;
+  if (!big_known && !skip_small) {
+    hipLaunchKernelGGL((k_sort_small<kSortNT, kSortIPT>), dim3(1), dim3(kSortNT), 0, s, b.keyA,
+                       b.valA, n_dev, passes, 1, ctl, nullptr, nullptr, final_b ? b.keyB : b.keyA,
+                       final_b ? b.valB : b.valA);
+  }
+  if (small_known) return final_b;
+  int hgrid = n_hint > 0 ? (int)std::min<long long>(1024, (n_hint + 4095) / 4096) : 1024;
+  hipLaunchKernelGGL(k_sort_hist<256>, dim3(hgrid), dim3(256), 0, s, b.keyA, n_dev, passes, b.hist,
+                     b.digoff, ctl, kSmallSortCap);
+  int pgrid = n_hint > 0 ? (int)std::min<long long>(kPersistGrid, (n_hint + kSortTile - 1) / kSortTile)
+                         : kPersistGrid;
+  for (int p = 0; p < passes; ++p) {
+    const unsigned* kin = (p & 1) ? b.keyB : b.keyA;
+    const float* vin = (p & 1) ? b.valB : b.valA;
+    unsigned* kout = (p & 1) ? b.keyA : b.keyB;
+    float* vout = (p & 1) ? b.valA : b.valB;
+    hipLaunchKernelGGL((k_sort_pass<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, vin,
+                       kout, vout, n_dev, 8 * p, b.digoff + p * 256, b.state, ctl, kSmallSortCap);
+  }
+  return final_b;
+}
+
+void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
+                        long long binlen_hint) {
+  const long long* n_dev = &ctl->bin_len;
+  const int passes = key_passes(m);
+  const bool small_known = binlen_hint >= 0 && binlen_hint <= kSmallSortCap;
+  const bool big_known = binlen_hint > kSmallSortCap;
+  if (!big_known) {
+    hipLaunchKernelGGL((k_sort_small<kSortNT, kSortIPT>), dim3(1), dim3(kSortNT), 0, s, b.keyA,
+                       b.valA, n_dev, passes, 0, ctl, ykey, yval, nullptr, nullptr);
+  }
+  if (small_known) return;
+  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, true);
+  const unsigned* kf = fb ? b.keyB : b.keyA;
+  const float* vf = fb ? b.valB : b.valA;
+  int rgrid = binlen_hint > 0 ? (int)std::min<long long>(kPersistGrid, (binlen_hint + kRedTile - 1) / kRedTile)
+                              : kPersistGrid;
+  hipLaunchKernelGGL((k_reduce<kRedNT, kRedIPT>), dim3(rgrid), dim3(kRedNT), 0, s, kf, vf, n_dev, ctl,
+                     ykey, yval, b.carry_val, b.carry_idx, b.red_state, kSmallSortCap);
+  int fgrid = binlen_hint > 0 ? (int)std::min<long long>(2048, (binlen_hint / kRedTile + 256) / 256) : 512;
+  hipLaunchKernelGGL(k_carry_fixup, dim3(std::max(fgrid, 1)), dim3(256), 0, s, b.carry_val,
+                     b.carry_idx, n_dev, kRedTile, kSmallSortCap, yval);
+}
+
+}  // namespace spmspv

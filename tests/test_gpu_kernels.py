@@ -1,0 +1,252 @@
+"""Parity of every kernel slot with the CPU oracle, through the C-ABI.
+
+Index sets must match bit-exactly (structural support, sorted ascending);
+FP32 values must satisfy |y - y64| <= 1e-5 * sum_j |a_ij x_j| against the
+FP64 oracle (the tolerance north of every value check below)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import spmspv_amd as sa
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def csc_of(m, n, rp, col, val):
+    return oracle.csr_to_csc(m, n, rp, col, val)
+
+
+def check_result(m, cp, ri, cv, key, val, yk, yv, tag):
+    ek, ev, eb = oracle.spmspv_f64(m, cp, ri, cv, key, val)
+    assert yk.size == ek.size, f"{tag}: ynnz {yk.size} != {ek.size}"
+    np.testing.assert_array_equal(yk, ek, err_msg=tag)
+    err = np.abs(yv.astype(np.float64) - ev)
+    lim = TOL * eb + 1e-30
+    bad = np.nonzero(err > lim)[0]
+    assert bad.size == 0, f"{tag}: {bad.size} values out of bound, first {bad[:5]}, err {err[bad[:5]]} lim {lim[bad[:5]]}"
+    return float((err / np.maximum(eb, 1e-300)).max()) if err.size else 0.0
+
+
+def run_slot(M, k, key, val, want_dense=False):
+    dk = sa.DeviceArray.from_host(key if key.size else np.zeros(1, np.int32), np.int32)
+    dv = sa.DeviceArray.from_host(val if val.size else np.zeros(1, np.float32), np.float32)
+    M.A.set_sparse_x(dk, dv, int(key.size))
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    yd = sa.DeviceArray(M.m, np.float32) if want_dense else None
+    nnz = M.A.run(k, y_dense=yd, y_key=yk, y_val=yv)
+    out = (yk.to_host(nnz), yv.to_host(nnz), yd.to_host() if yd else None)
+    return out
+
+
+SLOTS = list(range(8))
+
+
+@pytest.fixture(scope="module")
+def c1(gpu):
+    """Config 1: uniform n=65,536, 16 distinct rows per column, U[-1,1];
+    x with 655 distinct keys, U[-1,1]; seed 0x5EED0001."""
+    n, k = 65536, 16
+    cp, ri, cv = sa.gen_uniform_csc(n, k, 0x5EED0001)
+    # CSR of the same matrix (the handle takes CSR like the reference)
+    import scipy.sparse as sp
+    C = sp.csc_matrix((cv, ri, cp), shape=(n, n))
+    R = C.tocsr()
+    R.sort_indices()
+    M = sa.Matrix(n, n, R.indptr.astype(np.int32), R.indices.astype(np.int32), R.data.astype(np.float32))
+    key, val = sa.gen_sparse_vector(n, 655, 0x5EED0001)
+    return M, (cp, ri, cv), key, val
+
+
+@pytest.mark.parametrize("slot", SLOTS)
+def test_c1_every_slot(c1, slot):
+    M, (cp, ri, cv), key, val = c1
+    say(f"c1 slot {slot} ({sa.KERNEL_NAMES[slot]})")
+    yk, yv, _ = run_slot(M, slot, key, val)
+    err = check_result(M.m, cp, ri, cv, key, val, yk, yv, f"c1 slot {slot}")
+    say(f"  ynnz {yk.size} max rel err {err:.3e}")
+
+
+def test_c1_tocsc_bit_exact(c1):
+    M, (cp, ri, cv), _, _ = c1
+    say("c1 ToCSC vs oracle transpose")
+    ocp, ori, ocv = oracle.csr_to_csc(M.m, M.n, M.h_row_ptr, M.h_col, M.h_val)
+    np.testing.assert_array_equal(ocp, cp)
+    # the handle's CSC is internal; check it through a unit vector per column
+    for j in (0, 1, 777, 65535):
+        yk, yv, _ = run_slot(M, sa.K_COL_NAIVE_SORT, np.array([j], np.int32), np.array([1.0], np.float32))
+        np.testing.assert_array_equal(yk, ori[ocp[j]:ocp[j + 1]])
+        np.testing.assert_array_equal(yv, ocv[ocp[j]:ocp[j + 1]])
+
+
+def test_c1_vec_features_exact(c1):
+    M, (cp, ri, cv), key, val = c1
+    say("c1 vec features")
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    M.A.set_sparse_x(dk, dv, key.size)
+    b, mx, mn = M.A.computeVecFeature()
+    ref = oracle.vec_features(cp, key)
+    assert (b, mx, mn) == tuple(int(v) for v in ref)
+    gm = M.A.computeGM()
+    np.testing.assert_allclose(gm, oracle.gm(M.m, M.nnz, key.size, b))
+
+
+def test_c1_dense_output(c1):
+    M, (cp, ri, cv), key, val = c1
+    for slot in SLOTS:
+        say(f"c1 dense output slot {slot}")
+        yk, yv, yd = run_slot(M, slot, key, val, want_dense=True)
+        check_result(M.m, cp, ri, cv, key, val, yk, yv, f"dense slot {slot}")
+        dense = np.zeros(M.m, np.float32)
+        dense[yk] = yv
+        np.testing.assert_array_equal(yd, dense)
+
+
+def build(m, n, rp, col, val):
+    return sa.Matrix(m, n, np.asarray(rp, np.int32), np.asarray(col, np.int32), np.asarray(val, np.float32))
+
+
+@pytest.fixture(scope="module")
+def rmat12(gpu):
+    rp, col, val = sa.gen_rmat_csr(12, 16, seed=20)
+    n = 1 << 12
+    M = build(n, n, rp, col, val)
+    return M, csc_of(n, n, rp, col, val)
+
+
+@pytest.mark.parametrize("density", [1e-4, 1e-3, 1e-2, 0.1, 0.5, 1.0])
+def test_rmat_density_sweep(rmat12, density):
+    M, (cp, ri, cv) = rmat12
+    k = max(1, int(round(density * M.n)))
+    key, val = sa.gen_sparse_vector(M.n, k, 77 + k)
+    for slot in SLOTS:
+        say(f"rmat12 density {density} xnnz {k} slot {slot}")
+        yk, yv, _ = run_slot(M, slot, key, val)
+        check_result(M.m, cp, ri, cv, key, val, yk, yv, f"rmat d={density} slot {slot}")
+
+
+def test_edge_cases(gpu):
+    # rectangular, empty rows/columns, exact cancellation, zero x values
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    m, n = 3000, 2000
+    A = sp.random(m, n, density=0.002, format="csr", random_state=rng, dtype=np.float32)
+    A.data = rng.uniform(-1, 1, A.nnz).astype(np.float32)
+    A = A.tolil()
+    A[10, :] = 0
+    A[:, 7] = 0
+    A[20, 3] = 1.0
+    A[20, 4] = -1.0  # row 20 cancels exactly when x3 = x4 = 1
+    A = A.tocsr()
+    A.sort_indices()
+    A.eliminate_zeros()
+    M = build(m, n, A.indptr, A.indices, A.data)
+    cp, ri, cv = csc_of(m, n, A.indptr, A.indices, A.data)
+    cases = {
+        "empty x": (np.zeros(0, np.int32), np.zeros(0, np.float32)),
+        "empty column only": (np.array([7], np.int32), np.array([2.0], np.float32)),
+        "cancellation": (np.array([3, 4], np.int32), np.array([1.0, 1.0], np.float32)),
+        "zero x values": (np.array([3, 4, 5, 100], np.int32), np.zeros(4, np.float32)),
+        "full x": (np.arange(n, dtype=np.int32), rng.uniform(-1, 1, n).astype(np.float32)),
+        "unsorted keys": (np.array([1999, 5, 1000, 3], np.int32), np.array([1, 2, 3, 4], np.float32)),
+    }
+    for name, (key, val) in cases.items():
+        for slot in SLOTS:
+            say(f"edge case '{name}' slot {slot}")
+            yk, yv, _ = run_slot(M, slot, key, val)
+            check_result(m, cp, ri, cv, key, val, yk, yv, f"{name} slot {slot}")
+
+
+def test_hub_column_spans_many_tiles(gpu):
+    # one column with 50,000 entries + long rows: load-balanced tiles and
+    # the carry fix-ups must join partial sums across many tiles
+    import scipy.sparse as sp
+    rng = np.random.default_rng(8)
+    m = n = 60000
+    rows = np.concatenate([np.arange(50000), rng.integers(0, m, 20000), np.full(30000, 123)])
+    cols = np.concatenate([np.full(50000, 9), rng.integers(0, n, 20000), rng.choice(n, 30000, replace=False)])
+    vals = rng.uniform(-1, 1, rows.size).astype(np.float32)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    M = build(m, n, A.indptr, A.indices, A.data.astype(np.float32))
+    cp, ri, cv = csc_of(m, n, A.indptr, A.indices, A.data.astype(np.float32))
+    for key in (np.array([9], np.int32), np.array([9, 10, 11], np.int32), np.arange(0, n, 3, dtype=np.int32)):
+        val = rng.uniform(-1, 1, key.size).astype(np.float32)
+        for slot in SLOTS:
+            say(f"hub xnnz {key.size} slot {slot}")
+            yk, yv, _ = run_slot(M, slot, key, val)
+            check_result(m, cp, ri, cv, key, val, yk, yv, f"hub slot {slot}")
+
+
+def test_repeat_calls_keep_state_clean(rmat12):
+    """The accumulator and bitmaps are reset by every call: results of a
+    call must not depend on the previous one."""
+    M, (cp, ri, cv) = rmat12
+    for it in range(6):
+        key, val = sa.gen_sparse_vector(M.n, 50 + 300 * it, 1000 + it)
+        for slot in SLOTS:
+            yk, yv, _ = run_slot(M, slot, key, val)
+            check_result(M.m, cp, ri, cv, key, val, yk, yv, f"repeat {it} slot {slot}")
+    say("repeat calls ok")
+
+
+def test_mat_features_match_oracle(rmat12):
+    M, _ = rmat12
+    say("mat features")
+    f = M.A.computeMatFeature()
+    ref = oracle.mat_features(M.m, M.n, M.h_row_ptr)
+    np.testing.assert_allclose(f, ref.astype(np.float32), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(M.A.extractMatFeature(), oracle.std_row_trunc(M.m, M.h_row_ptr), rtol=1e-5)
+
+
+def test_conversions(rmat12):
+    M, _ = rmat12
+    say("conversions")
+    n = M.n
+    key, val = sa.gen_sparse_vector(n, 300, 4)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    dx = sa.DeviceArray(n, np.float32)
+    M.A.set_sparse_x(dk, dv, key.size)
+    M.A.set_x(dx)
+    M.A.sparse2dense()
+    ref = np.zeros(n, np.float32)
+    ref[key] = val
+    np.testing.assert_array_equal(dx.to_host(), ref)
+    # dense -> sparse keeps nonzeros in order
+    ok = sa.DeviceArray(n, np.int32)
+    ov = sa.DeviceArray(n, np.float32)
+    cnt = M.A.dense2sparse_outer(n, dx, ok, ov)
+    np.testing.assert_array_equal(ok.to_host(cnt), key)
+    np.testing.assert_array_equal(ov.to_host(cnt), val)
+    back = sa.DeviceArray(n, np.float32)
+    M.A.sparse2dense_outer(cnt, ok, ov, n, back)
+    np.testing.assert_array_equal(back.to_host(), ref)
+
+
+def test_selector_tree_mode_and_run_auto(rmat12):
+    M, (cp, ri, cv) = rmat12
+    M.A.selector_mode(1)
+    for k in (3, 400, 4096):
+        key, val = sa.gen_sparse_vector(M.n, k, 5 + k)
+        dk = sa.DeviceArray.from_host(key, np.int32)
+        dv = sa.DeviceArray.from_host(val, np.float32)
+        M.A.set_sparse_x(dk, dv, key.size)
+        yk = sa.DeviceArray(M.m, np.int32)
+        yv = sa.DeviceArray(M.m, np.float32)
+        nnz, used = M.A.run_auto(y_key=yk, y_val=yv)
+        say(f"run_auto xnnz {k} -> slot {used}")
+        assert 0 <= used < 8
+        check_result(M.m, cp, ri, cv, key, val, yk.to_host(nnz), yv.to_host(nnz), f"auto {k}")
+        f = M.A.features()
+        assert np.all(np.isfinite(f))

@@ -1,0 +1,255 @@
+"""Pins the CPU oracle (oracle/oracle.c) against closed forms, brute force,
+textbook library routines (scipy) and the reference's own known-answer
+pattern. The reference ships no golden vectors, so these checks are what
+fixes the oracle; each is chosen so that a plausible slip (dropped term,
+wrong index, transposed operand, off-by-one) fails one of them."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import shortest_path
+
+import oracle
+
+
+def rand_csr(m, n, density, seed, values="uniform"):
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=density, format="csr", random_state=rng, dtype=np.float32)
+    if values == "ones":
+        A.data[:] = 1.0
+    else:
+        A.data = rng.uniform(-1, 1, A.nnz).astype(np.float32)
+    A.sort_indices()
+    return A
+
+
+def rand_x(n, k, seed):
+    rng = np.random.default_rng(seed)
+    key = np.sort(rng.choice(n, size=k, replace=False)).astype(np.int32)
+    val = rng.uniform(-1, 1, k).astype(np.float32)
+    return key, val
+
+
+# ------------------------------------------------------------ transpose
+@pytest.mark.parametrize("m,n,d", [(7, 5, 0.4), (50, 80, 0.05), (300, 200, 0.01), (1, 1, 1.0)])
+def test_csr_to_csc_matches_scipy(m, n, d):
+    A = rand_csr(m, n, d, seed=m * 31 + n)
+    cp, ri, cv = oracle.csr_to_csc(m, n, A.indptr, A.indices, A.data)
+    C = A.tocsc()
+    C.sort_indices()
+    np.testing.assert_array_equal(cp, C.indptr)
+    np.testing.assert_array_equal(ri, C.indices)
+    np.testing.assert_array_equal(cv, C.data)
+
+
+def test_csr_to_csc_empty_rows_and_cols():
+    # rows 0 and 3 empty, columns 1 and 4 empty
+    dense = np.array([[0, 0, 0, 0, 0], [2, 0, 3, 0, 0], [0, 0, 0, 5, 0], [0, 0, 0, 0, 0]], np.float32)
+    A = sp.csr_matrix(dense)
+    cp, ri, cv = oracle.csr_to_csc(4, 5, A.indptr, A.indices, A.data)
+    np.testing.assert_array_equal(cp, [0, 1, 1, 2, 3, 3])
+    np.testing.assert_array_equal(ri, [1, 1, 2])
+    np.testing.assert_array_equal(cv, [2, 3, 5])
+
+
+# ------------------------------------------------------------- SpMSpV
+@pytest.mark.parametrize("seed", range(6))
+def test_spmspv_f64_brute_force(seed):
+    m, n = 40 + seed, 30 + 2 * seed
+    A = rand_csr(m, n, 0.15, seed)
+    C = A.tocsc()
+    C.sort_indices()
+    key, val = rand_x(n, 1 + seed * 3, seed + 100)
+    yk, yv, yb = oracle.spmspv_f64(m, C.indptr, C.indices, C.data, key, val)
+    Ad = A.toarray().astype(np.float64)
+    x = np.zeros(n)
+    x[key] = val
+    y = Ad @ x
+    support = np.nonzero((Ad[:, key] != 0).any(axis=1))[0]
+    np.testing.assert_array_equal(yk, support)
+    np.testing.assert_allclose(yv, y[support], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(yb, (np.abs(Ad) @ np.abs(x))[support], rtol=1e-12)
+
+
+def test_spmspv_f64_unit_vector_is_column():
+    A = rand_csr(60, 60, 0.1, 7)
+    C = A.tocsc()
+    C.sort_indices()
+    for j in (0, 13, 59):
+        yk, yv, _ = oracle.spmspv_f64(60, C.indptr, C.indices, C.data, np.array([j]), np.array([1.0], np.float32))
+        np.testing.assert_array_equal(yk, C.indices[C.indptr[j]:C.indptr[j + 1]])
+        np.testing.assert_array_equal(yv, C.data[C.indptr[j]:C.indptr[j + 1]].astype(np.float64))
+
+
+def test_spmspv_reference_kat_all_ones():
+    """generate_model_data.cu:639-643, 942-944: every value 1, x = 1 on
+    [0, k): y[i] = number of stored entries of row i in columns < k."""
+    A = rand_csr(500, 400, 0.03, 3, values="ones")
+    C = A.tocsc()
+    C.sort_indices()
+    for k in (1, 17, 400):
+        key = np.arange(k, dtype=np.int32)
+        yk, yv, _ = oracle.spmspv_f64(500, C.indptr, C.indices, C.data, key, np.ones(k, np.float32))
+        counts = np.array([(A.indices[A.indptr[i]:A.indptr[i + 1]] < k).sum() for i in range(500)])
+        np.testing.assert_array_equal(yk, np.nonzero(counts)[0])
+        np.testing.assert_array_equal(yv, counts[counts > 0].astype(np.float64))
+        # the FP32 reference paths are exact on this integer pattern
+        yd, k32, v32 = oracle.spmspv_csc_host_f32(500, C.indptr, C.indices, C.data, key, np.ones(k, np.float32))
+        np.testing.assert_array_equal(k32, yk)
+        np.testing.assert_array_equal(v32, counts[counts > 0].astype(np.float32))
+        xd = np.zeros(400, np.float32)
+        xd[:k] = 1
+        np.testing.assert_array_equal(oracle.serialspmv_f32(500, A.indptr, A.indices, A.data, xd), counts.astype(np.float32))
+
+
+def test_spmspv_structural_support_keeps_exact_zeros():
+    # row 0 = 1*x0 - 1*x1 cancels exactly; x value 0 at column 2 still counts
+    dense = np.array([[1, -1, 0], [0, 0, 4], [0, 0, 0]], np.float32)
+    C = sp.csc_matrix(dense)
+    C.sort_indices()
+    yk, yv, _ = oracle.spmspv_f64(3, C.indptr, C.indices, C.data, np.array([0, 1, 2]), np.array([1, 1, 0], np.float32))
+    np.testing.assert_array_equal(yk, [0, 1])
+    np.testing.assert_array_equal(yv, [0.0, 0.0])
+    # the reference's FLT_EPSILON filter (test_sort_spmspv.cu:354-372) drops both
+    _, k32, _ = oracle.spmspv_csc_host_f32(3, C.indptr, C.indices, C.data, np.array([0, 1, 2]), np.array([1, 1, 0], np.float32))
+    assert k32.size == 0
+
+
+def test_spmspv_column_sum_identity():
+    A = rand_csr(200, 150, 0.05, 11)
+    C = A.tocsc()
+    C.sort_indices()
+    key, val = rand_x(150, 40, 12)
+    _, yv, _ = oracle.spmspv_f64(200, C.indptr, C.indices, C.data, key, val)
+    colsum = np.asarray(A.astype(np.float64).sum(axis=0)).ravel()
+    assert abs(yv.sum() - (val.astype(np.float64) * colsum[key]).sum()) < 1e-9
+
+
+def test_spmspv_empty_inputs():
+    A = rand_csr(20, 20, 0.2, 1)
+    C = A.tocsc()
+    yk, yv, yb = oracle.spmspv_f64(20, C.indptr, C.indices, C.data, np.zeros(0, np.int32), np.zeros(0, np.float32))
+    assert yk.size == 0 and yv.size == 0
+    cp = np.zeros(21, np.int32)
+    yk, _, _ = oracle.spmspv_f64(20, cp, np.zeros(0, np.int32), np.zeros(0, np.float32), np.array([3]), np.array([1.0], np.float32))
+    assert yk.size == 0
+
+
+def test_spmv_f64_matches_dense():
+    A = rand_csr(90, 70, 0.1, 5)
+    x = np.random.default_rng(1).uniform(-1, 1, 70).astype(np.float32)
+    y, b = oracle.spmv_f64(90, A.indptr, A.indices, A.data, x)
+    Ad = A.toarray().astype(np.float64)
+    np.testing.assert_allclose(y, Ad @ x.astype(np.float64), atol=1e-12)
+    np.testing.assert_allclose(b, np.abs(Ad) @ np.abs(x.astype(np.float64)), rtol=1e-12)
+    y32 = oracle.serialspmv_f32(90, A.indptr, A.indices, A.data, x)
+    assert np.all(np.abs(y32 - y) <= 1e-5 * b + 1e-30)
+
+
+# --------------------------------------------------------------- scan
+def test_scan_col_lengths_with_empty_columns():
+    cp = np.array([0, 0, 0, 3, 3, 5, 5], np.int32)  # columns 0,1,3,5 empty
+    key = np.array([0, 1, 2, 3, 4, 5], np.int32)
+    ptr = oracle.scan_col_lengths(cp, key)
+    np.testing.assert_array_equal(ptr, [0, 0, 0, 3, 3, 5, 5])
+    ptr = oracle.scan_col_lengths(cp, np.array([4, 2], np.int32))
+    np.testing.assert_array_equal(ptr, [0, 2, 5])
+
+
+# ---------------------------------------------------------------- BFS
+@pytest.mark.parametrize("seed", range(4))
+def test_bfs_matches_scipy_shortest_path(seed):
+    A = rand_csr(300, 300, 0.008, seed, values="ones")
+    A = ((A + A.T) > 0).astype(np.float32).tocsc()
+    A.sort_indices()
+    src = seed * 17
+    lev, nl = oracle.bfs(300, A.indptr, A.indices, src)
+    d = shortest_path(A, unweighted=True, indices=src, directed=True)
+    ref = np.where(np.isinf(d), -1, d).astype(np.int32)
+    # BFS over column u visits the rows of column u: that is A^T; A is symmetric
+    np.testing.assert_array_equal(lev, ref)
+    assert nl == ref.max() + 1
+
+
+def test_bfs_levels_invariants_directed():
+    A = rand_csr(200, 200, 0.02, 9, values="ones").tocsc()
+    A.sort_indices()
+    lev, _ = oracle.bfs(200, A.indptr, A.indices, 5)
+    assert lev[5] == 0
+    for u in range(200):
+        if lev[u] < 0:
+            continue
+        for v in A.indices[A.indptr[u]:A.indptr[u + 1]]:
+            assert lev[v] >= 0 and lev[v] <= lev[u] + 1
+
+
+# ----------------------------------------------------------- features
+def test_mat_features_regular_closed_form():
+    """d-regular rows: std 0, equality 1, and gini 1/m, because S2 omits the
+    largest degree (spmspv.h:2318-2324): S1 = d*m(m-1)/2, S2 = d*(m-1)."""
+    m, d = 64, 5
+    rp = np.arange(0, (m + 1) * d, d, dtype=np.int32)
+    f = oracle.mat_features(m, 100, rp)
+    assert f[0] == d and f[1] == d and f[2] == d
+    assert f[3] == 0.0 and f[4] == 0.0
+    assert abs(f[5] - 1.0) < 1e-12
+    assert abs(f[6] - 1.0 / m) < 1e-12
+    assert oracle.std_row_trunc(m, rp) == 0.0
+
+
+def test_mat_features_hand_example():
+    # degrees 3,0,2,1 -> sorted 0,1,2,3; nnz 6, m 4, n 10
+    rp = np.array([0, 3, 3, 5, 6], np.int32)
+    f = oracle.mat_features(4, 10, rp)
+    avg = 1.5
+    std = np.sqrt(((3 - avg) ** 2 + avg ** 2 + 0.25 + 0.25) / 4)
+    eq = -sum(p * np.log(p) for p in (3 / 6, 2 / 6, 1 / 6)) / np.log(4)
+    s1 = 1 * 1 + 2 * 2 + 3 * 3
+    s2 = 0 + 1 + 2
+    gini = abs(2 * s1 / (4 * s2) - 5 / 4)
+    np.testing.assert_allclose(f, [3, 0, 1.5, 0.3, std, eq, gini], rtol=1e-12)
+    # int-truncated mean 1: (4 + 1 + 1 + 0)/4
+    assert abs(oracle.std_row_trunc(4, rp) - np.sqrt(6 / 4)) < 1e-12
+
+
+def test_vec_features_and_gm():
+    cp = np.array([0, 4, 4, 9, 10], np.int32)
+    out = oracle.vec_features(cp, np.array([0, 1, 3], np.int32))
+    np.testing.assert_array_equal(out, [5, 4, 0])
+    out = oracle.vec_features(cp, np.zeros(0, np.int32))
+    assert out[0] == 0 and out[1] == 0 and out[2] == 2**31 - 1
+    g = oracle.gm(100, 1000, 10, 50)
+    # GM1 = 4 xnnz + 4 bin + 4 bin + 4 xnnz + 4 xnnz + 4 bin
+    assert g[0] == 4 * 10 * 3 + 4 * 50 * 3
+    # GM2 = 4(m+1) + 4 bin + 4 nnz + 4 bin + 4 m
+    assert g[1] == 4 * 101 + 4 * 50 + 4 * 1000 + 4 * 50 + 4 * 100
+    # GM3 = 4(m+1) + 3*4 nnz + 4 m
+    assert g[2] == 4 * 101 + 12 * 1000 + 4 * 100
+
+
+# ---------------------------------------------------------------- MLP
+def test_mlp_gradient_finite_differences():
+    F, H1, H2, C, B = 16, 64, 64, 8, 5
+    rng = np.random.default_rng(0)
+    npar = oracle.mlp_param_count(F, H1, H2, C)
+    p = rng.normal(0, 0.3, npar)
+    X = rng.normal(0, 1, (B, F))
+    lab = rng.integers(0, C, B).astype(np.int32)
+    loss, g = oracle.mlp_loss_grad(F, H1, H2, C, p, X, labels=lab)
+    idx = rng.choice(npar, 60, replace=False)
+    eps = 1e-6
+    for i in idx:
+        q = p.copy()
+        q[i] += eps
+        lp, _ = oracle.mlp_loss_grad(F, H1, H2, C, q, X, labels=lab, want_grad=False)
+        q[i] -= 2 * eps
+        lm, _ = oracle.mlp_loss_grad(F, H1, H2, C, q, X, labels=lab, want_grad=False)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+    # soft targets equal to one-hot give the same loss
+    soft = np.eye(C)[lab]
+    l2, _ = oracle.mlp_loss_grad(F, H1, H2, C, p, X, soft=soft, want_grad=False)
+    assert abs(l2 - loss) < 1e-12
+    # uniform logits: loss = ln C
+    z = np.zeros(npar)
+    l0, _ = oracle.mlp_loss_grad(F, H1, H2, C, z, X, labels=lab, want_grad=False)
+    assert abs(l0 - np.log(C)) < 1e-12
