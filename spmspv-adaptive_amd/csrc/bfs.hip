@@ -1,0 +1,533 @@
+// Level-synchronous BFS on the boolean (OR, AND) semiring, one SpMSpV per
+// level with the frontier as sparse x. The reference has no BFS loop: it
+// replays frontier files recorded elsewhere (la/test/test_application.cu:
+// 207-261, frontier format readsvmdata.hpp:308-348), so this driver is new.
+//
+// A level runs the boolean form of the slot chosen for it:
+//   K0/K2  top-down: frontier vertices scan their columns and claim
+//          unvisited rows (naive team per vertex / merge-path balanced);
+//   K1/K3  top-down gather of all neighbours, in-house radix sort + unique,
+//          then claim (the next frontier comes out sorted);
+//   K4     bottom-up: each unvisited row scans its neighbours and stops at
+//          the first frontier member;
+//   K5/K7  merge-path row tiles testing the frontier bit array, then claim
+//          (on booleans the masked and unmasked row kernels coincide);
+//   K6     naive row scan without early exit, then claim.
+// Levels are int32 and unique, so any order of claims gives the same
+// levels. The selector for level L+1 runs in the same stream right after
+// level L (features of the new frontier + MLP, one launch), so a level
+// costs one host synchronisation; with graphs on, each (slot, parity) level
+// sequence is captured once as a hipGraph and replayed. Every kernel reads
+// the frontier size from device memory, which is what makes the capture
+// valid for any level.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+
+#include "handle.hpp"
+#include "selector.hpp"
+
+using namespace spmspv;
+
+struct BfsState {
+  unsigned* visited = nullptr;              // n bits
+  unsigned* fbits[2] = {nullptr, nullptr};  // frontier bit arrays
+  int* front[2] = {nullptr, nullptr};       // frontier lists
+  int* cnt = nullptr;                       // [0],[1] sizes, [2] level
+  unsigned long long* reached = nullptr;    // [0] vertices, [1] degree sum
+  float* ybuf = nullptr;                    // m, scratch output of row kernels
+  float* ones = nullptr;                    // n ones: x values of a frontier
+  void* alloc = nullptr;
+  hipStream_t own_stream = nullptr;
+  int* levels_bound = nullptr;              // levels array the graphs were captured with
+  int bound_mode = -9, bound_fixed = -9;    // selector settings captured in the graphs
+  hipGraphExec_t graph[8][2] = {};
+};
+
+namespace {
+
+constexpr int kBfsGrid = 2048;
+
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+  const uint64_t b = __ballot(pred);
+  if (!b) return -1;
+  const int leader = __builtin_ctzll(b);
+  int base = 0;
+  if (lane() == leader) base = atomicAdd(counter, __popcll(b));
+  base = __shfl(base, leader, kWave);
+  const uint64_t lt = lane() == 0 ? 0ull : (~0ull >> (kWave - lane()));
+  return pred ? base + __popcll(b & lt) : -1;
+}
+
+__device__ __forceinline__ bool try_claim(unsigned* visited, int v) {
+  const unsigned bit = 1u << (v & 31);
+  if (__hip_atomic_load(visited + (v >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)
+    return false;
+  return !(atomicOr(visited + (v >> 5), bit) & bit);
+}
+
+// Called by every lane of a wave (convergent): appends the claimed vertices.
+__device__ __forceinline__ void record(int v, bool claimed, int level, int* levels, int* nf,
+                                       int* ncnt, unsigned* nbits) {
+  const int pos = wave_append(ncnt, claimed);
+  if (claimed) {
+    levels[v] = level;
+    nf[pos] = v;
+    atomicOr(nbits + (v >> 5), 1u << (v & 31));
+  }
+}
+
+__global__ void k_bfs_init(int* levels, int n, unsigned* visited, unsigned* fb0, unsigned* fb1,
+                           int* front0, int* cnt, unsigned long long* reached, int src) {
+  const int words = (n + 31) >> 5;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    levels[i] = i == src ? 0 : -1;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x) {
+    const unsigned b = (w == (src >> 5)) ? (1u << (src & 31)) : 0u;
+    visited[w] = b;
+    fb0[w] = b;
+    fb1[w] = 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    front0[0] = src;
+    cnt[0] = 1;
+    cnt[1] = 0;
+    cnt[2] = 0;
+    reached[0] = 0;
+    reached[1] = 0;
+  }
+}
+
+__global__ void k_bfs_begin(int* cnt, int nxt, unsigned* nbits, int words) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt[nxt] = 0;
+    cnt[2] += 1;
+  }
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x)
+    nbits[w] = 0u;
+}
+
+// K0: top-down, a team of 2^lg lanes per frontier vertex; loops are
+// wave-uniform so that wave_append stays convergent.
+__global__ __launch_bounds__(256) void k_td_naive(const int* __restrict__ colptr,
+                                                  const int* __restrict__ crow,
+                                                  const int* __restrict__ f, int* cnt, int cur,
+                                                  int lg, unsigned* visited, int* levels, int* nf,
+                                                  unsigned* nbits) {
+  const int fsz = cnt[cur];
+  const int level = cnt[2];
+  int* ncnt = cnt + (cur ^ 1);
+  const int team = 1 << lg, tpw = kWave >> lg;
+  const int tl = threadIdx.x & (team - 1);
+  const int tin = (threadIdx.x & (kWave - 1)) >> lg;
+  const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long base = gw * tpw; base < fsz; base += nw * tpw) {
+    const long long e = base + tin;
+    int b = 0, len = 0;
+    if (e < fsz) {
+      const int u = f[e];
+      b = colptr[u];
+      len = colptr[u + 1] - b;
+    }
+    const int mx = wave_max(len);
+    for (int k0 = 0; k0 < mx; k0 += team) {
+      const int k = k0 + tl;
+      bool cl = false;
+      int v = 0;
+      if (k < len) {
+        v = crow[b + k];
+        cl = try_claim(visited, v);
+      }
+      record(v, cl, level, levels, nf, ncnt, nbits);
+    }
+  }
+}
+
+// K2: top-down over merge-path tiles of (frontier entries + edges), using
+// the splits k_prep wrote for the frontier.
+template <int NT, int VT>
+__global__ __launch_bounds__(NT) void k_td_lb(const int* __restrict__ ptr,
+                                              const int* __restrict__ xbeg,
+                                              const int* __restrict__ split,
+                                              const int* __restrict__ crow, const Ctl* ctl,
+                                              int* cnt, int cur, unsigned* visited, int* levels,
+                                              int* nf, unsigned* nbits) {
+  constexpr int T = NT * VT;
+  __shared__ int s_ptr[T + 2];
+  __shared__ int s_beg[T + 2];
+  __shared__ int s_seg[T];
+  const int xnnz = cnt[cur];
+  const int level = cnt[2];
+  int* ncnt = cnt + (cur ^ 1);
+  const long long ntiles = ctl->n_tiles;
+  const long long W = (long long)xnnz + ctl->bin_len;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long d0 = t * T, d1 = min(d0 + T, W);
+    const int a0 = split[t], a1 = split[t + 1];
+    const long long b0 = d0 - a0, b1 = d1 - a1;
+    const int sa = a0 > 0 ? a0 - 1 : 0;
+    const int nseg = a1 - sa + 1;
+    for (int k = threadIdx.x; k < nseg; k += NT) {
+      s_ptr[k] = ptr[sa + k];
+      if (sa + k < xnnz) s_beg[k] = xbeg[sa + k];
+    }
+    __syncthreads();
+    long long D = d0 + (long long)threadIdx.x * VT;
+    if (D < d1) {
+      int lo = a0, hi = a1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)mid + s_ptr[mid - sa] < D) lo = mid + 1;
+        else hi = mid;
+      }
+      int i = lo;
+      long long p = D - i;
+      for (int k = 0; k < VT && D < d1; ++k, ++D) {
+        if (i < a1 && (long long)i + s_ptr[i - sa] == D) ++i;
+        else s_seg[(p++) - b0] = i - 1 - sa;
+      }
+    }
+    __syncthreads();
+    const int np = (int)(b1 - b0);
+    for (int q0 = 0; q0 < np; q0 += NT) {
+      const int q = q0 + threadIdx.x;
+      bool cl = false;
+      int v = 0;
+      if (q < np) {
+        const int sg = s_seg[q];
+        v = crow[s_beg[sg] + (int)(b0 + q - s_ptr[sg])];
+        cl = try_claim(visited, v);
+      }
+      record(v, cl, level, levels, nf, ncnt, nbits);
+    }
+    __syncthreads();
+  }
+}
+
+// After the sort slots: claims the unique neighbour keys.
+__global__ void k_claim_list(const int* __restrict__ keys, const Ctl* ctl, int* cnt, int cur,
+                             unsigned* visited, int* levels, int* nf, unsigned* nbits) {
+  const int k = ctl->ynnz;
+  const int level = cnt[2];
+  int* ncnt = cnt + (cur ^ 1);
+  for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < k; i0 += (long long)gridDim.x * blockDim.x) {
+    const long long i = i0 + threadIdx.x;
+    bool cl = false;
+    int v = 0;
+    if (i < k) {
+      v = keys[i];
+      cl = try_claim(visited, v);
+    }
+    record(v, cl, level, levels, nf, ncnt, nbits);
+  }
+}
+
+// K4: bottom-up with early exit, a team per unvisited row.
+__global__ __launch_bounds__(256) void k_bu_naive(const int* __restrict__ row_ptr,
+                                                  const int* __restrict__ col, int m,
+                                                  const unsigned* __restrict__ fbits, int* cnt,
+                                                  int cur, int lg, unsigned* visited, int* levels,
+                                                  int* nf, unsigned* nbits) {
+  const int level = cnt[2];
+  int* ncnt = cnt + (cur ^ 1);
+  const int team = 1 << lg, tpw = kWave >> lg;
+  const int tl = threadIdx.x & (team - 1);
+  const int tin = (threadIdx.x & (kWave - 1)) >> lg;
+  const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long base = gw * tpw; base < m; base += nw * tpw) {
+    const long long r = base + tin;
+    bool active = r < m;
+    if (active) active = !((visited[r >> 5] >> (r & 31)) & 1u);
+    int b = 0, len = 0;
+    if (active) {
+      b = row_ptr[r];
+      len = row_ptr[r + 1] - b;
+    }
+    const int mx = wave_max(len);
+    bool found = false;
+    for (int k0 = 0; k0 < mx; k0 += team) {
+      const int k = k0 + tl;
+      int h = 0;
+      if (k < len && !found) {
+        const int u = col[b + k];
+        h = (fbits[u >> 5] >> (u & 31)) & 1u;
+      }
+      for (int o = team >> 1; o > 0; o >>= 1) h |= __shfl_xor(h, o, kWave);
+      found = found || h;
+      if (__ballot(!found && k0 + team < len) == 0ull) break;
+    }
+    const bool cl = active && found && tl == 0;
+    if (cl) atomicOr(visited + (r >> 5), 1u << (r & 31));
+    record((int)r, cl, level, levels, nf, ncnt, nbits);
+  }
+}
+
+// Claims rows whose hit bit a row kernel set (hits & ~visited); clears hits.
+__global__ void k_claim_bits(unsigned* hits, int m, int* cnt, int cur, unsigned* visited,
+                             int* levels, int* nf, unsigned* nbits) {
+  const int level = cnt[2];
+  int* ncnt = cnt + (cur ^ 1);
+  const int words = (m + 31) >> 5;
+  for (long long w0 = (long long)blockIdx.x * blockDim.x; w0 < words; w0 += (long long)gridDim.x * blockDim.x) {
+    const long long w = w0 + threadIdx.x;
+    unsigned nw = 0;
+    if (w < words) {
+      const unsigned h = hits[w];
+      if (h) {
+        hits[w] = 0u;
+        nw = h & ~visited[w];
+        if (nw) {
+          visited[w] |= nw;  // one thread per word: no race
+          nbits[w] |= nw;
+        }
+      }
+    }
+    const int c = __popc(nw);
+    const int inc = wave_incl_sum(c);
+    const int tot = __shfl(inc, kWave - 1, kWave);
+    int base = 0;
+    if (lane() == kWave - 1 && tot) base = atomicAdd(ncnt, tot);
+    base = __shfl(base, kWave - 1, kWave);
+    int pos = base + inc - c;
+    while (nw) {
+      const int bit = __ffs(nw) - 1;
+      nw &= nw - 1;
+      const int v = (int)(w << 5) + bit;
+      levels[v] = level;
+      nf[pos++] = v;
+    }
+  }
+}
+
+// Graph500 edge count: sum of degrees over reached vertices.
+__global__ void k_reached(const int* __restrict__ levels, const int* __restrict__ colptr, int n,
+                          unsigned long long* out) {
+  unsigned long long c = 0, d = 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    if (levels[v] >= 0) {
+      ++c;
+      d += (unsigned long long)(colptr[v + 1] - colptr[v]);
+    }
+  c = wave_sum(c);
+  d = wave_sum(d);
+  if (lane() == 0) {
+    atomicAdd(out, c);
+    atomicAdd(out + 1, d);
+  }
+}
+
+int lg2(int t) {
+  int l = 0;
+  while ((1 << l) < t && l < 6) ++l;
+  return l;
+}
+
+int bfs_alloc(spmspv_handle_t h) {
+  if (h->bfs) return SPMSPV_SUCCESS;
+  auto* b = new BfsState();
+  const size_t n = h->n, m = h->m, words = (std::max(n, m) + 31) / 32 + 2;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = (off + bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_vis = take(words * 4), o_fb0 = take(words * 4), o_fb1 = take(words * 4);
+  const size_t o_f0 = take(n * 4 + 4), o_f1 = take(n * 4 + 4), o_cnt = take(64), o_r = take(64);
+  const size_t o_y = take(m * 4 + 4), o_ones = take(n * 4 + 4);
+  if (hipMalloc(&b->alloc, off) != hipSuccess) {
+    delete b;
+    return set_error(h, SPMSPV_OUT_OF_MEMORY, "bfs buffers");
+  }
+  char* p = static_cast<char*>(b->alloc);
+  b->visited = reinterpret_cast<unsigned*>(p + o_vis);
+  b->fbits[0] = reinterpret_cast<unsigned*>(p + o_fb0);
+  b->fbits[1] = reinterpret_cast<unsigned*>(p + o_fb1);
+  b->front[0] = reinterpret_cast<int*>(p + o_f0);
+  b->front[1] = reinterpret_cast<int*>(p + o_f1);
+  b->cnt = reinterpret_cast<int*>(p + o_cnt);
+  b->reached = reinterpret_cast<unsigned long long*>(p + o_r);
+  b->ybuf = reinterpret_cast<float*>(p + o_y);
+  b->ones = reinterpret_cast<float*>(p + o_ones);
+  if (!h->stream) {
+    if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
+      hipFree(b->alloc);
+      delete b;
+      return set_error(h, SPMSPV_DEVICE_ERROR, "bfs stream");
+    }
+  }
+  hipStream_t s = h->stream ? h->stream : b->own_stream;
+  launch_fill_f32(s, b->ones, (long long)n, 1.0f);
+  launch_fill_f32(s, b->ybuf, (long long)m, 0.0f);
+  if (hipStreamSynchronize(s) != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, "bfs init");
+  h->bfs = b;
+  return SPMSPV_SUCCESS;
+}
+
+// Enqueues one level on stream s: begin, the slot's kernels on frontier
+// `cur`, the selector on the new frontier, and the readback of
+// (new frontier size, next slot) into pinned host memory.
+void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* levels,
+                   const SelectorArgs& sel) {
+  BfsState* b = h->bfs;
+  const int nxt = cur ^ 1;
+  const int words = (h->n + 31) / 32;
+  int* f = b->front[cur];
+  int* nf = b->front[nxt];
+  unsigned* nbits = b->fbits[nxt];
+  hipLaunchKernelGGL(k_bfs_begin, dim3(std::min(1024, (words + 255) / 256)), dim3(256), 0, s, b->cnt,
+                     nxt, nbits, words);
+  switch (slot) {
+    case SPMSPV_K_COL_NAIVE_ATOMIC:
+      hipLaunchKernelGGL(k_td_naive, dim3(kBfsGrid), dim3(256), 0, s, h->csc_colptr, h->csc_row, f,
+                         b->cnt, cur, lg2(h->team_col), b->visited, levels, nf, nbits);
+      break;
+    case SPMSPV_K_COL_LB_ATOMIC:
+      hipLaunchKernelGGL((k_td_lb<kLbNT, kLbVT>), dim3(kBfsGrid), dim3(kLbNT), 0, s, h->ptr, h->xbeg,
+                         h->split, h->csc_row, h->ctl, b->cnt, cur, b->visited, levels, nf, nbits);
+      break;
+    case SPMSPV_K_COL_NAIVE_SORT:
+    case SPMSPV_K_COL_LB_SORT: {
+      int* bk = reinterpret_cast<int*>(h->sb.keyA);
+      if (slot == SPMSPV_K_COL_NAIVE_SORT)
+        launch_naive_col(s, 1, h->csc_colptr, h->csc_row, h->csc_val, f, b->ones, h->ptr, h->n, 0,
+                         b->cnt + cur, h->team_col, nullptr, nullptr, bk, h->sb.valA);
+      else
+        launch_lb_gather(s, 1, h->ptr, h->xbeg, b->ones, h->split, h->csc_row, h->csc_val, h->ctl, 0,
+                         b->cnt + cur, nullptr, nullptr, bk, h->sb.valA, -1);
+      launch_sort_reduce(s, h->sb, h->m, h->ctl, h->yk_tmp, h->yv_tmp, -1);
+      hipLaunchKernelGGL(k_claim_list, dim3(kBfsGrid), dim3(256), 0, s, h->yk_tmp, h->ctl, b->cnt, cur,
+                         b->visited, levels, nf, nbits);
+      break;
+    }
+    case SPMSPV_K_ROW_NAIVE:
+      hipLaunchKernelGGL(k_bu_naive, dim3(kBfsGrid), dim3(256), 0, s, h->csr_row, h->csr_col, h->m,
+                         b->fbits[cur], b->cnt, cur, lg2(h->team_row), b->visited, levels, nf, nbits);
+      break;
+    default: {  // 5, 6, 7: row kernels with hit tracking, then claim
+      if (slot == SPMSPV_K_SPMV_NAIVE)
+        launch_row_naive(s, true, true, h->csr_row, h->csr_col, h->csr_val, b->ones, b->fbits[cur],
+                         h->m, h->team_row, b->ybuf, h->bitmap);
+      else
+        launch_row_merge(s, true, true, h->csr_row, h->csr_col, h->csr_val, b->ones, b->fbits[cur],
+                         h->m, h->nnz, h->row_split, h->n_row_tiles, b->ybuf, h->bitmap,
+                         h->row_carry_val, h->row_carry_row, h->row_carry_hit);
+      hipLaunchKernelGGL(k_claim_bits, dim3(std::min(kBfsGrid, (words + 255) / 256)), dim3(256), 0, s,
+                         h->bitmap, h->m, b->cnt, cur, b->visited, levels, nf, nbits);
+      break;
+    }
+  }
+  launch_prep(s, h->csc_colptr, nf, h->n, 0, b->cnt + nxt, h->ptr, h->xbeg, h->split, h->scan_state,
+              h->tile_mm, h->ctl, &sel, 0);
+  (void)hipMemcpyAsync(h->h_pin + 16, b->cnt + nxt, sizeof(int), hipMemcpyDeviceToHost, s);
+  (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spmspv_bfs_release(spmspv_handle_t h) {
+  if (!h || !h->bfs) return SPMSPV_SUCCESS;
+  for (auto& g : h->bfs->graph)
+    for (auto& e : g)
+      if (e) (void)hipGraphExecDestroy(e);
+  if (h->bfs->own_stream) (void)hipStreamDestroy(h->bfs->own_stream);
+  if (h->bfs->alloc) (void)hipFree(h->bfs->alloc);
+  delete h->bfs;
+  h->bfs = nullptr;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
+               double stats[4]) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (src < 0 || src >= h->n || !d_levels || policy < -1 || policy >= SPMSPV_NUM_SLOTS)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "bfs: bad source, levels or policy");
+  if (h->m != h->n) return set_error(h, SPMSPV_INVALID_ARGUMENT, "bfs needs a square matrix");
+  if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "bfs needs CSR input");
+  int rc;
+  if ((rc = ensure_arena(h))) return rc;
+  if (!h->csc_colptr && (rc = spmspv_to_csc(h))) return rc;
+  if ((rc = ensure_row_tiles(h)) || (rc = ensure_mat_features(h)) || (rc = bfs_alloc(h))) return rc;
+  BfsState* b = h->bfs;
+  hipStream_t s = h->stream ? h->stream : b->own_stream;
+  const int smode = policy >= 0 ? 2 : h->sel_mode, sfixed = policy >= 0 ? policy : h->sel_fixed;
+  if (use_graphs && (b->levels_bound != d_levels || b->bound_mode != smode || b->bound_fixed != sfixed)) {
+    for (auto& g : b->graph)
+      for (auto& e : g)
+        if (e) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+    b->levels_bound = d_levels;
+    b->bound_mode = smode;
+    b->bound_fixed = sfixed;
+  }
+  // level L's kernels are enqueued on s, which the handle's own launches
+  // (prep, sort) must share
+  hipStream_t saved = h->stream;
+  h->stream = s;
+  SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, smode, sfixed};
+  const auto t0 = std::chrono::steady_clock::now();
+  hipLaunchKernelGGL(k_bfs_init, dim3(std::min(4096, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,
+                     h->n, b->visited, b->fbits[0], b->fbits[1], b->front[0], b->cnt, b->reached, src);
+  launch_prep(s, h->csc_colptr, b->front[0], h->n, 0, b->cnt, h->ptr, h->xbeg, h->split,
+              h->scan_state, h->tile_mm, h->ctl, &sel, 0);
+  (void)hipMemcpyAsync(h->h_pin + 16, b->cnt, sizeof(int), hipMemcpyDeviceToHost, s);
+  (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
+  hipError_t e = hipStreamSynchronize(s);
+  int cur = 0, nlev = 1;
+  int slot_hist[8] = {0};
+  while (e == hipSuccess && h->h_pin[16] > 0) {
+    int slot = policy >= 0 ? policy : h->h_pin[17];
+    if (slot < 0 || slot >= SPMSPV_NUM_SLOTS) slot = SPMSPV_K_COL_LB_ATOMIC;
+    slot_hist[slot]++;
+    if (use_graphs) {
+      hipGraphExec_t& ge = b->graph[slot][cur];
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) break;
+        enqueue_level(h, s, slot, cur, d_levels, sel);
+        e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) break;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          ge = nullptr;
+          break;
+        }
+      }
+      e = hipGraphLaunch(ge, s);
+    } else {
+      enqueue_level(h, s, slot, cur, d_levels, sel);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cur ^= 1;
+    ++nlev;
+    if (nlev > h->n + 1) break;  // cannot happen on a correct run; bounds the loop
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  h->stream = saved;
+  if (e != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, std::string("bfs: ") + hipGetErrorString(e));
+  hipLaunchKernelGGL(k_reached, dim3(std::min(1024, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,
+                     h->csc_colptr, h->n, b->reached);
+  unsigned long long red[2] = {0, 0};
+  e = hipMemcpyAsync(red, b->reached, sizeof(red), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, std::string("bfs: ") + hipGetErrorString(e));
+  if (stats) {
+    stats[0] = nlev - 1;             // levels with a nonempty frontier
+    stats[1] = (double)red[1] / 2.0; // undirected edges in the reached component
+    stats[2] = secs;
+    stats[3] = (double)red[0];
+  }
+  return SPMSPV_SUCCESS;
+}
+
+}  // extern "C"

@@ -232,6 +232,8 @@ def test_conversions(rmat12):
     back = sa.DeviceArray(n, np.float32)
     M.A.sparse2dense_outer(cnt, ok, ov, n, back)
     np.testing.assert_array_equal(back.to_host(), ref)
+    # unbind the borrowed dense buffer before it is freed
+    M.A.set_x(None)
 
 
 def test_selector_tree_mode_and_run_auto(rmat12):
