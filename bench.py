@@ -39,6 +39,37 @@ def alg_bytes(xnnz, bin_len, ynnz):
     return 16.0 * xnnz + 8.0 * bin_len + 8.0 * ynnz
 
 
+def profile_slot(sa, A, n, args, yk, yv):
+    """One slot at one density, cold, `reps` times (rocprofv3 runs read the
+    kernel durations and counters of exactly these launches)."""
+    if args.calib_mb:
+        nb = args.calib_mb << 20
+        src = sa.DeviceArray(nb // 4, np.float32)
+        dst = sa.DeviceArray(nb // 4, np.float32)
+        src.fill_zero()
+        sa.synchronize()
+        sa.lib().spmspv_memcpy_d2d(dst.ptr, src.ptr, nb)
+        sa.synchronize()
+    k = max(1, int(round(args.density * n)))
+    key, xv = sa.gen_sparse_vector(n, k, 0x5EED0100 + DENSITIES.index(args.density)
+                                   if args.density in DENSITIES else 0x5EED0200)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(xv, np.float32)
+    A.set_sparse_x(dk, dv, k)
+    bl = A.computeBinlen()
+    ms = []
+    ynnz = 0
+    for _ in range(args.reps):
+        A.set_sparse_x(dk, dv, k)
+        A.flush_cache(args.flush_mb << 20)
+        A.timer_start()
+        ynnz = A.run(args.only_slot, y_key=yk, y_val=yv)
+        ms.append(A.timer_stop())
+    print(json.dumps({"slot": sa.KERNEL_NAMES[args.only_slot], "density": args.density, "xnnz": k,
+                      "bin_len": int(bl), "ynnz": int(ynnz), "ms": ms,
+                      "alg_bytes": alg_bytes(k, bl, ynnz), "calib_bytes": args.calib_mb << 20}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +81,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fixed", action="store_true")
     ap.add_argument("--flush-mb", type=int, default=512)
+    ap.add_argument("--only-slot", type=int, default=-1,
+                    help="profiling mode: time one slot at --density only (for rocprofv3 runs)")
+    ap.add_argument("--density", type=float, default=1.0)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--calib-mb", type=int, default=0,
+                    help="profiling mode: first copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -79,6 +116,9 @@ def main():
         xs.append((k, sa.DeviceArray.from_host(key, np.int32), sa.DeviceArray.from_host(xv, np.float32), key, xv))
     yk = sa.DeviceArray(n, np.int32)
     yv = sa.DeviceArray(n, np.float32)
+    if args.only_slot >= 0:
+        profile_slot(sa, A, n, args, yk, yv)
+        return
     meta = []
     for (k, dk, dv, _, _) in xs:
         A.set_sparse_x(dk, dv, k)

@@ -252,6 +252,7 @@ int spmspv_free(void* d_ptr);
 int spmspv_memcpy_h2d(void* d_dst, const void* src, size_t bytes);
 int spmspv_memcpy_d2h(void* dst, const void* d_src, size_t bytes);
 int spmspv_memset(void* d_ptr, int value, size_t bytes);
+int spmspv_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
 int spmspv_synchronize(void);
 /* Stream timing: events recorded on the handle's stream. */
 int spmspv_timer_start(spmspv_handle_t h);

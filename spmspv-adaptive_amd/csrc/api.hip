@@ -67,7 +67,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
   const size_t prep_tiles = (n + kPrepTile - 1) / kPrepTile + 2;
   const size_t mwords = (m + 31) / 32;
   const size_t nwords = (n + 31) / 32;
-  const size_t cmp_tiles = (std::max(mwords, nwords) + kCmpTile - 1) / kCmpTile + 2;
+  const size_t cmp_tiles = (std::max(mwords, nwords) + kCmpNT - 1) / kCmpNT + 2;
   const size_t sort_tiles = (nnz + kSortTile - 1) / kSortTile + 2;
   const size_t red_tiles = (nnz + kRedTile - 1) / kRedTile + 2;
   const size_t row_tiles = (m + nnz + kRowTile - 1) / kRowTile + 2;
@@ -79,9 +79,12 @@ static size_t carve(spmspv_handle_t h, char* base) {
   int* xbeg = c.take<int>(n);
   int* split = c.take<int>(lb_tiles);
   int* tmm = c.take<int>(2 * prep_tiles);
-  unsigned long long* ss = c.take<unsigned long long>(std::max(prep_tiles, cmp_tiles));
+  const size_t bucket_tiles = m / 8192 + 258;
+  unsigned long long* ss =
+      c.take<unsigned long long>(std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles));
   float* acc = c.take<float>(m);
   unsigned* bm = c.take<unsigned>(mwords + 2);
+  unsigned char* mk = c.take<unsigned char>(mwords * 32 + 64);
   SortBufs sb;
   sb.keyA = c.take<unsigned>(nnz);
   sb.valA = c.take<float>(nnz);
@@ -106,6 +109,9 @@ static size_t carve(spmspv_handle_t h, char* base) {
   long long* ndev = c.take<long long>(4);
   float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
   if (base) {
+    h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
+    h->sort_words = sort_tiles * 256;
+    h->red_words = red_tiles;
     h->ctl = ctl;
     h->mat = mat;
     h->mat_partials = mp;
@@ -116,6 +122,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
     h->scan_state = ss;
     h->acc = acc;
     h->bitmap = bm;
+    h->marks = mk;
     h->sb = sb;
     h->yk_tmp = ykt;
     h->yv_tmp = yvt;
@@ -221,9 +228,34 @@ static int need_bits(spmspv_handle_t h) {
   return set_error(h, SPMSPV_VEC_FORMAT_ERR, "x is not set");
 }
 
+Ep next_ep(spmspv_handle_t h, unsigned k) {
+  // host epochs live in [1, 2^23); before they wrap, every look-back state
+  // array is cleared so that no stale word can carry a reused epoch
+  if (h->epoch_host + k >= (1u << 23) - 1) {
+    launch_fill_i32(h->stream, reinterpret_cast<int*>(h->sb.state), (long long)h->sort_words * 2, 0);
+    launch_fill_i32(h->stream, reinterpret_cast<int*>(h->sb.red_state), (long long)h->red_words * 2, 0);
+    launch_fill_i32(h->stream, reinterpret_cast<int*>(h->scan_state), (long long)h->scan_words * 2, 0);
+    h->epoch_host = 1;
+  }
+  Ep e{h->epoch_host, -1};
+  h->epoch_host += k;
+  return e;
+}
+
+int ensure_binlen_host(spmspv_handle_t h) {
+  if (h->binlen_host >= 0) return SPMSPV_SUCCESS;
+  HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->bin_len, sizeof(long long), hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  long long bl;
+  std::memcpy(&bl, h->h_pin, sizeof(bl));
+  h->binlen_host = bl;
+  return SPMSPV_SUCCESS;
+}
+
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel) {
   launch_prep(h->stream, h->csc_colptr, h->x_key, h->n, h->xnnz, nullptr, h->ptr, h->xbeg,
-              with_split ? h->split : nullptr, h->scan_state, h->tile_mm, h->ctl, sel, 0);
+              with_split ? h->split : nullptr, h->scan_state, h->tile_mm, h->ctl, sel, next_ep(h, 1));
   h->launches++;
   int rc = check_launch(h, "prep");
   if (rc) return rc;
@@ -253,13 +285,14 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       if (y_dense) launch_fill_f32(s, y_dense, m, 0.f);
       if (kid == SPMSPV_K_COL_NAIVE_ATOMIC) {
         launch_naive_col(s, 0, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, nullptr,
-                         h->n, h->xnnz, nullptr, h->team_col, acc, h->bitmap, nullptr, nullptr);
+                         h->n, h->xnnz, nullptr, h->team_col, acc, h->marks, nullptr, nullptr);
       } else {
         if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
         launch_lb_gather(s, 0, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
-                         h->xnnz, nullptr, acc, h->bitmap, nullptr, nullptr, lb_tiles_hint(h));
+                         h->xnnz, nullptr, acc, h->marks, nullptr, nullptr, lb_tiles_hint(h));
       }
-      launch_compact(s, h->bitmap, m, acc, y_dense ? 0 : 1, y_key, y_val, h->scan_state, h->ctl);
+      launch_compact(s, nullptr, h->marks, m, acc, y_dense ? 0 : 1, y_key, y_val, h->scan_state, h->ctl,
+                     next_ep(h, 1));
       h->launches += 2;
       return check_launch(h, "column atomic kernel");
     }
@@ -278,13 +311,31 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
                          h->n, h->xnnz, nullptr, h->team_col, nullptr, nullptr, bk, bv);
       int* ok = y_key ? y_key : h->yk_tmp;
       float* ov = y_val ? y_val : h->yv_tmp;
-      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, h->binlen_host);
+      if ((rc = ensure_binlen_host(h))) return rc;
+      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, h->binlen_host, h->nnz, next_ep(h, 5));
       h->launches += 6;
       if (y_dense) {
         launch_fill_f32(s, y_dense, m, 0.f);
         launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
       }
       return check_launch(h, "column sort kernel");
+    }
+    case SPMSPV_K_COL_BUCKET: {
+      if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+      if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
+      launch_lb_gather(s, 1, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
+                       h->xnnz, nullptr, nullptr, nullptr, reinterpret_cast<int*>(h->sb.keyA),
+                       h->sb.valA, lb_tiles_hint(h));
+      int* ok = y_key ? y_key : h->yk_tmp;
+      float* ov = y_val ? y_val : h->yv_tmp;
+      if ((rc = ensure_binlen_host(h))) return rc;
+      launch_bucket(s, h->sb, m, h->ctl, h->scan_state, ok, ov, h->binlen_host, h->nnz, next_ep(h, 2));
+      h->launches += 4;
+      if (y_dense) {
+        launch_fill_f32(s, y_dense, m, 0.f);
+        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
+      }
+      return check_launch(h, "bucket kernel");
     }
     case SPMSPV_K_ROW_NAIVE:
     case SPMSPV_K_ROW_LB:
@@ -309,8 +360,8 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
                          m, h->team_row, yout, h->bitmap);
       }
       if (hits)
-        launch_compact(s, h->bitmap, m, yout, yout == h->acc ? 1 : 0, y_key, y_val, h->scan_state,
-                       h->ctl);
+        launch_compact(s, h->bitmap, nullptr, m, yout, yout == h->acc ? 1 : 0, y_key, y_val,
+                       h->scan_state, h->ctl, next_ep(h, 1));
       h->launches += 2;
       return check_launch(h, "row kernel");
     }
@@ -427,7 +478,7 @@ int spmspv_to_csc(spmspv_handle_t h) {
     h->csc_owned = true;
   }
   launch_csr_to_csc_sorted(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->n, h->nnz,
-                           h->csc_colptr, h->csc_row, h->csc_val, h->sb, h->n_dev, h->ctl);
+                           h->csc_colptr, h->csc_row, h->csc_val, h->sb, h->n_dev, h->ctl, next_ep(h, 4));
   h->launches += 6;
   rc = check_launch(h, "ToCSC");
   if (rc) return rc;
@@ -586,7 +637,8 @@ int spmspv_dense2sparse(spmspv_handle_t h) {
   float* v = h->x_val ? h->x_val : h->xv_int;
   unsigned* b = h->bits();
   launch_bitmap_from_dense(h->stream, h->dense_x(), h->n, b);
-  launch_compact(h->stream, b, h->n, h->dense_x(), 0, k, v, h->scan_state, h->ctl);
+  launch_compact(h->stream, b, nullptr, h->n, h->dense_x(), 0, k, v, h->scan_state, h->ctl,
+                 next_ep(h, 1));
   h->launches++;
   rc = check_launch(h, "dense2sparse");
   if (rc) return rc;
@@ -621,7 +673,8 @@ int spmspv_dense2sparse_outer(spmspv_handle_t h, int m, float* d_dense, int* ynn
   // use the m-sized bitmap region when it fits, else the n-sized one
   unsigned* b = m <= h->m ? h->bitmap : h->bits_int;
   launch_bitmap_from_dense(h->stream, d_dense, m, b);
-  launch_compact(h->stream, b, m, d_dense, 0, d_key, d_val, h->scan_state, h->ctl);
+  launch_compact(h->stream, b, nullptr, m, d_dense, 0, d_key, d_val, h->scan_state, h->ctl,
+                 next_ep(h, 1));
   h->launches++;
   rc = check_launch(h, "dense2sparse_outer");
   if (rc) return rc;
@@ -856,6 +909,10 @@ int spmspv_memcpy_d2h(void* d, const void* s, size_t bytes) {
 }
 int spmspv_memset(void* d, int v, size_t bytes) {
   hipError_t e = hipMemset(d, v, bytes);
+  return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
+}
+int spmspv_memcpy_d2d(void* d, const void* s, size_t bytes) {
+  hipError_t e = hipMemcpy(d, s, bytes, hipMemcpyDeviceToDevice);
   return e == hipSuccess ? SPMSPV_SUCCESS : set_error(nullptr, SPMSPV_DEVICE_ERROR, hipGetErrorString(e));
 }
 int spmspv_synchronize(void) {

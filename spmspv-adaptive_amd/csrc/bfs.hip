@@ -99,10 +99,14 @@ __global__ void k_bfs_init(int* levels, int n, unsigned* visited, unsigned* fb0,
   }
 }
 
-__global__ void k_bfs_begin(int* cnt, int nxt, unsigned* nbits, int words) {
+// Opens a level: empties the next frontier, advances the level number and
+// the device epoch that the level's look-back kernels use (their launch
+// arguments are frozen inside a captured graph).
+__global__ void k_bfs_begin(int* cnt, int nxt, unsigned* nbits, int words, Ctl* ctl) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[nxt] = 0;
     cnt[2] += 1;
+    ctl->epoch += 16;
   }
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x)
     nbits[w] = 0u;
@@ -379,7 +383,7 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
   int* nf = b->front[nxt];
   unsigned* nbits = b->fbits[nxt];
   hipLaunchKernelGGL(k_bfs_begin, dim3(std::min(1024, (words + 255) / 256)), dim3(256), 0, s, b->cnt,
-                     nxt, nbits, words);
+                     nxt, nbits, words, h->ctl);
   switch (slot) {
     case SPMSPV_K_COL_NAIVE_ATOMIC:
       hipLaunchKernelGGL(k_td_naive, dim3(kBfsGrid), dim3(256), 0, s, h->csc_colptr, h->csc_row, f,
@@ -398,7 +402,7 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
       else
         launch_lb_gather(s, 1, h->ptr, h->xbeg, b->ones, h->split, h->csc_row, h->csc_val, h->ctl, 0,
                          b->cnt + cur, nullptr, nullptr, bk, h->sb.valA, -1);
-      launch_sort_reduce(s, h->sb, h->m, h->ctl, h->yk_tmp, h->yv_tmp, -1);
+      launch_sort_reduce(s, h->sb, h->m, h->ctl, h->yk_tmp, h->yv_tmp, -1, h->nnz, Ep{0, 1});
       hipLaunchKernelGGL(k_claim_list, dim3(kBfsGrid), dim3(256), 0, s, h->yk_tmp, h->ctl, b->cnt, cur,
                          b->visited, levels, nf, nbits);
       break;
@@ -421,7 +425,7 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
     }
   }
   launch_prep(s, h->csc_colptr, nf, h->n, 0, b->cnt + nxt, h->ptr, h->xbeg, h->split, h->scan_state,
-              h->tile_mm, h->ctl, &sel, 0);
+              h->tile_mm, h->ctl, &sel, Ep{0, 0});
   (void)hipMemcpyAsync(h->h_pin + 16, b->cnt + nxt, sizeof(int), hipMemcpyDeviceToHost, s);
   (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
 }
@@ -476,7 +480,7 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
   hipLaunchKernelGGL(k_bfs_init, dim3(std::min(4096, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,
                      h->n, b->visited, b->fbits[0], b->fbits[1], b->front[0], b->cnt, b->reached, src);
   launch_prep(s, h->csc_colptr, b->front[0], h->n, 0, b->cnt, h->ptr, h->xbeg, h->split,
-              h->scan_state, h->tile_mm, h->ctl, &sel, 0);
+              h->scan_state, h->tile_mm, h->ctl, &sel, next_ep(h, 1));
   (void)hipMemcpyAsync(h->h_pin + 16, b->cnt, sizeof(int), hipMemcpyDeviceToHost, s);
   (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
   hipError_t e = hipStreamSynchronize(s);
@@ -513,6 +517,15 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
     if (nlev > h->n + 1) break;  // cannot happen on a correct run; bounds the loop
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // device epochs advance 16 per level; clear the look-back states long
+  // before (ctl->epoch + offset) could wrap onto a stale word
+  h->dev_levels += nlev;
+  if (h->dev_levels > 200000) {
+    launch_fill_i32(s, reinterpret_cast<int*>(h->sb.state), (long long)h->sort_words * 2, 0);
+    launch_fill_i32(s, reinterpret_cast<int*>(h->sb.red_state), (long long)h->red_words * 2, 0);
+    launch_fill_i32(s, reinterpret_cast<int*>(h->scan_state), (long long)h->scan_words * 2, 0);
+    h->dev_levels = 0;
+  }
   h->stream = saved;
   if (e != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, std::string("bfs: ") + hipGetErrorString(e));
   hipLaunchKernelGGL(k_reached, dim3(std::min(1024, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,

@@ -24,97 +24,105 @@
 namespace spmspv {
 
 // ------------------------------------------------------------------ prep
+// Ordered tiles (tile = blockIdx.x). Each tile publishes its max/min column
+// length before its look-back aggregate; the last tile, whose look-back has
+// seen every earlier tile publish, finalises bin_len, max, min, the merge
+// tile count and the selector.
 template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
                                              const int* __restrict__ xkey, int ncols, int xnnz_arg,
                                              const int* __restrict__ xnnz_dev, int* __restrict__ ptr,
                                              int* __restrict__ xbeg, int* __restrict__ split,
                                              unsigned long long* state, int* tile_mm, Ctl* ctl,
-                                             SelectorArgs sel) {
-  __shared__ unsigned s_tile;
+                                             Ep ep, SelectorArgs sel) {
   __shared__ unsigned long long s_pre;
   __shared__ long long s_l[NT / kWave];
   __shared__ int s_i[NT / kWave];
-  __shared__ unsigned s_flag;
   const int xnnz = xnnz_dev ? *xnnz_dev : xnnz_arg;
   const unsigned ntiles = (unsigned)((xnnz + NT * IPT - 1) / (NT * IPT));
-  const unsigned epoch = ctl->epoch;
-  while (true) {
-    const unsigned t = next_tile(ctl, kSlotPrep, &s_tile);
-    if (t >= ntiles) break;
-    const long long base = (long long)t * (NT * IPT) + (long long)threadIdx.x * IPT;
-    int len[IPT], beg[IPT];
-    long long tsum = 0;
-    int mx = 0, mn = INT_MAX;
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const long long i = base + j;
-      len[j] = 0;
-      beg[j] = 0;
-      if (i < xnnz) {
-        const int c = xkey[i];
-        // an out-of-range key contributes nothing instead of a wild read
-        const bool ok = (unsigned)c < (unsigned)ncols;
-        const int b = ok ? colptr[c] : 0;
-        const int e = ok ? colptr[c + 1] : 0;
-        beg[j] = b;
-        len[j] = e - b;
-        mx = max(mx, len[j]);
-        mn = min(mn, len[j]);
-      }
-      tsum += len[j];
-    }
-    long long btot;
-    const long long bex = block_excl_sum<NT, long long>(tsum, s_l, &btot);
-    const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
-    long long run = (long long)pre + bex;
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const long long i = base + j;
-      if (i < xnnz) {
-        ptr[i] = (int)run;
-        xbeg[i] = beg[j];
-        if (split) {
-          const long long Mi = i + run;
-          const long long Mn = Mi + len[j] + 1;
-          for (long long d = (Mi / kLbTile + 1) * kLbTile; d <= Mn; d += kLbTile)
-            split[d / kLbTile] = (int)(i + 1);
-        }
-        run += len[j];
-        if (i == xnnz - 1) {
-          ptr[xnnz] = (int)run;
-          ctl->bin_len = run;
-        }
-      }
-    }
-    mx = block_max<NT, int>(mx, s_i);
-    mn = block_min<NT, int>(mn, s_i);
+  const unsigned t = blockIdx.x;
+  if (ntiles == 0) {
+    // empty x: block 0 alone writes the results
+    if (t != 0) return;
     if (threadIdx.x == 0) {
-      tile_mm[2 * t] = mx;
-      tile_mm[2 * t + 1] = mn;
+      ptr[0] = 0;
+      ctl->bin_len = 0;
+      ctl->max_len = 0;
+      ctl->min_len = INT_MAX;
+      ctl->xnnz = 0;
+      ctl->n_tiles = 0;
+      if (split) split[0] = 0;
     }
+    __syncthreads();
+    if (sel.mat) select_in_block<NT>(ctl, sel);
+    return;
   }
-  const bool last = finish_persistent(ctl, kSlotPrep, &s_flag);
-  if (!last) return;
-  // Final reductions by the last workgroup; every other workgroup's writes
-  // are visible here (finish_persistent fences).
+  if (t >= ntiles) return;
+  const unsigned epoch = ep_of(ctl, ep);
+  const long long base = (long long)t * (NT * IPT) + (long long)threadIdx.x * IPT;
+  int len[IPT], beg[IPT];
+  long long tsum = 0;
   int mx = 0, mn = INT_MAX;
-  for (unsigned t = threadIdx.x; t < ntiles; t += NT) {
-    mx = max(mx, tile_mm[2 * t]);
-    mn = min(mn, tile_mm[2 * t + 1]);
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const long long i = base + j;
+    len[j] = 0;
+    beg[j] = 0;
+    if (i < xnnz) {
+      const int c = xkey[i];
+      // an out-of-range key contributes nothing instead of a wild read
+      const bool ok = (unsigned)c < (unsigned)ncols;
+      const int b = ok ? colptr[c] : 0;
+      const int e = ok ? colptr[c + 1] : 0;
+      beg[j] = b;
+      len[j] = e - b;
+      mx = max(mx, len[j]);
+      mn = min(mn, len[j]);
+    }
+    tsum += len[j];
   }
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
   if (threadIdx.x == 0) {
-    long long bl = 0;
-    if (xnnz == 0) {
-      ptr[0] = 0;
-      ctl->bin_len = 0;
-    } else {
-      bl = __hip_atomic_load(&ctl->bin_len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tile_mm[2 * t] = mx;
+    tile_mm[2 * t + 1] = mn;
+    __threadfence();  // visible before this tile's aggregate is
+  }
+  long long btot;
+  const long long bex = block_excl_sum<NT, long long>(tsum, s_l, &btot);
+  const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
+  long long run = (long long)pre + bex;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const long long i = base + j;
+    if (i < xnnz) {
+      ptr[i] = (int)run;
+      xbeg[i] = beg[j];
+      if (split) {
+        const long long Mi = i + run;
+        const long long Mn = Mi + len[j] + 1;
+        for (long long d = (Mi / kLbTile + 1) * kLbTile; d <= Mn; d += kLbTile)
+          split[d / kLbTile] = (int)(i + 1);
+      }
+      run += len[j];
+      if (i == xnnz - 1) ptr[xnnz] = (int)run;
     }
-    ctl->max_len = xnnz ? mx : 0;
-    ctl->min_len = mn;
+  }
+  if (t != ntiles - 1) return;
+  // last tile: every earlier tile has published, so its tile_mm is visible
+  __threadfence();
+  int gmx = 0, gmn = INT_MAX;
+  for (unsigned u = threadIdx.x; u < ntiles; u += NT) {
+    gmx = max(gmx, __hip_atomic_load(&tile_mm[2 * u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    gmn = min(gmn, __hip_atomic_load(&tile_mm[2 * u + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  gmx = block_max<NT, int>(gmx, s_i);
+  gmn = block_min<NT, int>(gmn, s_i);
+  if (threadIdx.x == 0) {
+    const long long bl = (long long)pre + btot;
+    ctl->bin_len = bl;
+    ctl->max_len = gmx;
+    ctl->min_len = gmn;
     ctl->xnnz = xnnz;
     const long long W = (long long)xnnz + bl;
     const long long nt = (W + kLbTile - 1) / kLbTile;
@@ -129,16 +137,15 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
 }
 
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
-                 const int* xnnz_dev,
-                 int* ptr, int* xbeg, int* split, unsigned long long* state, int* tile_mm, Ctl* ctl,
-                 const SelectorArgs* sel, int grid_hint) {
-  int grid = kPersistGrid;
-  if (!xnnz_dev) grid = max(1, min(kPersistGrid, (xnnz + kPrepTile - 1) / kPrepTile));
-  if (grid_hint > 0) grid = min(grid, grid_hint);
+                 const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
+                 int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep) {
+  // exact grid when the host knows xnnz, else the bound from ncols
+  const long long cnt = xnnz_dev ? ncols : xnnz;
+  const int grid = (int)std::max<long long>(1, (cnt + kPrepTile - 1) / kPrepTile);
   SelectorArgs a{};
   if (sel) a = *sel;
   hipLaunchKernelGGL((k_prep<kPrepNT, kPrepIPT>), dim3(grid), dim3(kPrepNT), 0, s, colptr, xkey,
-                     ncols, xnnz, xnnz_dev, ptr, xbeg, split, state, tile_mm, ctl, a);
+                     ncols, xnnz, xnnz_dev, ptr, xbeg, split, state, tile_mm, ctl, ep, a);
 }
 
 // ------------------------------------------------------------- naive col
@@ -151,8 +158,8 @@ __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr
                                                   const float* __restrict__ xval,
                                                   const int* __restrict__ ptr, int ncols, int xnnz_arg,
                                                   const int* __restrict__ xnnz_dev, int lg_team,
-                                                  float* acc, unsigned* bitmap, int* __restrict__ bkey,
-                                                  float* __restrict__ bval) {
+                                                  float* acc, unsigned char* marks,
+                                                  int* __restrict__ bkey, float* __restrict__ bval) {
   const int xnnz = xnnz_dev ? *xnnz_dev : xnnz_arg;
   const int team = 1 << lg_team;
   const int tpb = NT >> lg_team;
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr
       for (int j = b + tid_t; j < en; j += team) {
         const int r = crow[j];
         atomic_add_f32(acc + r, cval[j] * xv);
-        atomicOr(bitmap + (r >> 5), 1u << (r & 31));
+        marks[r] = 1;  // plain byte store: idempotent, no atomic needed
       }
     } else {
       const int o = ptr[e] - b;
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr
 
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
-                      const int* xnnz_dev, int team, float* acc, unsigned* bitmap, int* bkey,
+                      const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,
                       float* bval) {
   int lg = 0;
   while ((1 << lg) < team && lg < 6) ++lg;
@@ -192,10 +199,10 @@ void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* cro
   if (grid < 1) grid = 1;
   if (mode == 0)
     hipLaunchKernelGGL((k_naive_col<256, 0>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
-                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, bitmap, bkey, bval);
+                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval);
   else
     hipLaunchKernelGGL((k_naive_col<256, 1>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
-                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, bitmap, bkey, bval);
+                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval);
 }
 
 // ------------------------------------------------------------ LB gather
@@ -211,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
                                                   const float* __restrict__ cval,
                                                   const Ctl* __restrict__ ctl, int xnnz_arg,
                                                   const int* __restrict__ xnnz_dev, float* acc,
-                                                  unsigned* bitmap, int* __restrict__ bkey,
+                                                  unsigned char* marks, int* __restrict__ bkey,
                                                   float* __restrict__ bval) {
   constexpr int T = NT * VT;
   __shared__ int s_ptr[T + 2];
@@ -271,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
       const float v = cval[idx] * s_xv[sg];
       if (MODE == 0) {
         atomic_add_f32(acc + r, v);
-        atomicOr(bitmap + (r >> 5), 1u << (r & 31));
+        marks[r] = 1;
       } else {
         bkey[p] = r;
         bval[p] = v;
@@ -283,73 +290,97 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
 
 void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval,
                       const int* split, const int* crow, const float* cval, const Ctl* ctl,
-                      int xnnz, const int* xnnz_dev, float* acc, unsigned* bitmap, int* bkey,
+                      int xnnz, const int* xnnz_dev, float* acc, unsigned char* marks, int* bkey,
                       float* bval, long long tiles_hint) {
   long long grid = tiles_hint > 0 ? std::min<long long>(tiles_hint, 8192) : 2048;
   if (grid < 1) grid = 1;
   if (mode == 0)
     hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 0>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
-                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, bitmap, bkey, bval);
+                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval);
   else
     hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 1>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
-                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, bitmap, bkey, bval);
+                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval);
 }
 
 // -------------------------------------------------------------- compact
-template <int NT, int WPT>
-__global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap, int m,
-                                                float* __restrict__ acc, int reset_acc,
-                                                int* __restrict__ ykey, float* __restrict__ yval,
-                                                unsigned long long* state, Ctl* ctl) {
-  __shared__ unsigned s_tile;
-  __shared__ unsigned long long s_pre;
-  __shared__ int s_i[NT / kWave];
-  __shared__ unsigned s_flag;
-  const int nwords = (m + 31) >> 5;
-  const unsigned ntiles = (unsigned)((nwords + NT * WPT - 1) / (NT * WPT));
-  const unsigned epoch = ctl->epoch;
-  while (true) {
-    const unsigned t = next_tile(ctl, kSlotCompact, &s_tile);
-    if (t >= ntiles) break;
-    const int w0 = (int)t * (NT * WPT) + threadIdx.x * WPT;
-    unsigned wd[WPT];
-    int cnt = 0;
-#pragma unroll
-    for (int j = 0; j < WPT; ++j) {
-      wd[j] = (w0 + j < nwords) ? bitmap[w0 + j] : 0u;
-      cnt += __popc(wd[j]);
-    }
-    int btot;
-    const int bex = block_excl_sum<NT, int>(cnt, s_i, &btot);
-    const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
-    int pos = (int)pre + bex;
-#pragma unroll
-    for (int j = 0; j < WPT; ++j) {
-      unsigned b = wd[j];
-      if (!b) continue;
-      bitmap[w0 + j] = 0u;
-      while (b) {
-        const int bit = __ffs(b) - 1;
-        b &= b - 1;
-        const int r = ((w0 + j) << 5) + bit;
-        if (ykey) ykey[pos] = r;
-        if (yval) yval[pos] = acc[r];
-        if (reset_acc) acc[r] = 0.0f;
-        ++pos;
-      }
-    }
-    if (t == ntiles - 1 && threadIdx.x == 0) ctl->ynnz = (int)(pre + btot);
-  }
-  const bool last = finish_persistent(ctl, kSlotCompact, &s_flag);
-  if (last && threadIdx.x == 0 && ntiles == 0) ctl->ynnz = 0;
+// One word of 32 rows per thread, 8192 rows per tile, ordered tiles. The
+// hits come either from a bit array (row kernels) or from one mark byte per
+// row (column atomic kernels); both are cleared on the way.
+__device__ __forceinline__ unsigned bytes_to_bits(uint4 a, uint4 b) {
+  // byte k (0 or 1) of a 32-bit lane -> bit k: the four partial products of
+  // x * 0x01020408 land on distinct bits, bits 24..27 hold bytes 0..3
+  auto nib = [](unsigned x) { return (((x & 0x01010101u) * 0x01020408u) >> 24) & 0xFu; };
+  return nib(a.x) | (nib(a.y) << 4) | (nib(a.z) << 8) | (nib(a.w) << 12) | (nib(b.x) << 16) |
+         (nib(b.y) << 20) | (nib(b.z) << 24) | (nib(b.w) << 28);
 }
 
-void launch_compact(hipStream_t s, unsigned* bitmap, int m, float* acc, int reset_acc, int* ykey,
-                    float* yval, unsigned long long* state, Ctl* ctl) {
+template <int NT, bool BYTES>
+__global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
+                                                unsigned char* __restrict__ marks, int m,
+                                                float* __restrict__ acc, int reset_acc,
+                                                int* __restrict__ ykey, float* __restrict__ yval,
+                                                unsigned long long* state, Ctl* ctl, Ep ep) {
+  __shared__ unsigned long long s_pre;
+  __shared__ int s_i[NT / kWave];
+  __shared__ int s_rows[NT * 32];
   const int nwords = (m + 31) >> 5;
-  int grid = std::max(1, std::min(kPersistGrid, (nwords + kCmpTile - 1) / kCmpTile));
-  hipLaunchKernelGGL((k_compact<kCmpNT, kCmpWPT>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, m, acc,
-                     reset_acc, ykey, yval, state, ctl);
+  const unsigned ntiles = (unsigned)((nwords + NT - 1) / NT);
+  const unsigned t = blockIdx.x;
+  if (t >= ntiles) return;
+  const unsigned epoch = ep_of(ctl, ep);
+  const int w = (int)t * NT + threadIdx.x;
+  unsigned b = 0u;
+  if (w < nwords) {
+    if (BYTES) {
+      const uint4* p = reinterpret_cast<const uint4*>(marks) + 2 * (size_t)w;
+      b = bytes_to_bits(p[0], p[1]);
+    } else {
+      b = bitmap[w];
+    }
+  }
+  const int cnt = __popc(b);
+  int btot;
+  const int bex = block_excl_sum<NT, int>(cnt, s_i, &btot);
+  const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
+  // rows of the tile land in LDS at their local rank, then leave as
+  // contiguous, coalesced writes
+  int loc = bex;
+  if (b) {
+    if (BYTES) {
+      uint4* p = reinterpret_cast<uint4*>(marks) + 2 * (size_t)w;
+      p[0] = make_uint4(0u, 0u, 0u, 0u);
+      p[1] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+      bitmap[w] = 0u;
+    }
+    while (b) {
+      const int bit = __ffs(b) - 1;
+      b &= b - 1;
+      s_rows[loc++] = (w << 5) + bit;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < btot; i += NT) {
+    const int r = s_rows[i];
+    const long long o = (long long)pre + i;
+    if (ykey) ykey[o] = r;
+    if (yval) yval[o] = acc[r];
+    if (reset_acc) acc[r] = 0.0f;
+  }
+  if (t == ntiles - 1 && threadIdx.x == 0) ctl->ynnz = (int)(pre + btot);
+}
+
+void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
+                    int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
+                    Ep ep) {
+  const int nwords = (m + 31) >> 5;
+  const int grid = std::max(1, (nwords + kCmpNT - 1) / kCmpNT);
+  if (marks)
+    hipLaunchKernelGGL((k_compact<kCmpNT, true>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
+                       reset_acc, ykey, yval, state, ctl, ep);
+  else
+    hipLaunchKernelGGL((k_compact<kCmpNT, false>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
+                       reset_acc, ykey, yval, state, ctl, ep);
 }
 
 // ------------------------------------------------------------ utilities
@@ -409,10 +440,25 @@ void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, co
 __global__ void k_set_bits(const int* __restrict__ key, int nnz_arg, const int* nnz_dev, int len,
                            unsigned* bits) {
   const int nnz = nnz_dev ? *nnz_dev : nnz_arg;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
-    const int k = key[i];
-    if ((unsigned)k >= (unsigned)len) continue;
-    atomicOr(bits + (k >> 5), 1u << (k & 31));
+  // loop bound is wave-uniform so the shuffles below see every lane
+  for (int i0 = blockIdx.x * blockDim.x; i0 < nnz; i0 += gridDim.x * blockDim.x) {
+    const int i = i0 + threadIdx.x;
+    int k = -1;
+    if (i < nnz) k = key[i];
+    const bool ok = (unsigned)k < (unsigned)len;
+    const int w = ok ? (k >> 5) : -1;
+    unsigned bit = ok ? (1u << (k & 31)) : 0u;
+    // segmented OR over runs of lanes with the same word (sorted keys make
+    // long runs); the last lane of each run issues the one atomic
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int wo = __shfl_up(w, o, kWave);
+      const unsigned bo = __shfl_up(bit, o, kWave);
+      if (lane() >= o && wo == w) bit |= bo;
+    }
+    const int wn = __shfl_down(w, 1, kWave);
+    const bool last = lane() == kWave - 1 || wn != w;
+    if (ok && last) atomicOr(bits + w, bit);
   }
 }
 void launch_set_bits(hipStream_t s, const int* key, int nnz, const int* nnz_dev, int len,

@@ -188,6 +188,25 @@ __device__ __forceinline__ bool finish_persistent(Ctl* ctl, int slot, unsigned* 
   return last;
 }
 
+// Epoch of a look-back launch. Host-driven launches pass a fresh value per
+// launch (bit 23 clear); launches inside captured graphs, whose arguments
+// are frozen, take ctl->epoch + dev, which a small kernel at the start of
+// each captured sequence advances (bit 23 set). Stale states of earlier
+// launches then never match.
+struct Ep {
+  unsigned v;
+  int dev;
+};
+__device__ __forceinline__ unsigned ep_of(const Ctl* ctl, Ep e) {
+  const unsigned mask = (1u << kEpochBits) - 1;
+  return e.dev < 0 ? (e.v & mask) : (((ctl->epoch + (unsigned)e.dev) | 0x800000u) & mask);
+}
+
+// Ordered tiles: tile = blockIdx.x. Workgroups are dispatched in
+// increasing index order, so every tile a workgroup waits on in the
+// look-back below belongs to a workgroup that was dispatched earlier and
+// runs to completion without waiting on later ones.
+//
 // Decoupled look-back: given this tile's aggregate (thread 0's argument is
 // used), returns the exclusive prefix of all earlier tiles, to every thread.
 // `state` holds one word per tile. Wave 0 does the look-back, 64 tiles at a

@@ -15,8 +15,6 @@
 
 namespace spmspv {
 
-int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, bool skip_small);
 
 __global__ void k_expand_rows(const int* __restrict__ row_ptr, int m, int* __restrict__ rowid) {
   for (long long r = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < m;
@@ -56,7 +54,7 @@ __global__ void k_set_ll(long long* p, long long v) { *p = v; }
 
 void launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col, const float* val,
                               int m, int n, int nnz, int* colptr, int* crow, float* cval,
-                              const SortBufs& b, long long* n_dev, Ctl* ctl) {
+                              const SortBufs& b, long long* n_dev, Ctl* ctl, Ep ep) {
   if (nnz == 0) {
     launch_fill_i32(s, colptr, (long long)n + 1, 0);
     return;
@@ -68,7 +66,7 @@ void launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col,
   hipLaunchKernelGGL(k_keys_from_cols, dim3(g), dim3(256), 0, s, col, nnz, b.keyA,
                      reinterpret_cast<unsigned*>(b.valA));
   hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, s, n_dev, (long long)nnz);
-  const int fb = launch_radix_sort(s, b, n, n_dev, ctl, nnz, false);
+  const int fb = launch_radix_sort(s, b, n, n_dev, ctl, nnz, nnz, false, ep);
   const unsigned* sk = fb ? b.keyB : b.keyA;
   const unsigned* si = reinterpret_cast<const unsigned*>(fb ? b.valB : b.valA);
   // gather into the free key buffer, then move the rows into crow

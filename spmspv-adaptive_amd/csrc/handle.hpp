@@ -56,8 +56,10 @@ struct spmspv_handle_s {
   int* split = nullptr;      // LB tiles + 2
   int* tile_mm = nullptr;    // 2 per prep tile
   unsigned long long* scan_state = nullptr;
+  size_t scan_words = 0, sort_words = 0, red_words = 0;  // look-back state sizes
   float* acc = nullptr;      // m, all-zero between calls
   unsigned* bitmap = nullptr;  // ceil(m/32), all-zero between calls
+  unsigned char* marks = nullptr;  // one byte per row (padded to 32), all-zero between calls
   spmspv::SortBufs sb{};
   int* yk_tmp = nullptr;     // m
   float* yv_tmp = nullptr;   // m
@@ -88,6 +90,8 @@ struct spmspv_handle_s {
   int* h_pin = nullptr;       // small pinned readback area (64 ints)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   long long launches = 0;
+  unsigned epoch_host = 1;   // next host-side look-back epoch (bit 23 clear)
+  long long dev_levels = 0;  // device-epoch sequences run (captured BFS levels)
 
   BfsState* bfs = nullptr;
 
@@ -103,4 +107,8 @@ int ensure_mat_features(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
 int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);
 int readback_ynnz(spmspv_handle_t h, int* y_nnz);
+// Reserves k consecutive host epochs for the look-back kernels of one call.
+Ep next_ep(spmspv_handle_t h, unsigned k);
+// Reads bin_len of the current x back to the host (one synchronisation).
+int ensure_binlen_host(spmspv_handle_t h);
 }  // namespace spmspv

@@ -11,11 +11,13 @@ namespace spmspv {
 
 // Tile geometry (one place, so the arena sizing in the handle agrees).
 constexpr int kPrepNT = 256, kPrepIPT = 8, kPrepTile = kPrepNT * kPrepIPT;
-constexpr int kLbNT = 256, kLbVT = 8, kLbTile = kLbNT * kLbVT;
-constexpr int kCmpNT = 256, kCmpWPT = 8, kCmpTile = kCmpNT * kCmpWPT;  // bitmap words
+// odd items per thread: a lane walking VT consecutive LDS words then hits
+// a different bank than its neighbours (stride 7 is coprime with 32 banks)
+constexpr int kLbNT = 256, kLbVT = 7, kLbTile = kLbNT * kLbVT;
+constexpr int kCmpNT = 256;  // bitmap words per compaction tile
 constexpr int kSortNT = 256, kSortIPT = 12, kSortTile = kSortNT * kSortIPT;
-constexpr int kRedNT = 256, kRedIPT = 8, kRedTile = kRedNT * kRedIPT;
-constexpr int kRowNT = 256, kRowVT = 8, kRowTile = kRowNT * kRowVT;
+constexpr int kRedNT = 256, kRedIPT = 7, kRedTile = kRedNT * kRedIPT;
+constexpr int kRowNT = 256, kRowVT = 7, kRowTile = kRowNT * kRowVT;
 constexpr int kSmallSortCap = kSortTile;  // single-workgroup LDS sort + reduce
 constexpr int kPersistGrid = 1024;        // workgroups of a persistent launch
 
@@ -39,21 +41,22 @@ struct SelectorArgs {
   int fixed;
 };
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
-                 const int* xnnz_dev,
-                 int* ptr, int* xbeg, int* split, unsigned long long* state, int* tile_mm, Ctl* ctl,
-                 const SelectorArgs* sel, int grid_hint);
+                 const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
+                 int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep);
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
-                      const int* xnnz_dev, int team, float* acc, unsigned* bitmap, int* bkey,
+                      const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,
                       float* bval);
 void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval,
                       const int* split, const int* crow, const float* cval, const Ctl* ctl,
-                      int xnnz, const int* xnnz_dev, float* acc, unsigned* bitmap, int* bkey,
+                      int xnnz, const int* xnnz_dev, float* acc, unsigned char* marks, int* bkey,
                       float* bval, long long tiles_hint);
-// bitmap -> sorted (key, val); resets the bitmap words (and acc entries when
-// reset_acc) so the accumulator returns to all-zero.
-void launch_compact(hipStream_t s, unsigned* bitmap, int m, float* acc, int reset_acc, int* ykey,
-                    float* yval, unsigned long long* state, Ctl* ctl);
+// hits (bit array, or mark bytes when marks != null) -> sorted (key, val);
+// clears the hits (and acc entries when reset_acc) so the accumulator
+// returns to all-zero.
+void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
+                    int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
+                    Ep ep);
 // bitmap of (v != 0) over a dense vector (whole words, no atomics)
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
 void launch_fill_f32(hipStream_t s, float* p, long long n, float v);
@@ -73,19 +76,27 @@ struct SortBufs {
   float* valA;
   unsigned* keyB;
   float* valB;
-  unsigned* hist;       // 4*256 (zero between calls)
-  unsigned* digoff;     // 4*256
+  unsigned* hist;       // 4*256 digit counts (zeroed before each histogram)
+  unsigned* digoff;     // 4*256 (spare)
   unsigned long long* state;      // per (tile, digit)
   unsigned long long* red_state;  // per reduce tile
   float* carry_val;
   int* carry_idx;
 };
+// n_hint: item count when the host knows it (exact grids), else -1 and the
+// grids cover n_cap. ep: first of the 5 epochs the sort + reduce use.
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
-                        long long binlen_hint);
+                        long long binlen_hint, long long n_cap, Ep ep);
 // Sorts (keyA, valA)[*n_dev] by key < range; returns 0 if the result is in
 // A, 1 if in B.
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, bool skip_small);
+                      long long n_hint, long long n_cap, bool skip_small, Ep ep);
+
+// Bucket slot: partition the gathered bins by the top 8 row bits, then one
+// workgroup per (bucket, 8192-row window) accumulates in LDS and writes its
+// sorted structural support.
+void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, unsigned long long* state,
+                   int* ykey, float* yval, long long binlen_hint, long long n_cap, Ep ep);
 
 // --- row path ------------------------------------------------------------
 void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles);
@@ -100,7 +111,7 @@ void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr,
 // --- CSC construction ----------------------------------------------------
 void launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col, const float* val,
                               int m, int n, int nnz, int* colptr, int* crow, float* cval,
-                              const SortBufs& b, long long* n_dev, Ctl* ctl);
+                              const SortBufs& b, long long* n_dev, Ctl* ctl, Ep ep);
 
 // --- features ------------------------------------------------------------
 void launch_mat_features(hipStream_t s, const int* row_ptr, int m, int n, int nnz, int* hist,

@@ -124,16 +124,39 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
                                                   int* __restrict__ carry_hit) {
   constexpr int T = NT * VT;
   constexpr int NW = NT / kWave;
+  // ~16 KB of LDS per workgroup (rows are written straight to y), so that
+  // more workgroups per CU keep loads in flight
   __shared__ int s_re[T + 1];
   __shared__ float s_p[T];
-  __shared__ float s_y[T];
   __shared__ unsigned char s_ph[HITS ? T : 1];
   __shared__ unsigned char s_yh[HITS ? T : 1];
   __shared__ int s_f[NW];
   __shared__ float s_sv[NW];
   __shared__ int s_sh[NW];
   const long long W = (long long)m + nnz;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  // the tile's nonzeros are loaded one tile ahead: the loads of tile t+grid
+  // are in flight while tile t is walked, scanned and written
+  int cj[VT];
+  float vj[VT];
+  auto load_tile = [&](int t) {
+    const long long d0 = (long long)t * T;
+    const long long d1 = std::min(d0 + T, W);
+    const long long j0 = d0 - split[t];
+    const int nj = (int)(d1 - split[t + 1] - j0);
+#pragma unroll
+    for (int i = 0; i < VT; ++i) {
+      const int k = threadIdx.x + i * NT;
+      cj[i] = 0;
+      vj[i] = 0.f;
+      if (k < nj) {
+        cj[i] = __builtin_nontemporal_load(col + j0 + k);
+        vj[i] = __builtin_nontemporal_load(val + j0 + k);
+      }
+    }
+  };
+  int t = blockIdx.x;
+  if (t < ntiles) load_tile(t);
+  for (; t < ntiles; t += gridDim.x) {
     const long long d0 = (long long)t * T;
     const long long d1 = std::min(d0 + T, W);
     const int i0 = split[t];
@@ -141,27 +164,31 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
     const long long j0 = d0 - i0;
     const long long j1 = d1 - i1;
     const int nr = i1 - i0;
+    const int nj = (int)(j1 - j0);
     for (int k = threadIdx.x; k <= nr; k += NT) {
       const int r = i0 + k;
       s_re[k] = r < m ? row_ptr[r + 1] : nnz;
     }
-    const int nj = (int)(j1 - j0);
-    for (int k = threadIdx.x; k < nj; k += NT) {
-      const long long j = j0 + k;
-      const int c = col[j];
-      float p;
-      int h = 0;
-      if (MASKED) {
-        h = test_bit(xbits, c) ? 1 : 0;
-        p = h ? val[j] * x[c] : 0.f;
-      } else {
-        p = val[j] * x[c];
-        if (HITS) h = test_bit(xbits, c) ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < VT; ++i) {
+      const int k = threadIdx.x + i * NT;
+      if (k < nj) {
+        const int c = cj[i];
+        float p;
+        int h = 0;
+        if (MASKED) {
+          h = test_bit(xbits, c) ? 1 : 0;
+          p = h ? vj[i] * x[c] : 0.f;
+        } else {
+          p = vj[i] * x[c];
+          if (HITS) h = test_bit(xbits, c) ? 1 : 0;
+        }
+        s_p[k] = p;
+        if (HITS) s_ph[k] = (unsigned char)h;
       }
-      s_p[k] = p;
-      if (HITS) s_ph[k] = (unsigned char)h;
     }
     __syncthreads();
+    if (t + (int)gridDim.x < ntiles) load_tile(t + gridDim.x);
     long long D = d0 + (long long)threadIdx.x * VT;
     bool had_end = false;
     float first_sum = 0.f, sum = 0.f;
@@ -189,7 +216,7 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
             first_hit = hit;
             had_end = true;
           } else {
-            s_y[i - i0] = sum;
+            y[i] = sum;
             if (HITS) s_yh[i - i0] = (unsigned char)hit;
           }
           sum = 0.f;
@@ -203,7 +230,7 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
     int hin, tail_h;
     seg_scan_row<NT>(had_end, sum, hit, &cin, &hin, &tail_v, &tail_h, s_f, s_sv, s_sh);
     if (had_end) {
-      s_y[first_row - i0] = cin + first_sum;
+      y[first_row] = cin + first_sum;
       if (HITS) s_yh[first_row - i0] = (unsigned char)(hin | first_hit);
     }
     if (threadIdx.x == 0) {
@@ -211,12 +238,19 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
       carry_val[t] = tail_v;
       carry_hit[t] = tail_h;
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < nr; k += NT) {
-      y[i0 + k] = s_y[k];
-      if (HITS && s_yh[k]) {
-        const int r = i0 + k;
-        atomicOr(ybits + (r >> 5), 1u << (r & 31));
+    if (HITS) {
+      __syncthreads();
+      if (nr > 0) {
+        // hit bits of rows [i0, i1): a word wholly inside the range belongs
+        // to this tile alone and is stored; the two edge words are OR-ed in
+        const int w0 = i0 >> 5, w1 = (i1 - 1) >> 5;
+        for (int w = w0 + threadIdx.x; w <= w1; w += NT) {
+          const int rlo = max(i0, w << 5), rhi = min(i1, (w << 5) + 32);
+          unsigned bits = 0u;
+          for (int r = rlo; r < rhi; ++r) bits |= (unsigned)s_yh[r - i0] << (r & 31);
+          if (rlo == (w << 5) && rhi == (w << 5) + 32) ybits[w] = bits;
+          else if (bits) atomicOr(ybits + w, bits);
+        }
       }
     }
     __syncthreads();
@@ -246,7 +280,9 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                       const int* split, int ntiles, float* y, unsigned* ybits, float* carry_val,
                       int* carry_row, int* carry_hit) {
   if (ntiles <= 0) return;
-  const int grid = std::min(ntiles, 16384);
+  // about eight workgroups per CU, each walking several tiles with the next
+  // tile's loads in flight
+  const int grid = std::min(ntiles, 2048);
 #define SPMSPV_ROW_MERGE(MK, HT)                                                                 \
   hipLaunchKernelGGL((k_row_merge<kRowNT, kRowVT, MK, HT>), dim3(grid), dim3(kRowNT), 0, s,      \
                      row_ptr, col, val, x, xbits, m, nnz, split, ntiles, y, ybits, carry_val,    \
@@ -282,7 +318,30 @@ __global__ __launch_bounds__(NT) void k_row_naive(const int* __restrict__ row_pt
     const int e = row_ptr[r + 1];
     float s = 0.f;
     int h = 0;
-    for (int j = b + tl; j < e; j += team) {
+    // four independent loads in flight per lane (long rows are latency bound)
+    int j = b + tl;
+    for (; j + 3 * team < e; j += 4 * team) {
+      int c[4];
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c[u] = __builtin_nontemporal_load(col + j + u * team);
+        v[u] = __builtin_nontemporal_load(val + j + u * team);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (MASKED) {
+          if (test_bit(xbits, c[u])) {
+            s += v[u] * x[c[u]];
+            h = 1;
+          }
+        } else {
+          s += v[u] * x[c[u]];
+          if (HITS) h |= test_bit(xbits, c[u]) ? 1 : 0;
+        }
+      }
+    }
+    for (; j < e; j += team) {
       const int c = col[j];
       if (MASKED) {
         if (test_bit(xbits, c)) {

@@ -172,35 +172,24 @@ __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const floa
 }
 
 // ------------------------------------------------------------- histogram
+// Digit counts of every pass into hist[pass][256] (zeroed by the caller's
+// preceding fill); each pass kernel scans its row of it.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sort_hist(const unsigned* __restrict__ keys,
-                                                  const long long* __restrict__ n_dev, int passes,
-                                                  unsigned* hist, unsigned* digoff, Ctl* ctl,
-                                                  int small_cap) {
+                                                  const long long* __restrict__ n_dev, int shift0,
+                                                  int passes, unsigned* hist, int small_cap) {
   __shared__ unsigned s_h[4 * 256];
-  __shared__ unsigned s_flag;
-  __shared__ unsigned s_t[NT / kWave];
   const long long n = *n_dev;
   if (n <= small_cap) return;
   for (int k = threadIdx.x; k < 4 * 256; k += NT) s_h[k] = 0u;
   __syncthreads();
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
     const unsigned k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (shift0 + 8 * p)) & 255u)], 1u);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < passes * 256; k += NT)
     if (s_h[k]) atomicAdd(&hist[k], s_h[k]);
-  const bool last = finish_persistent(ctl, kSlotHist, &s_flag);
-  if (!last) return;
-  for (int p = 0; p < passes; ++p) {
-    const unsigned c = __hip_atomic_load(&hist[p * 256 + threadIdx.x], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-    unsigned tot;
-    const unsigned ex = block_excl_sum<NT, unsigned>(c, s_t, &tot);
-    digoff[p * 256 + threadIdx.x] = ex;
-    hist[p * 256 + threadIdx.x] = 0u;
-  }
 }
 
 // ------------------------------------------------------------ one pass
@@ -210,8 +199,8 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
                                                   unsigned* __restrict__ kout,
                                                   float* __restrict__ vout,
                                                   const long long* __restrict__ n_dev, int shift,
-                                                  const unsigned* __restrict__ digoff,
-                                                  unsigned long long* state, Ctl* ctl,
+                                                  const unsigned* __restrict__ hist,
+                                                  unsigned long long* state, Ctl* ctl, Ep epa,
                                                   int small_cap) {
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
@@ -222,15 +211,17 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
   __shared__ unsigned s_dstart[256];
   __shared__ unsigned s_gbase[256];
   __shared__ unsigned s_t[NW];
-  __shared__ unsigned s_tile, s_flag;
   const long long n = *n_dev;
   if (n <= small_cap) return;
   const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
-  const unsigned ep = ctl->epoch & ((1u << kEpochBits) - 1);
+  const unsigned t = blockIdx.x;
+  if (t >= ntiles) return;
+  const unsigned ep = ep_of(ctl, epa);
   const int w = threadIdx.x / kWave;
-  while (true) {
-    const unsigned t = next_tile(ctl, kSlotSort, &s_tile);
-    if (t >= ntiles) break;
+  {
+    // global start of every digit: exclusive scan of this pass's histogram
+    unsigned dtot0;
+    const unsigned doff = block_excl_sum<NT, unsigned>(hist[threadIdx.x], s_t, &dtot0);
     const long long base = (long long)t * TILE;
     unsigned key[IPT];
     float val[IPT];
@@ -271,7 +262,7 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
       }
       st_relaxed(st, pack_state(2u, ep, excl + tot));
     }
-    s_gbase[d] = digoff[d] + (unsigned)excl;
+    s_gbase[d] = doff + (unsigned)excl;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -290,9 +281,7 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
       kout[g] = k;
       vout[g] = s_v[i];
     }
-    __syncthreads();
   }
-  finish_persistent(ctl, kSlotSort, &s_flag);
 }
 
 // --------------------------------------------------- single-workgroup path
@@ -381,7 +370,7 @@ __global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys
                                                const float* __restrict__ vals,
                                                const long long* __restrict__ n_dev, Ctl* ctl,
                                                int* ykey, float* yval, float* carry_val,
-                                               int* carry_idx, unsigned long long* state,
+                                               int* carry_idx, unsigned long long* state, Ep epa,
                                                int small_cap) {
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
@@ -390,15 +379,14 @@ __global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys
   __shared__ int s_i[NW];
   __shared__ int s_f[NW];
   __shared__ float s_fv[NW];
-  __shared__ unsigned s_tile, s_flag;
   __shared__ unsigned long long s_pre;
   const long long n = *n_dev;
   if (n <= small_cap) return;
   const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
-  const unsigned epoch = ctl->epoch;
-  while (true) {
-    const unsigned t = next_tile(ctl, kSlotReduce, &s_tile);
-    if (t >= ntiles) break;
+  const unsigned t = blockIdx.x;
+  if (t >= ntiles) return;
+  const unsigned epoch = ep_of(ctl, epa);
+  {
     const long long base = (long long)t * TILE;
     const int cnt = (int)min((long long)TILE, n - base);
     for (int i = threadIdx.x; i < cnt; i += NT) {
@@ -416,9 +404,7 @@ __global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys
       ctl->ynnz = (int)end;
       ctl->sort_small = 0;
     }
-    __syncthreads();
   }
-  finish_persistent(ctl, kSlotReduce, &s_flag);
 }
 
 __global__ void k_carry_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_idx,
@@ -444,11 +430,18 @@ static int key_passes(int range) {
   return (bits + 7) / 8;
 }
 
-// Sorts (keyA,valA)[*n_dev] by key < range; returns 0 when the result is in
-// A, 1 when in B. Used by the SpMSpV sort path (mode reduce) and by the
-// CSR->CSC transpose (mode sorted).
+// Grid for a launch over n items in tiles of `tile`: exact when the host
+// knows n (n_hint >= 0), else the bound n_cap.
+static int tiles_grid(long long n_hint, long long n_cap, int tile) {
+  const long long n = n_hint >= 0 ? n_hint : n_cap;
+  return (int)std::max<long long>(1, (n + tile - 1) / tile);
+}
+
+// Sorts (keyA, valA)[*n_dev] by key < range; returns 0 when the result is
+// in A, 1 when in B. ep.dev >= 0 (captured graphs) uses device epochs
+// ep.dev + pass; otherwise ep.v + pass.
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, bool skip_small) {
+                      long long n_hint, long long n_cap, bool skip_small, Ep ep) {
   const int passes = key_passes(range);
   const int final_b = passes & 1;
   const bool small_known = n_hint >= 0 && n_hint <= kSmallSortCap;
@@ -459,24 +452,38 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
                        final_b ? b.valB : b.valA);
   }
   if (small_known) return final_b;
-  int hgrid = n_hint > 0 ? (int)std::min<long long>(1024, (n_hint + 4095) / 4096) : 1024;
-  hipLaunchKernelGGL(k_sort_hist<256>, dim3(hgrid), dim3(256), 0, s, b.keyA, n_dev, passes, b.hist,
-                     b.digoff, ctl, kSmallSortCap);
-  int pgrid = n_hint > 0 ? (int)std::min<long long>(kPersistGrid, (n_hint + kSortTile - 1) / kSortTile)
-                         : kPersistGrid;
+  launch_fill_i32(s, reinterpret_cast<int*>(b.hist), 4 * 256, 0);
+  const int hgrid = std::min(1024, tiles_grid(n_hint, n_cap, 4096));
+  hipLaunchKernelGGL(k_sort_hist<256>, dim3(hgrid), dim3(256), 0, s, b.keyA, n_dev, 0, passes, b.hist,
+                     kSmallSortCap);
+  const int pgrid = tiles_grid(n_hint, n_cap, kSortTile);
   for (int p = 0; p < passes; ++p) {
     const unsigned* kin = (p & 1) ? b.keyB : b.keyA;
     const float* vin = (p & 1) ? b.valB : b.valA;
     unsigned* kout = (p & 1) ? b.keyA : b.keyB;
     float* vout = (p & 1) ? b.valA : b.valB;
+    const Ep e{ep.v + (unsigned)p, ep.dev < 0 ? -1 : ep.dev + p};
     hipLaunchKernelGGL((k_sort_pass<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, vin,
-                       kout, vout, n_dev, 8 * p, b.digoff + p * 256, b.state, ctl, kSmallSortCap);
+                       kout, vout, n_dev, 8 * p, b.hist + p * 256, b.state, ctl, e, kSmallSortCap);
   }
   return final_b;
 }
 
+// One stable partition pass of (keyA, valA) into (keyB, valB) on the 8-bit
+// digit (key >> shift) & 255; the digit starts are the exclusive scan of
+// b.hist[0..255].
+void launch_partition_pass(hipStream_t s, const SortBufs& b, int shift, const long long* n_dev, Ctl* ctl,
+                           long long n_hint, long long n_cap, Ep ep) {
+  launch_fill_i32(s, reinterpret_cast<int*>(b.hist), 256, 0);
+  const int hgrid = std::min(1024, tiles_grid(n_hint, n_cap, 4096));
+  hipLaunchKernelGGL(k_sort_hist<256>, dim3(hgrid), dim3(256), 0, s, b.keyA, n_dev, shift, 1, b.hist, -1);
+  hipLaunchKernelGGL((k_sort_pass<kSortNT, kSortIPT>), dim3(tiles_grid(n_hint, n_cap, kSortTile)),
+                     dim3(kSortNT), 0, s, b.keyA, b.valA, b.keyB, b.valB, n_dev, shift, b.hist, b.state,
+                     ctl, ep, -1);
+}
+
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
-                        long long binlen_hint) {
+                        long long binlen_hint, long long n_cap, Ep ep) {
   const long long* n_dev = &ctl->bin_len;
   const int passes = key_passes(m);
   const bool small_known = binlen_hint >= 0 && binlen_hint <= kSmallSortCap;
@@ -486,16 +493,16 @@ void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* 
                        b.valA, n_dev, passes, 0, ctl, ykey, yval, nullptr, nullptr);
   }
   if (small_known) return;
-  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, true);
+  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, n_cap, true, ep);
   const unsigned* kf = fb ? b.keyB : b.keyA;
   const float* vf = fb ? b.valB : b.valA;
-  int rgrid = binlen_hint > 0 ? (int)std::min<long long>(kPersistGrid, (binlen_hint + kRedTile - 1) / kRedTile)
-                              : kPersistGrid;
-  hipLaunchKernelGGL((k_reduce<kRedNT, kRedIPT>), dim3(rgrid), dim3(kRedNT), 0, s, kf, vf, n_dev, ctl,
-                     ykey, yval, b.carry_val, b.carry_idx, b.red_state, kSmallSortCap);
-  int fgrid = binlen_hint > 0 ? (int)std::min<long long>(2048, (binlen_hint / kRedTile + 256) / 256) : 512;
-  hipLaunchKernelGGL(k_carry_fixup, dim3(std::max(fgrid, 1)), dim3(256), 0, s, b.carry_val,
-                     b.carry_idx, n_dev, kRedTile, kSmallSortCap, yval);
+  const Ep er{ep.v + 4u, ep.dev < 0 ? -1 : ep.dev + 4};
+  hipLaunchKernelGGL((k_reduce<kRedNT, kRedIPT>), dim3(tiles_grid(binlen_hint, n_cap, kRedTile)),
+                     dim3(kRedNT), 0, s, kf, vf, n_dev, ctl, ykey, yval, b.carry_val, b.carry_idx,
+                     b.red_state, er, kSmallSortCap);
+  const int fgrid = std::min(2048, std::max(1, tiles_grid(binlen_hint, n_cap, kRedTile) / 256 + 1));
+  hipLaunchKernelGGL(k_carry_fixup, dim3(fgrid), dim3(256), 0, s, b.carry_val, b.carry_idx, n_dev,
+                     kRedTile, kSmallSortCap, yval);
 }
 
 }  // namespace spmspv

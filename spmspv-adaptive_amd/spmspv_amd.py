@@ -120,6 +120,7 @@ _SIGS = {
     "spmspv_memcpy_h2d": [_vp, _vp, _sz],
     "spmspv_memcpy_d2h": [_vp, _vp, _sz],
     "spmspv_memset": [_vp, _i, _sz],
+    "spmspv_memcpy_d2d": [_vp, _vp, _sz],
     "spmspv_synchronize": [],
     "spmspv_timer_start": [_vp],
     "spmspv_timer_stop": [_vp, ctypes.POINTER(_f)],

@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Selector corpus, labels and training (the learned part of the engine).
+
+The reference labels each (matrix, x) pair with the fastest of its kernels
+(model/1-generate_matrix_class.py:36-38, 100) and fits sklearn trees offline
+(model/1-test_dtree.py:170-196), scoring them by loss_ratio =
+(t_pred - t_best) / t_best (model/1-test_dtree.py:300-317). Here:
+
+  corpus  builds synthetic matrices on the host, times all eight slots on
+          the GPU (cold cache, median of reps) and records the 16 device
+          features and the rule fallback's choice for each (matrix, x):
+            uniform random (n = 2^14..2^21 log-uniform, mean nnz/col 2..64),
+            R-MAT (a in [0.45, 0.65], d = 0.05, b = c),
+            banded (bandwidth 3..201), block-diagonal (dense 32x32 blocks);
+          4 x densities per matrix, log-uniform in [1e-5, 1]; matrix seeds
+          0..1999 train, 2000..2399 held out. Sizes are capped at 2^25
+          nonzeros per matrix so that a shard fits one GPU session.
+  train   trains the MLP 16->64->64->8 on the GPU with the library's own
+          kernels (spmspv_mlp_train) on soft targets
+          p_k ~ exp(-(t_k / t_best - 1) / tau), then reports held-out
+          regret against oracle-best, the best single fixed kernel and the
+          rule fallback, and writes the weights.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import spmspv_amd as sa  # noqa: E402
+
+NNZ_CAP = 1 << 25
+
+
+def matrix_spec(seed):
+    rng = np.random.default_rng(seed + 1_000_003)
+    kind = ["uniform", "rmat", "banded", "blockdiag"][seed % 4]
+    lg = rng.uniform(14, 21)
+    n = int(2 ** lg)
+    if kind == "uniform":
+        mean = float(np.exp(rng.uniform(np.log(2), np.log(64))))
+        mean = min(mean, NNZ_CAP / n)
+        return dict(kind=kind, n=n, mean=mean)
+    if kind == "rmat":
+        scale = int(round(lg))
+        a = float(rng.uniform(0.45, 0.65))
+        ef = int(np.exp(rng.uniform(np.log(2), np.log(32))))
+        ef = max(1, min(ef, NNZ_CAP // (2 << scale)))
+        return dict(kind=kind, scale=scale, a=a, ef=ef)
+    if kind == "banded":
+        bw = int(rng.integers(3, 202))
+        n = min(n, NNZ_CAP // (2 * bw + 1))
+        return dict(kind=kind, n=n, bw=bw)
+    n = min(n, NNZ_CAP // 32)
+    return dict(kind=kind, n=n, bs=32)
+
+
+def build_matrix(spec, seed):
+    k = spec["kind"]
+    if k == "uniform":
+        rp, col, val = sa.gen_random_csr(spec["n"], spec["n"], spec["mean"], seed)
+        n = spec["n"]
+    elif k == "rmat":
+        a = spec["a"]
+        b = c = (0.95 - a) / 2
+        rp, col, val = sa.gen_rmat_csr(spec["scale"], spec["ef"], a, b, c, seed)
+        n = 1 << spec["scale"]
+    elif k == "banded":
+        rp, col, val = sa.gen_banded_csr(spec["n"], spec["bw"], seed)
+        n = spec["n"]
+    else:
+        rp, col, val = sa.gen_blockdiag_csr(spec["n"], spec["bs"], seed)
+        n = spec["n"]
+    return n, rp, col, val
+
+
+def time_slot(A, slot, yk, yv, reps, flush):
+    ts = []
+    for r in range(reps + 1):
+        if flush:
+            A.flush_cache(flush)
+        A.timer_start()
+        A.run(slot, y_key=yk, y_val=yv)
+        ms = A.timer_stop()
+        if r:
+            ts.append(ms)
+    return float(np.median(ts))
+
+
+def cmd_corpus(args):
+    feats, times, rule, meta = [], [], [], []
+    t0 = time.time()
+    flush = args.flush_mb << 20
+    for seed in range(args.start, args.start + args.count):
+        spec = matrix_spec(seed)
+        n, rp, col, val = build_matrix(spec, seed)
+        if rp[-1] == 0:
+            continue
+        M = sa.Matrix(n, n, rp, col, val)
+        A = M.A
+        yk = sa.DeviceArray(n, np.int32)
+        yv = sa.DeviceArray(n, np.float32)
+        rng = np.random.default_rng(seed + 7)
+        for j in range(4):
+            d = float(np.exp(rng.uniform(np.log(1e-5), 0.0)))
+            k = max(1, min(n, int(round(d * n))))
+            key, xv = sa.gen_sparse_vector(n, k, seed * 16 + j)
+            dk = sa.DeviceArray.from_host(key, np.int32)
+            dv = sa.DeviceArray.from_host(xv, np.float32)
+            A.set_sparse_x(dk, dv, k)
+            A.selector_mode(1)
+            rule.append(A.select())
+            feats.append(A.features())
+            times.append([time_slot(A, s, yk, yv, args.reps, flush) for s in range(sa.NUM_SLOTS)])
+            meta.append([seed, n, int(rp[-1]), k])
+        del M, A
+        if (seed - args.start) % 25 == 0:
+            print(f"corpus seed {seed} {spec} {time.time() - t0:.1f}s", flush=True)
+    np.savez(args.out, feats=np.array(feats, np.float32), times=np.array(times, np.float64),
+             rule=np.array(rule, np.int32), meta=np.array(meta, np.int64))
+    print(f"corpus {len(feats)} samples -> {args.out} in {time.time() - t0:.1f}s", flush=True)
+
+
+def soft_targets(times, tau):
+    rel = times / times.min(axis=1, keepdims=True) - 1.0
+    z = np.exp(-rel / tau)
+    return (z / z.sum(axis=1, keepdims=True)).astype(np.float32)
+
+
+def predict(w, X):
+    F = sa.MLP_F
+    mean, istd = w[sa.MLP_NPARAM:sa.MLP_NPARAM + F], w[sa.MLP_NPARAM + F:]
+    Xn = np.ascontiguousarray((X - mean) * istd, np.float32)
+    z = np.zeros((X.shape[0], sa.MLP_C), np.float32)
+    rc = sa.lib().spmspv_mlp_forward(w.ctypes.data_as(ctypes.c_void_p), Xn.ctypes.data_as(ctypes.c_void_p),
+                                     X.shape[0], z.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise sa.SpmspvError(rc, "mlp forward")
+    return z.argmax(axis=1)
+
+
+def geo_regret(times, choice):
+    t = times[np.arange(len(choice)), choice]
+    return float(np.exp(np.mean(np.log(t / times.min(axis=1)))) - 1.0)
+
+
+def cmd_train(args):
+    parts = [np.load(p, allow_pickle=False) for p in args.corpus]
+    feats = np.concatenate([p["feats"] for p in parts])
+    times = np.concatenate([p["times"] for p in parts])
+    rule = np.concatenate([p["rule"] for p in parts])
+    meta = np.concatenate([p["meta"] for p in parts])
+    train = meta[:, 0] < args.holdout_from
+    test = ~train
+    X, T = feats[train], soft_targets(times[train], args.tau)
+    w = np.zeros(sa.MLP_NWEIGHTS, np.float32)
+    loss = ctypes.c_float(0)
+    t0 = time.time()
+    rc = sa.lib().spmspv_mlp_train(np.ascontiguousarray(X).ctypes.data_as(ctypes.c_void_p),
+                                   T.ctypes.data_as(ctypes.c_void_p), int(X.shape[0]), args.epochs,
+                                   args.batch, args.lr, args.seed, w.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.byref(loss))
+    if rc:
+        raise sa.SpmspvError(rc, sa.lib().spmspv_last_error(None).decode())
+    train_s = time.time() - t0
+    rep = {"train_samples": int(train.sum()), "test_samples": int(test.sum()), "final_loss": loss.value,
+           "train_seconds": round(train_s, 2), "epochs": args.epochs, "tau": args.tau}
+    for name, mask in (("train", train), ("heldout", test)):
+        if not mask.any():
+            continue
+        tm = times[mask]
+        pred = predict(w, feats[mask])
+        fixed = [geo_regret(tm, np.full(mask.sum(), s)) for s in range(sa.NUM_SLOTS)]
+        best_fixed = int(np.argmin(fixed))
+        rep[name] = {
+            "mlp_regret": geo_regret(tm, pred),
+            "rule_regret": geo_regret(tm, rule[mask]),
+            "best_fixed_slot": sa.KERNEL_NAMES[best_fixed],
+            "best_fixed_regret": fixed[best_fixed],
+            "fixed_regrets": {sa.KERNEL_NAMES[s]: fixed[s] for s in range(sa.NUM_SLOTS)},
+            "mlp_accuracy": float((pred == tm.argmin(axis=1)).mean()),
+            "mlp_total_ms": float(tm[np.arange(len(pred)), pred].sum()),
+            "best_fixed_total_ms": float(tm[:, best_fixed].sum()),
+            "oracle_total_ms": float(tm.min(axis=1).sum()),
+        }
+    np.save(args.out, w)
+    print(json.dumps(rep, indent=1), flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(rep, f, indent=1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("corpus")
+    c.add_argument("--start", type=int, default=0)
+    c.add_argument("--count", type=int, default=100)
+    c.add_argument("--reps", type=int, default=3)
+    c.add_argument("--flush-mb", type=int, default=512)
+    c.add_argument("--out", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--corpus", nargs="+", required=True)
+    t.add_argument("--holdout-from", type=int, default=2000)
+    t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--batch", type=int, default=256)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--tau", type=float, default=0.1)
+    t.add_argument("--seed", type=int, default=11)
+    t.add_argument("--out", default=os.path.join(HERE, "selector_weights.npy"))
+    t.add_argument("--report", default=None)
+    args = ap.parse_args()
+    {"corpus": cmd_corpus, "train": cmd_train}[args.cmd](args)
+
+
+if __name__ == "__main__":
+    main()

@@ -47,7 +47,7 @@ def run_slot(M, k, key, val, want_dense=False):
     return out
 
 
-SLOTS = list(range(8))
+SLOTS = list(range(9))  # the eight selector slots + the bucket kernel
 
 
 @pytest.fixture(scope="module")
@@ -252,3 +252,16 @@ def test_selector_tree_mode_and_run_auto(rmat12):
         check_result(M.m, cp, ri, cv, key, val, yk.to_host(nnz), yv.to_host(nnz), f"auto {k}")
         f = M.A.features()
         assert np.all(np.isfinite(f))
+
+
+def test_cpp_handle_adaptive_app(gpu):
+    """The reference's adaptive-app flow written against the C++
+    SpmspvHandle shim (examples/adaptive_app.cpp), checked against the
+    reference's all-ones known-answer pattern inside the program."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(sa.LIB_PATH), "adaptive_app")
+    out = subprocess.run([exe, "12"], capture_output=True, text=True, timeout=120)
+    say(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.count("RESULT OK") == 6
