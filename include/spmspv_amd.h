@@ -190,6 +190,16 @@ int spmspv_holaspmv(spmspv_handle_t h, float alpha, float* d_y);
 int spmspv_csr5spmv(spmspv_handle_t h, float alpha, float* d_y);
 int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y);
 
+/* ---- autotuner (no reference counterpart: the reference's tile sizes and
+ * thread counts are compile-time macros, la/include/spmspv/config.h:8-30,
+ * scan_config.h:4-26) ------------------------------------------------------ */
+/* On the current x, times every candidate team size of the naive column
+ * and row slots (2..64 lanes) and every row-merge tile width (5, 7 or 11
+ * items per lane) with the handle's stream events, median of `reps`, and
+ * keeps the fastest for this matrix. report (nullable, 3 ints) gets
+ * {team_col, team_row, row_vt}. */
+int spmspv_autotune(spmspv_handle_t h, int reps, int* report);
+
 /* ---- learned selector (no reference counterpart in C++: the reference's
  * predictors are stubs returning 0, la/test/new_test_app.cu:121-131) ------ */
 /* MLP 16 -> 64 -> 64 -> 8, FP32. Parameters packed W1[64*16] b1[64]

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "handle.hpp"
 #include "selector.hpp"
@@ -70,7 +71,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
   const size_t cmp_tiles = (std::max(mwords, nwords) + kCmpNT - 1) / kCmpNT + 2;
   const size_t sort_tiles = (nnz + kSortTile - 1) / kSortTile + 2;
   const size_t red_tiles = (nnz + kRedTile - 1) / kRedTile + 2;
-  const size_t row_tiles = (m + nnz + kRowTile - 1) / kRowTile + 2;
+  const size_t row_tiles = (m + nnz + kRowNT * kRowVTMin - 1) / (kRowNT * kRowVTMin) + 2;
   ;
   Ctl* ctl = c.take<Ctl>(1);
   MatFeat* mat = c.take<MatFeat>(1);
@@ -164,8 +165,9 @@ int ensure_row_tiles(spmspv_handle_t h) {
   if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "row kernels need CSR input");
   int rc = ensure_arena(h);
   if (rc) return rc;
-  h->n_row_tiles = (int)(((long long)h->m + h->nnz + kRowTile - 1) / kRowTile);
-  launch_row_split(h->stream, h->csr_row, h->m, h->nnz, h->row_split, h->n_row_tiles);
+  const int tile = kRowNT * h->row_vt;
+  h->n_row_tiles = (int)(((long long)h->m + h->nnz + tile - 1) / tile);
+  launch_row_split(h->stream, h->csr_row, h->m, h->nnz, h->row_split, h->n_row_tiles, tile);
   rc = check_launch(h, "row split");
   if (rc) return rc;
   h->row_ready = true;
@@ -353,8 +355,8 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       if (lb) {
         if ((rc = ensure_row_tiles(h))) return rc;
         launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
-                         m, h->nnz, h->row_split, h->n_row_tiles, yout, h->bitmap, h->row_carry_val,
-                         h->row_carry_row, h->row_carry_hit);
+                         m, h->nnz, h->row_split, h->n_row_tiles, h->row_vt, yout, h->bitmap,
+                         h->row_carry_val, h->row_carry_row, h->row_carry_hit);
       } else {
         launch_row_naive(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
                          m, h->team_row, yout, h->bitmap);
@@ -880,6 +882,77 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   if (rc) return rc;
   if (kernel_used) *kernel_used = kid;
   return spmspv_run(h, kid, alpha, d_y_dense, d_y_key, d_y_val, y_nnz);
+}
+
+// ------------------------------------------------------------- autotuner
+int spmspv_autotune(spmspv_handle_t h, int reps, int* report) {
+  if (!h || reps <= 0) return set_error(h, SPMSPV_INVALID_ARGUMENT, "autotune: bad arguments");
+  int rc;
+  if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+  auto time_slot = [&](int kid, float* out) -> int {
+    std::vector<float> ms;
+    for (int r = 0; r <= reps; ++r) {
+      HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
+      int e = run_kernel(h, kid, nullptr, h->yk_tmp, h->yv_tmp, true);
+      if (e) return e;
+      HIP_TRY(h, hipEventRecord(h->ev1, h->stream));
+      HIP_TRY(h, hipEventSynchronize(h->ev1));
+      float t = 0.f;
+      HIP_TRY(h, hipEventElapsedTime(&t, h->ev0, h->ev1));
+      if (r) ms.push_back(t);  // the first run warms up
+    }
+    std::sort(ms.begin(), ms.end());
+    *out = ms[ms.size() / 2];
+    return SPMSPV_SUCCESS;
+  };
+  const int teams[] = {2, 4, 8, 16, 32, 64};
+  float best = 1e30f, t = 0.f;
+  int pick = h->team_col;
+  for (int tc : teams) {
+    h->team_col = tc;
+    if ((rc = time_slot(SPMSPV_K_COL_NAIVE_ATOMIC, &t))) return rc;
+    if (t < best) {
+      best = t;
+      pick = tc;
+    }
+  }
+  h->team_col = pick;
+  if (h->csr_row) {
+    best = 1e30f;
+    pick = h->team_row;
+    for (int tr : teams) {
+      h->team_row = tr;
+      if ((rc = time_slot(SPMSPV_K_SPMV_NAIVE, &t))) return rc;
+      if (t < best) {
+        best = t;
+        pick = tr;
+      }
+    }
+    h->team_row = pick;
+    best = 1e30f;
+    pick = h->row_vt;
+    for (int vt : {5, 7, 11}) {
+      h->row_vt = vt;
+      h->row_ready = false;
+      if ((rc = ensure_row_tiles(h)) || (rc = time_slot(SPMSPV_K_SPMV_LB, &t))) return rc;
+      if (t < best) {
+        best = t;
+        pick = vt;
+      }
+    }
+    h->row_vt = pick;
+    h->row_ready = false;
+    if ((rc = ensure_row_tiles(h))) return rc;
+    // captured BFS levels hold the old tiling
+    spmspv_bfs_release(h);
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (report) {
+    report[0] = h->team_col;
+    report[1] = h->team_row;
+    report[2] = h->row_vt;
+  }
+  return SPMSPV_SUCCESS;
 }
 
 // ------------------------------------------------------------- utilities

@@ -228,11 +228,13 @@ __device__ __forceinline__ unsigned long long tile_prefix(unsigned long long* st
         unsigned long long s = 0;
         unsigned flag = 0;
         if (idx >= 0) {
-          do {
+          while (true) {
             s = ld_relaxed(&state[idx]);
             flag = (unsigned)(s >> 62);
             if (((s >> 38) & ((1u << kEpochBits) - 1)) != ep) flag = 0;
-          } while (flag == 0);
+            if (flag) break;
+            __builtin_amdgcn_s_sleep(1);  // back off instead of hammering the line
+          }
         } else {
           flag = 2;  // before tile 0: acts as an inclusive zero
           s = 0;

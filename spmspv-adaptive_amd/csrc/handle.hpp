@@ -85,6 +85,7 @@ struct spmspv_handle_s {
   int sel_fixed = 0;
   int team_col = 16;
   int team_row = 8;
+  int row_vt = 7;  // row-merge items per thread (autotuned)
 
   // host mirrors
   int* h_pin = nullptr;       // small pinned readback area (64 ints)

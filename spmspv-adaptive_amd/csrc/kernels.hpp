@@ -18,6 +18,8 @@ constexpr int kCmpNT = 256;  // bitmap words per compaction tile
 constexpr int kSortNT = 256, kSortIPT = 12, kSortTile = kSortNT * kSortIPT;
 constexpr int kRedNT = 256, kRedIPT = 7, kRedTile = kRedNT * kRedIPT;
 constexpr int kRowNT = 256, kRowVT = 7, kRowTile = kRowNT * kRowVT;
+// row-merge items per thread the autotuner may pick (odd: conflict-free)
+constexpr int kRowVTMin = 5;
 constexpr int kSmallSortCap = kSortTile;  // single-workgroup LDS sort + reduce
 constexpr int kPersistGrid = 1024;        // workgroups of a persistent launch
 
@@ -99,11 +101,13 @@ void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, unsigned l
                    int* ykey, float* yval, long long binlen_hint, long long n_cap, Ep ep);
 
 // --- row path ------------------------------------------------------------
-void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles);
+void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles,
+                      int tile);
+// vt in {5, 7, 11}: merged items per thread (tile = 256 * vt)
 void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
-                      const int* split, int ntiles, float* y, unsigned* ybits, float* carry_val,
-                      int* carry_row, int* carry_hit);
+                      const int* split, int ntiles, int vt, float* y, unsigned* ybits,
+                      float* carry_val, int* carry_row, int* carry_hit);
 void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int team,
                       float* y, unsigned* ybits);

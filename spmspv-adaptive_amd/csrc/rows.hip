@@ -40,10 +40,10 @@ __global__ void k_row_split(const int* __restrict__ row_ptr, int m, int nnz, int
   split[t] = (int)lo;
 }
 
-void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles) {
+void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles,
+                      int tile) {
   const int grid = (ntiles + 1 + 255) / 256;
-  hipLaunchKernelGGL(k_row_split, dim3(grid), dim3(256), 0, s, row_ptr, m, nnz, kRowTile, ntiles,
-                     split);
+  hipLaunchKernelGGL(k_row_split, dim3(grid), dim3(256), 0, s, row_ptr, m, nnz, tile, ntiles, split);
 }
 
 template <int NT>
@@ -277,20 +277,29 @@ __global__ void k_row_fixup(const float* __restrict__ carry_val, const int* __re
 
 void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
-                      const int* split, int ntiles, float* y, unsigned* ybits, float* carry_val,
-                      int* carry_row, int* carry_hit) {
+                      const int* split, int ntiles, int vt, float* y, unsigned* ybits,
+                      float* carry_val, int* carry_row, int* carry_hit) {
   if (ntiles <= 0) return;
   // about eight workgroups per CU, each walking several tiles with the next
   // tile's loads in flight
   const int grid = std::min(ntiles, 2048);
-#define SPMSPV_ROW_MERGE(MK, HT)                                                                 \
-  hipLaunchKernelGGL((k_row_merge<kRowNT, kRowVT, MK, HT>), dim3(grid), dim3(kRowNT), 0, s,      \
-                     row_ptr, col, val, x, xbits, m, nnz, split, ntiles, y, ybits, carry_val,    \
-                     carry_row, carry_hit)
-  if (masked && hits) SPMSPV_ROW_MERGE(true, true);
-  else if (masked) SPMSPV_ROW_MERGE(true, false);
-  else if (hits) SPMSPV_ROW_MERGE(false, true);
-  else SPMSPV_ROW_MERGE(false, false);
+#define SPMSPV_ROW_MERGE(V, MK, HT)                                                             \
+  hipLaunchKernelGGL((k_row_merge<kRowNT, V, MK, HT>), dim3(grid), dim3(kRowNT), 0, s, row_ptr,  \
+                     col, val, x, xbits, m, nnz, split, ntiles, y, ybits, carry_val, carry_row,  \
+                     carry_hit)
+#define SPMSPV_ROW_MERGE_VT(V)                          \
+  if (masked && hits) SPMSPV_ROW_MERGE(V, true, true);  \
+  else if (masked) SPMSPV_ROW_MERGE(V, true, false);    \
+  else if (hits) SPMSPV_ROW_MERGE(V, false, true);      \
+  else SPMSPV_ROW_MERGE(V, false, false);
+  if (vt == 5) {
+    SPMSPV_ROW_MERGE_VT(5)
+  } else if (vt == 11) {
+    SPMSPV_ROW_MERGE_VT(11)
+  } else {
+    SPMSPV_ROW_MERGE_VT(7)
+  }
+#undef SPMSPV_ROW_MERGE_VT
 #undef SPMSPV_ROW_MERGE
   const int fgrid = std::min((ntiles + 255) / 256, 1024);
   if (hits)

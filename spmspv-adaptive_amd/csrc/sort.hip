@@ -251,11 +251,13 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
       while (true) {
         unsigned long long s;
         unsigned f;
-        do {
+        while (true) {
           s = ld_relaxed(state + (size_t)k * 256 + d);
           f = (unsigned)(s >> 62);
           if (((s >> 38) & ((1u << kEpochBits) - 1)) != ep) f = 0;
-        } while (f == 0);
+          if (f) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
         excl += s & kValMask;
         if (f == 2) break;
         --k;

@@ -105,6 +105,7 @@ _SIGS = {
     "spmspv_holaspmv": [_vp, _f, _vp],
     "spmspv_csr5spmv": [_vp, _f, _vp],
     "spmspv_spmv": [_vp, _f, _vp],
+    "spmspv_autotune": [_vp, _i, _vp],
     "spmspv_selector_set_weights": [_vp, _vp, _i],
     "spmspv_selector_mode": [_vp, _i, _i],
     "spmspv_select": [_vp, ctypes.POINTER(_i)],
@@ -355,6 +356,11 @@ class SpmspvHandle:
                                       y_key.ptr if y_key else None, y_val.ptr if y_val else None,
                                       ctypes.byref(c), ctypes.byref(k)))
         return c.value, k.value
+
+    def autotune(self, reps=3):
+        rep = np.zeros(3, np.int32)
+        self._c(lib().spmspv_autotune(self.h, reps, _np(rep)))
+        return {"team_col": int(rep[0]), "team_row": int(rep[1]), "row_vt": int(rep[2])}
 
     def select(self):
         k = _i(-1)

@@ -265,3 +265,23 @@ def test_cpp_handle_adaptive_app(gpu):
     say(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.count("RESULT OK") == 6
+
+
+def test_autotune_then_parity(rmat12):
+    """The autotuner changes team sizes and the row tile width per matrix;
+    every slot must stay exact afterwards, for every candidate width."""
+    M, (cp, ri, cv) = rmat12
+    key, val = sa.gen_sparse_vector(M.n, 400, 99)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    M.A.set_sparse_x(dk, dv, key.size)
+    rep = M.A.autotune(reps=2)
+    say(f"autotune picked {rep}")
+    assert rep["team_col"] in (2, 4, 8, 16, 32, 64) and rep["row_vt"] in (5, 7, 11)
+    for slot in SLOTS:
+        yk, yv, _ = run_slot(M, slot, key, val)
+        check_result(M.m, cp, ri, cv, key, val, yk, yv, f"after autotune slot {slot}")
+    lev = sa.DeviceArray(M.n, np.int32)
+    M.A.bfs(0, lev, policy=5, use_graphs=1)
+    ref, _ = oracle.bfs(M.n, cp, ri, 0)
+    np.testing.assert_array_equal(lev.to_host(), ref)
