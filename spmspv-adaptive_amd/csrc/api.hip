@@ -174,6 +174,30 @@ int ensure_row_tiles(spmspv_handle_t h) {
   return SPMSPV_SUCCESS;
 }
 
+static int need_csc(spmspv_handle_t h);
+
+int ensure_xblocks(spmspv_handle_t h) {
+  const int tile = kRowNT * h->row_vt;
+  if (h->xb_ready && h->xs.xb.nb == h->xb_nb && h->xs.tile == tile) return SPMSPV_SUCCESS;
+  if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "row kernels need CSR input");
+  int rc = ensure_arena(h);
+  if (rc || (rc = need_csc(h))) return rc;
+  const XStoreLayout L = xstore_layout(h->m, h->nnz, h->xb_nb, tile);
+  if (h->xmem_bytes < L.bytes) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->xmem) HIP_TRY(h, hipFree(h->xmem));
+    h->xmem = nullptr;
+    h->xmem_bytes = 0;
+    HIP_TRY(h, hipMalloc(&h->xmem, L.bytes));
+    h->xmem_bytes = L.bytes;
+  }
+  HIP_TRY(h, build_xblocks(h->stream, h->csr_row, h->csr_col, h->csr_val, h->csc_colptr, h->m, h->n,
+                           h->nnz, h->xb_nb, tile, static_cast<char*>(h->xmem), h->h_pin, &h->xs));
+  h->launches += 10 + h->xb_nb;
+  h->xb_ready = true;
+  return SPMSPV_SUCCESS;
+}
+
 int ensure_mat_features(spmspv_handle_t h) {
   if (h->mat_ready) return SPMSPV_SUCCESS;
   int rc = ensure_arena(h);
@@ -352,6 +376,17 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       const bool hits = want_sparse || !y_dense;
       if ((rc = need_dense_x(h))) return rc;
       if ((masked || hits) && (rc = need_bits(h))) return rc;
+      if (lb && h->xb_nb > 1) {
+        if ((rc = ensure_xblocks(h))) return rc;
+        launch_row_merge_x(s, masked, hits, h->xs.xb, h->dense_x(), h->bits(), m, h->row_vt,
+                           h->xs.ypart, h->marks, h->xs.carry_val, h->xs.carry_row, h->xs.carry_hit,
+                           yout);
+        if (hits)
+          launch_compact(s, nullptr, h->marks, m, yout, yout == h->acc ? 1 : 0, y_key, y_val,
+                         h->scan_state, h->ctl, next_ep(h, 1));
+        h->launches += 4;
+        return check_launch(h, "column-blocked row kernel");
+      }
       if (lb) {
         if ((rc = ensure_row_tiles(h))) return rc;
         launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
@@ -401,8 +436,8 @@ int spmspv_create(spmspv_handle_t* out, int m, int n, int nnz) {
     delete h;
     return set_error(nullptr, SPMSPV_DEVICE_ERROR, std::string("pinned buffer: ") + hipGetErrorString(e));
   }
-  hipEventCreate(&h->ev0);
-  hipEventCreate(&h->ev1);
+  (void)hipEventCreate(&h->ev0);
+  (void)hipEventCreate(&h->ev1);
   *out = h;
   return SPMSPV_SUCCESS;
 }
@@ -411,14 +446,15 @@ int spmspv_bfs_release(spmspv_handle_t h);
 
 int spmspv_destroy(spmspv_handle_t h) {
   if (!h) return SPMSPV_SUCCESS;
-  hipStreamSynchronize(h->stream);
+  (void)hipStreamSynchronize(h->stream);
   spmspv_bfs_release(h);
-  if (h->arena) hipFree(h->arena);
-  if (h->csc_alloc) hipFree(h->csc_alloc);
-  if (h->flush_buf) hipFree(h->flush_buf);
-  if (h->h_pin) hipHostFree(h->h_pin);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->arena) (void)hipFree(h->arena);
+  if (h->csc_alloc) (void)hipFree(h->csc_alloc);
+  if (h->xmem) (void)hipFree(h->xmem);
+  if (h->flush_buf) (void)hipFree(h->flush_buf);
+  if (h->h_pin) (void)hipHostFree(h->h_pin);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
   return SPMSPV_SUCCESS;
 }
@@ -443,6 +479,7 @@ int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* 
   h->csr_col = col;
   h->csr_val = val;
   h->row_ready = false;
+  h->xb_ready = false;
   h->mat_ready = false;
   return SPMSPV_SUCCESS;
 }
@@ -452,7 +489,7 @@ int spmspv_input_csc(spmspv_handle_t h, int nnz, int* row, int* col_ptr, float* 
     return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSC: null argument");
   if (nnz != h->nnz) return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSC: nnz differs from handle");
   if (h->csc_alloc) {
-    hipFree(h->csc_alloc);
+    (void)hipFree(h->csc_alloc);
     h->csc_alloc = nullptr;
   }
   h->csc_row = row;
@@ -479,8 +516,8 @@ int spmspv_to_csc(spmspv_handle_t h) {
     h->csc_val = reinterpret_cast<float*>(p + a + b);
     h->csc_owned = true;
   }
-  launch_csr_to_csc_sorted(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->n, h->nnz,
-                           h->csc_colptr, h->csc_row, h->csc_val, h->sb, h->n_dev, h->ctl, next_ep(h, 4));
+  HIP_TRY(h, launch_csr_to_csc_sorted(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->n, h->nnz,
+                           h->csc_colptr, h->csc_row, h->csc_val, h->sb, h->n_dev, h->ctl, next_ep(h, 4)));
   h->launches += 6;
   rc = check_launch(h, "ToCSC");
   if (rc) return rc;
@@ -500,8 +537,8 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
 
 int spmspv_dealloc_pre_buffer(spmspv_handle_t h) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
-  hipStreamSynchronize(h->stream);
-  if (h->arena) hipFree(h->arena);
+  (void)hipStreamSynchronize(h->stream);
+  if (h->arena) (void)hipFree(h->arena);
   h->arena = nullptr;
   h->reserved = false;
   h->row_ready = false;
@@ -531,6 +568,18 @@ int spmspv_set_spmv_type(spmspv_handle_t h, int code) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   if (code != 0 && code != 1) return set_error(h, SPMSPV_INVALID_ARGUMENT, "spmv type must be 0 or 1");
   h->spmv_type = code;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_col_blocks(spmspv_handle_t h, int nb) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (nb != 1 && nb != 2 && nb != 4 && nb != 8)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "column blocks must be 1, 2, 4 or 8");
+  if (nb != h->xb_nb) {
+    h->xb_nb = nb;
+    h->xb_ready = false;
+    spmspv_bfs_release(h);  // captured levels hold the old kernels
+  }
   return SPMSPV_SUCCESS;
 }
 
@@ -1007,7 +1056,7 @@ int spmspv_timer_stop(spmspv_handle_t h, float* ms) {
 int spmspv_flush_cache(spmspv_handle_t h, size_t bytes) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   if (h->flush_bytes < bytes) {
-    if (h->flush_buf) hipFree(h->flush_buf);
+    if (h->flush_buf) (void)hipFree(h->flush_buf);
     h->flush_buf = nullptr;
     HIP_TRY(h, hipMalloc(&h->flush_buf, bytes));
     h->flush_bytes = bytes;

@@ -358,7 +358,7 @@ int bfs_alloc(spmspv_handle_t h) {
   b->ones = reinterpret_cast<float*>(p + o_ones);
   if (!h->stream) {
     if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
-      hipFree(b->alloc);
+      (void)hipFree(b->alloc);
       delete b;
       return set_error(h, SPMSPV_DEVICE_ERROR, "bfs stream");
     }

@@ -35,7 +35,6 @@ __global__ __launch_bounds__(NT) void k_bucket_acc(const unsigned* __restrict__ 
   __shared__ unsigned s_hit[WW];
   __shared__ unsigned long long s_pre;
   __shared__ int s_i[NT / kWave];
-  const long long n = *n_dev;
   const int nbuckets = ((m - 1) >> bshift) + 1;
   const int wpb = 1 << (bshift - wbits);  // windows per bucket
   const unsigned ntiles = (unsigned)(nbuckets * wpb);

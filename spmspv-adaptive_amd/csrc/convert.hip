@@ -52,12 +52,12 @@ __global__ void k_gather_csc(const unsigned* __restrict__ skey, const unsigned* 
 
 __global__ void k_set_ll(long long* p, long long v) { *p = v; }
 
-void launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col, const float* val,
+hipError_t launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col, const float* val,
                               int m, int n, int nnz, int* colptr, int* crow, float* cval,
                               const SortBufs& b, long long* n_dev, Ctl* ctl, Ep ep) {
   if (nnz == 0) {
     launch_fill_i32(s, colptr, (long long)n + 1, 0);
-    return;
+    return hipGetLastError();
   }
   const int g = (int)std::min<long long>(8192, ((long long)nnz + 255) / 256);
   // row id of every nonzero, parked in crow until the gather below
@@ -73,7 +73,7 @@ void launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col,
   int* rows_out = reinterpret_cast<int*>(fb ? b.keyA : b.keyB);
   hipLaunchKernelGGL(k_gather_csc, dim3(g), dim3(256), 0, s, sk, si, crow, val, nnz, n, rows_out,
                      cval, colptr);
-  hipMemcpyAsync(crow, rows_out, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, s);
+  return hipMemcpyAsync(crow, rows_out, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, s);
 }
 
 // ------------------------------------------------------- mat features

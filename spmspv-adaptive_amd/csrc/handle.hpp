@@ -86,6 +86,11 @@ struct spmspv_handle_s {
   int team_col = 16;
   int team_row = 8;
   int row_vt = 7;  // row-merge items per thread (autotuned)
+  int xb_nb = 1;   // column blocks of the row-merge copy (1: plain CSR)
+  void* xmem = nullptr;
+  size_t xmem_bytes = 0;
+  bool xb_ready = false;
+  spmspv::XStore xs{};
 
   // host mirrors
   int* h_pin = nullptr;       // small pinned readback area (64 ints)
@@ -104,6 +109,8 @@ namespace spmspv {
 int set_error(spmspv_handle_t h, int code, const std::string& msg);
 int ensure_arena(spmspv_handle_t h);
 int ensure_row_tiles(spmspv_handle_t h);
+// Builds the column-blocked CSR copy (xb_nb blocks) for the row-merge kernel.
+int ensure_xblocks(spmspv_handle_t h);
 int ensure_mat_features(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
 int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);

@@ -101,6 +101,43 @@ void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, unsigned l
                    int* ykey, float* yval, long long binlen_hint, long long n_cap, Ep ep);
 
 // --- row path ------------------------------------------------------------
+// Column-blocked copy of a CSR (nb <= 8 blocks over column ranges, each a
+// CSR over all m rows whose row pointer starts at 0).
+struct XBlocks {
+  const int* row_ptr;  // nb * (m + 1)
+  const int* col;      // block-major
+  const float* val;
+  const int* split;    // per block ntiles[b] + 1 entries at split_base[b]
+  long long col_base[8];
+  int nnz[8];
+  int ntiles[8];
+  int tile_base[8];
+  int split_base[8];
+  int nb;
+};
+void launch_row_merge_x(hipStream_t s, bool masked, bool hits, const XBlocks& xb, const float* x,
+                        const unsigned* xbits, int m, int vt, float* ypart, unsigned char* marks,
+                        float* carry_val, int* carry_row, int* carry_hit, float* y);
+// Device memory of one column-blocked copy: byte offsets inside one buffer.
+struct XStoreLayout {
+  size_t bounds, base, bptr, sums, col, val, split, carry_val, carry_row, carry_hit, ypart, bytes;
+  int max_tiles;
+};
+XStoreLayout xstore_layout(int m, int nnz, int nb, int tile);
+struct XStore {
+  XBlocks xb;
+  float* ypart;
+  float* carry_val;
+  int* carry_row;
+  int* carry_hit;
+  int tile;
+};
+// Builds the nb-block copy of a CSR with sorted rows into `mem`
+// (xstore_layout(...).bytes); reads the block sizes back through the pinned
+// `h_pin` (nb + 1 ints) and synchronises the stream.
+hipError_t build_xblocks(hipStream_t s, const int* row_ptr, const int* col, const float* val,
+                         const int* colptr, int m, int n, int nnz, int nb, int tile, char* mem,
+                         int* h_pin, XStore* out);
 void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles,
                       int tile);
 // vt in {5, 7, 11}: merged items per thread (tile = 256 * vt)
@@ -113,7 +150,7 @@ void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                       float* y, unsigned* ybits);
 
 // --- CSC construction ----------------------------------------------------
-void launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col, const float* val,
+hipError_t launch_csr_to_csc_sorted(hipStream_t s, const int* row_ptr, const int* col, const float* val,
                               int m, int n, int nnz, int* colptr, int* crow, float* cval,
                               const SortBufs& b, long long* n_dev, Ctl* ctl, Ep ep);
 
