@@ -110,12 +110,6 @@ int spmspv_csr5_postprocess(spmspv_handle_t h);
 int spmspv_set_sigma(spmspv_handle_t h, int sigma);
 /* set_spmv_type, spmspv.h:1766-1771: 0 Hola-style, 1 CSR5 for K5/K7. */
 int spmspv_set_spmv_type(spmspv_handle_t h, int code);
-/* No counterpart in the reference: K5/K7 on a copy of the CSR split into
- * nb (1, 2, 4 or 8) column ranges of equal nnz, one range per group of XCDs,
- * so that each XCD's L2 holds only its range of x; the partial results are
- * added in block order. nb = 1 (default) runs on the CSR as given. The copy
- * costs 8 * nnz + 8 * nb * m bytes of device memory. */
-int spmspv_set_col_blocks(spmspv_handle_t h, int nb);
 
 /* ---- x input ------------------------------------------------------------ */
 /* set_vector_type, spmspv.h:977-982. */

@@ -100,6 +100,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
   int* ykt = c.take<int>(m);
   float* yvt = c.take<float>(m);
   float* xdi = c.take<float>(n);
+  float* xen = c.take<float>(n);
   unsigned* bi = c.take<unsigned>(nwords + 2);
   int* xki = c.take<int>(n);
   float* xvi = c.take<float>(n);
@@ -128,6 +129,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
     h->yk_tmp = ykt;
     h->yv_tmp = yvt;
     h->x_dense_int = xdi;
+    h->x_enc = xen;
     h->bits_int = bi;
     h->xk_int = xki;
     h->xv_int = xvi;
@@ -171,30 +173,6 @@ int ensure_row_tiles(spmspv_handle_t h) {
   rc = check_launch(h, "row split");
   if (rc) return rc;
   h->row_ready = true;
-  return SPMSPV_SUCCESS;
-}
-
-static int need_csc(spmspv_handle_t h);
-
-int ensure_xblocks(spmspv_handle_t h) {
-  const int tile = kRowNT * h->row_vt;
-  if (h->xb_ready && h->xs.xb.nb == h->xb_nb && h->xs.tile == tile) return SPMSPV_SUCCESS;
-  if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "row kernels need CSR input");
-  int rc = ensure_arena(h);
-  if (rc || (rc = need_csc(h))) return rc;
-  const XStoreLayout L = xstore_layout(h->m, h->nnz, h->xb_nb, tile);
-  if (h->xmem_bytes < L.bytes) {
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
-    if (h->xmem) HIP_TRY(h, hipFree(h->xmem));
-    h->xmem = nullptr;
-    h->xmem_bytes = 0;
-    HIP_TRY(h, hipMalloc(&h->xmem, L.bytes));
-    h->xmem_bytes = L.bytes;
-  }
-  HIP_TRY(h, build_xblocks(h->stream, h->csr_row, h->csr_col, h->csr_val, h->csc_colptr, h->m, h->n,
-                           h->nnz, h->xb_nb, tile, static_cast<char*>(h->xmem), h->h_pin, &h->xs));
-  h->launches += 10 + h->xb_nb;
-  h->xb_ready = true;
   return SPMSPV_SUCCESS;
 }
 
@@ -252,6 +230,22 @@ static int need_bits(spmspv_handle_t h) {
   }
   if (h->dense_valid) return spmspv_dense2bitarray(h);
   return set_error(h, SPMSPV_VEC_FORMAT_ERR, "x is not set");
+}
+
+// Support encoding of x for the row-merge kernels: from the sparse form
+// directly, else from the dense values and the bit array.
+static int need_enc_x(spmspv_handle_t h) {
+  if (h->enc_valid) return SPMSPV_SUCCESS;
+  if (h->sparse_valid) {
+    launch_fill_f32(h->stream, h->x_enc, h->n, 0.f);
+    launch_scatter(h->stream, h->x_key, h->x_val, h->xnnz, nullptr, h->n, h->x_enc, true);
+  } else {
+    int rc;
+    if ((rc = need_dense_x(h)) || (rc = need_bits(h))) return rc;
+    launch_encode_x(h->stream, h->dense_x(), h->bits(), h->n, h->x_enc);
+  }
+  h->enc_valid = true;
+  return check_launch(h, "encode x");
 }
 
 Ep next_ep(spmspv_handle_t h, unsigned k) {
@@ -374,25 +368,22 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       // writing into the internal accumulator always tracks hits so that
       // the compaction can restore its all-zero state
       const bool hits = want_sparse || !y_dense;
-      if ((rc = need_dense_x(h))) return rc;
-      if ((masked || hits) && (rc = need_bits(h))) return rc;
-      if (lb && h->xb_nb > 1) {
-        if ((rc = ensure_xblocks(h))) return rc;
-        launch_row_merge_x(s, masked, hits, h->xs.xb, h->dense_x(), h->bits(), m, h->row_vt,
-                           h->xs.ypart, h->marks, h->xs.carry_val, h->xs.carry_row, h->xs.carry_hit,
-                           yout);
-        if (hits)
-          launch_compact(s, nullptr, h->marks, m, yout, yout == h->acc ? 1 : 0, y_key, y_val,
-                         h->scan_state, h->ctl, next_ep(h, 1));
-        h->launches += 4;
-        return check_launch(h, "column-blocked row kernel");
-      }
       if (lb) {
+        const float* xin = nullptr;
+        if (masked || hits) {
+          if ((rc = need_enc_x(h))) return rc;
+          xin = h->x_enc;
+        } else {
+          if ((rc = need_dense_x(h))) return rc;
+          xin = h->dense_x();
+        }
         if ((rc = ensure_row_tiles(h))) return rc;
-        launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
-                         m, h->nnz, h->row_split, h->n_row_tiles, h->row_vt, yout, h->bitmap,
-                         h->row_carry_val, h->row_carry_row, h->row_carry_hit);
+        launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, xin, nullptr, m, h->nnz,
+                         h->row_split, h->n_row_tiles, h->row_vt, yout, h->bitmap, h->row_carry_val,
+                         h->row_carry_row, h->row_carry_hit);
       } else {
+        if ((rc = need_dense_x(h))) return rc;
+        if ((masked || hits) && (rc = need_bits(h))) return rc;
         launch_row_naive(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, h->dense_x(), h->bits(),
                          m, h->team_row, yout, h->bitmap);
       }
@@ -450,7 +441,6 @@ int spmspv_destroy(spmspv_handle_t h) {
   spmspv_bfs_release(h);
   if (h->arena) (void)hipFree(h->arena);
   if (h->csc_alloc) (void)hipFree(h->csc_alloc);
-  if (h->xmem) (void)hipFree(h->xmem);
   if (h->flush_buf) (void)hipFree(h->flush_buf);
   if (h->h_pin) (void)hipHostFree(h->h_pin);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -479,7 +469,6 @@ int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* 
   h->csr_col = col;
   h->csr_val = val;
   h->row_ready = false;
-  h->xb_ready = false;
   h->mat_ready = false;
   return SPMSPV_SUCCESS;
 }
@@ -571,18 +560,6 @@ int spmspv_set_spmv_type(spmspv_handle_t h, int code) {
   return SPMSPV_SUCCESS;
 }
 
-int spmspv_set_col_blocks(spmspv_handle_t h, int nb) {
-  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
-  if (nb != 1 && nb != 2 && nb != 4 && nb != 8)
-    return set_error(h, SPMSPV_INVALID_ARGUMENT, "column blocks must be 1, 2, 4 or 8");
-  if (nb != h->xb_nb) {
-    h->xb_nb = nb;
-    h->xb_ready = false;
-    spmspv_bfs_release(h);  // captured levels hold the old kernels
-  }
-  return SPMSPV_SUCCESS;
-}
-
 int spmspv_set_vector_type(spmspv_handle_t h, int code) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   if (code < 0 || code > 2) return set_error(h, SPMSPV_INVALID_ARGUMENT, "vector type must be 0..2");
@@ -601,6 +578,7 @@ int spmspv_set_sparse_x(spmspv_handle_t h, int* key, float* val, int xnnz) {
   h->sparse_valid = true;
   h->dense_valid = false;
   h->bits_valid = false;
+  h->enc_valid = false;
   h->prep_valid = false;
   h->binlen_host = -1;
   return SPMSPV_SUCCESS;
@@ -614,10 +592,12 @@ int spmspv_set_x(spmspv_handle_t h, float* x) {
     h->dense_valid = x != nullptr;
     h->sparse_valid = false;
     h->bits_valid = false;
+    h->enc_valid = false;
     h->prep_valid = false;
     h->binlen_host = -1;
   } else {
     h->dense_valid = false;
+    h->enc_valid = false;
   }
   return SPMSPV_SUCCESS;
 }
@@ -629,6 +609,7 @@ int spmspv_set_bitvector(spmspv_handle_t h, int* bits, int num_ints) {
   h->bits_user = reinterpret_cast<unsigned*>(bits);
   h->num_ints_user = num_ints;
   h->bits_valid = false;
+  h->enc_valid = false;
   return SPMSPV_SUCCESS;
 }
 
@@ -637,6 +618,7 @@ int spmspv_set_sparse_x_dense_x(spmspv_handle_t h, int* key, float* val, float* 
   if (rc) return rc;
   h->x_dense_user = x;
   h->dense_valid = false;  // the caller's dense copy is filled on demand
+  h->enc_valid = false;
   return SPMSPV_SUCCESS;
 }
 
@@ -658,6 +640,7 @@ int spmspv_sparse2bitarray(spmspv_handle_t h) {
   int rc = spmspv_sparse2dense(h);
   if (rc) return rc;
   h->bits_valid = false;
+  h->enc_valid = false;
   rc = need_bits(h);
   if (rc) return rc;
   h->vec_type = SPMSPV_VEC_BITARRAY;
@@ -702,6 +685,7 @@ int spmspv_dense2sparse(spmspv_handle_t h) {
   h->xnnz = cnt;
   h->sparse_valid = true;
   h->bits_valid = false;
+  h->enc_valid = false;
   h->prep_valid = false;
   h->binlen_host = -1;
   h->vec_type = SPMSPV_VEC_SPARSE;

@@ -416,7 +416,7 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
         launch_row_naive(s, true, true, h->csr_row, h->csr_col, h->csr_val, b->ones, b->fbits[cur],
                          h->m, h->team_row, b->ybuf, h->bitmap);
       else
-        launch_row_merge(s, true, true, h->csr_row, h->csr_col, h->csr_val, b->ones, b->fbits[cur],
+        launch_row_merge(s, true, true, h->csr_row, h->csr_col, h->csr_val, nullptr, b->fbits[cur],
                          h->m, h->nnz, h->row_split, h->n_row_tiles, h->row_vt, b->ybuf, h->bitmap,
                          h->row_carry_val, h->row_carry_row, h->row_carry_hit);
       hipLaunchKernelGGL(k_claim_bits, dim3(std::min(kBfsGrid, (words + 255) / 256)), dim3(256), 0, s,

@@ -423,18 +423,35 @@ void launch_fill_i32(hipStream_t s, int* p, long long n, int v) {
 }
 
 __global__ void k_scatter(const int* __restrict__ key, const float* __restrict__ val, int nnz_arg,
-                          const int* nnz_dev, int len, float* __restrict__ dense) {
+                          const int* nnz_dev, int len, float* __restrict__ dense, int encode) {
   const int nnz = nnz_dev ? *nnz_dev : nnz_arg;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
     const int k = key[i];
-    if ((unsigned)k < (unsigned)len) dense[k] = val[i];
+    const float v = val[i];
+    if ((unsigned)k < (unsigned)len) dense[k] = (encode && v == 0.f) ? -0.f : v;
   }
 }
 void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, const int* nnz_dev,
-                    int len, float* dense) {
+                    int len, float* dense, bool encode) {
   int grid = nnz_dev ? 2048 : std::max(1, std::min(8192, (nnz + 255) / 256));
   if (!nnz_dev && nnz <= 0) return;
-  hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, key, val, nnz, nnz_dev, len, dense);
+  hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, key, val, nnz, nnz_dev, len, dense,
+                     encode ? 1 : 0);
+}
+
+__global__ void k_encode_x(const float* __restrict__ x, const unsigned* __restrict__ bits, int n,
+                           float* __restrict__ enc) {
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x) {
+    const bool in = (bits[c >> 5] >> (c & 31)) & 1u;
+    const float v = x[c];
+    enc[c] = in ? (v == 0.f ? -0.f : v) : 0.f;
+  }
+}
+void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_encode_x, dim3(std::min(8192, (n + 255) / 256)), dim3(256), 0, s, x, bits, n,
+                     enc);
 }
 
 __global__ void k_set_bits(const int* __restrict__ key, int nnz_arg, const int* nnz_dev, int len,

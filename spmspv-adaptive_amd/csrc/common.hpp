@@ -92,6 +92,48 @@ __device__ __forceinline__ T wave_min(T v) {
   return v;
 }
 
+// ---- DPP lane moves (gfx9 data-parallel primitives): a lane whose source
+// lane is outside the pattern, or whose row is masked off, keeps `old`.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i32(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f32(float old, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, v), CTRL,
+                                                               ROWS, 0xf, false));
+}
+
+// One step of a segmented inclusive scan: (f, v, h) of this lane combined
+// with the running value of the lane CTRL points at. f marks that a
+// segment starts inside the lane's range; v sums and h ORs up to it.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(int& f, float& v, int& h) {
+  const int fo = dpp_i32<CTRL, ROWS>(0, f);
+  const float vo = dpp_f32<CTRL, ROWS>(0.f, v);
+  const int ho = dpp_i32<CTRL, ROWS>(0, h);
+  if (!f) {
+    v += vo;
+    h |= ho;
+  }
+  f |= fo;
+}
+
+// Wave-wide segmented inclusive scan in six DPP steps (rows of 16 lanes,
+// then the row broadcasts of lane 15 and lane 31).
+__device__ __forceinline__ void wave_seg_scan(int& f, float& v, int& h) {
+  seg_step<kDppRowShr1, 0xf>(f, v, h);
+  seg_step<kDppRowShr2, 0xf>(f, v, h);
+  seg_step<kDppRowShr4, 0xf>(f, v, h);
+  seg_step<kDppRowShr8, 0xf>(f, v, h);
+  seg_step<kDppRowBcast15, 0xa>(f, v, h);
+  seg_step<kDppRowBcast31, 0xc>(f, v, h);
+}
+
 // Block-wide exclusive sum of one value per thread; returns the exclusive
 // prefix and writes the block total to *total. `lds` needs NT/64 slots.
 template <int NT, typename T>

@@ -40,6 +40,7 @@ struct spmspv_handle_s {
   bool dense_valid = false;   // dense x holds the current x
   bool bits_valid = false;    // bit array holds the current x's support
   bool prep_valid = false;    // ptr/xbeg/split/bin_len computed for the current x
+  bool enc_valid = false;     // x_enc holds the support encoding of the current x
   long long binlen_host = -1; // bin_len read back for the current x (or -1)
   int spmv_type = 0;          // 0 merge (Hola role), 1 CSR5
   int sigma = -1;
@@ -64,6 +65,7 @@ struct spmspv_handle_s {
   int* yk_tmp = nullptr;     // m
   float* yv_tmp = nullptr;   // m
   float* x_dense_int = nullptr;  // n
+  float* x_enc = nullptr;        // n: +0 outside x's support, nonzero bits inside
   unsigned* bits_int = nullptr;  // ceil(n/32)
   int* xk_int = nullptr;     // n (x converted from dense)
   float* xv_int = nullptr;   // n
@@ -86,11 +88,6 @@ struct spmspv_handle_s {
   int team_col = 16;
   int team_row = 8;
   int row_vt = 7;  // row-merge items per thread (autotuned)
-  int xb_nb = 1;   // column blocks of the row-merge copy (1: plain CSR)
-  void* xmem = nullptr;
-  size_t xmem_bytes = 0;
-  bool xb_ready = false;
-  spmspv::XStore xs{};
 
   // host mirrors
   int* h_pin = nullptr;       // small pinned readback area (64 ints)
@@ -109,8 +106,6 @@ namespace spmspv {
 int set_error(spmspv_handle_t h, int code, const std::string& msg);
 int ensure_arena(spmspv_handle_t h);
 int ensure_row_tiles(spmspv_handle_t h);
-// Builds the column-blocked CSR copy (xb_nb blocks) for the row-merge kernel.
-int ensure_xblocks(spmspv_handle_t h);
 int ensure_mat_features(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
 int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);

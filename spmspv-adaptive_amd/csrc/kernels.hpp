@@ -63,8 +63,12 @@ void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
 void launch_fill_f32(hipStream_t s, float* p, long long n, float v);
 void launch_fill_i32(hipStream_t s, int* p, long long n, int v);
+// dense[key[i]] = val[i]; with `encode`, a zero value is stored as -0.0
+// (the support encoding: +0.0 outside the support, nonzero bits inside).
 void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, const int* nnz_dev,
-                    int len, float* dense);
+                    int len, float* dense, bool encode = false);
+// enc[c] = bit c of bits ? (x[c] == 0 ? -0.0 : x[c]) : +0.0
+void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc);
 void launch_set_bits(hipStream_t s, const int* key, int nnz, const int* nnz_dev, int len,
                      unsigned* bits);
 void launch_clear_keys(hipStream_t s, const int* key, int nnz, float* dense, unsigned* bits);
@@ -101,46 +105,16 @@ void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, unsigned l
                    int* ykey, float* yval, long long binlen_hint, long long n_cap, Ep ep);
 
 // --- row path ------------------------------------------------------------
-// Column-blocked copy of a CSR (nb <= 8 blocks over column ranges, each a
-// CSR over all m rows whose row pointer starts at 0).
-struct XBlocks {
-  const int* row_ptr;  // nb * (m + 1)
-  const int* col;      // block-major
-  const float* val;
-  const int* split;    // per block ntiles[b] + 1 entries at split_base[b]
-  long long col_base[8];
-  int nnz[8];
-  int ntiles[8];
-  int tile_base[8];
-  int split_base[8];
-  int nb;
-};
-void launch_row_merge_x(hipStream_t s, bool masked, bool hits, const XBlocks& xb, const float* x,
-                        const unsigned* xbits, int m, int vt, float* ypart, unsigned char* marks,
-                        float* carry_val, int* carry_row, int* carry_hit, float* y);
-// Device memory of one column-blocked copy: byte offsets inside one buffer.
-struct XStoreLayout {
-  size_t bounds, base, bptr, sums, col, val, split, carry_val, carry_row, carry_hit, ypart, bytes;
-  int max_tiles;
-};
-XStoreLayout xstore_layout(int m, int nnz, int nb, int tile);
-struct XStore {
-  XBlocks xb;
-  float* ypart;
-  float* carry_val;
-  int* carry_row;
-  int* carry_hit;
-  int tile;
-};
-// Builds the nb-block copy of a CSR with sorted rows into `mem`
-// (xstore_layout(...).bytes); reads the block sizes back through the pinned
-// `h_pin` (nb + 1 ints) and synchronises the stream.
-hipError_t build_xblocks(hipStream_t s, const int* row_ptr, const int* col, const float* val,
-                         const int* colptr, int m, int n, int nnz, int nb, int tile, char* mem,
-                         int* h_pin, XStore* out);
+// Workgroups of `block` threads of `kernel` resident on the whole device
+// at once (occupancy x CUs), cached per kernel.
+int resident_blocks(const void* kernel, int block);
 void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles,
                       int tile);
-// vt in {5, 7, 11}: merged items per thread (tile = 256 * vt)
+// vt in {5, 7, 11}: merged items per thread (tile = 256 * vt).
+// x == nullptr: x is given by its bit array xbits alone (x = 1 on it; the
+// kernel runs masked with hits). Otherwise, with masked or hits, x must be
+// the support-encoded vector (launch_encode_x / launch_scatter with
+// encode); without either it is plain dense x.
 void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
                       const int* split, int ntiles, int vt, float* y, unsigned* ybits,

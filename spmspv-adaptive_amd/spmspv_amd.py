@@ -71,7 +71,6 @@ _SIGS = {
     "spmspv_csr5_postprocess": [_vp],
     "spmspv_set_sigma": [_vp, _i],
     "spmspv_set_spmv_type": [_vp, _i],
-    "spmspv_set_col_blocks": [_vp, _i],
     "spmspv_set_vector_type": [_vp, _i],
     "spmspv_set_sparse_x": [_vp, _vp, _vp, _i],
     "spmspv_set_x": [_vp, _vp],
@@ -273,10 +272,6 @@ class SpmspvHandle:
 
     def set_spmv_type(self, code):
         return self._c(lib().spmspv_set_spmv_type(self.h, code))
-
-    def set_col_blocks(self, nb):
-        """K5/K7 on an nb-way column-blocked copy (1 = the CSR as given)."""
-        return self._c(lib().spmspv_set_col_blocks(self.h, nb))
 
     # ---- x (spmspv.h:977-1025)
     def set_vector_type(self, code):

@@ -267,65 +267,6 @@ def test_cpp_handle_adaptive_app(gpu):
     assert out.stdout.count("RESULT OK") == 6
 
 
-@pytest.mark.parametrize("nb", [2, 4, 8])
-def test_column_blocked_row_merge(rmat12, nb):
-    """K5/K7 on the nb-way column-blocked copy: same support and bound as
-    the oracle, sparse and dense outputs, every x density; then back to the
-    plain CSR."""
-    M, (cp, ri, cv) = rmat12
-    M.A.set_col_blocks(nb)
-    try:
-        for k in (1, 40, 400, M.n):
-            key, val = sa.gen_sparse_vector(M.n, k, 500 + k + nb)
-            for slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
-                say(f"col blocks {nb} xnnz {k} slot {slot}")
-                yk, yv, yd = run_slot(M, slot, key, val, want_dense=True)
-                check_result(M.m, cp, ri, cv, key, val, yk, yv, f"nb={nb} k={k} slot {slot}")
-                dense = np.zeros(M.m, np.float32)
-                dense[yk] = yv
-                np.testing.assert_array_equal(yd, dense)
-                # dense output only (no support tracking)
-                yd2 = sa.DeviceArray(M.m, np.float32)
-                M.A.run(slot, y_dense=yd2, want_nnz=False)
-                np.testing.assert_array_equal(yd2.to_host(), dense)
-    finally:
-        M.A.set_col_blocks(1)
-    key, val = sa.gen_sparse_vector(M.n, 300, 4)
-    yk, yv, _ = run_slot(M, sa.K_SPMV_LB, key, val)
-    check_result(M.m, cp, ri, cv, key, val, yk, yv, "back to plain CSR")
-
-
-def test_column_blocked_unsorted_rows_and_hub(gpu):
-    """The blocked copy keeps row order for rows whose columns are not
-    sorted, and joins rows that span many tiles in every block."""
-    rng = np.random.default_rng(11)
-    m = n = 20000
-    rows = np.concatenate([np.full(30000, 17), rng.integers(0, m, 60000)])
-    cols = np.concatenate([rng.permutation(n)[:30000] if n >= 30000 else rng.integers(0, n, 30000),
-                           rng.integers(0, n, 60000)])
-    import scipy.sparse as sp
-    A = sp.csr_matrix((rng.uniform(-1, 1, rows.size).astype(np.float32), (rows, cols)), shape=(m, n))
-    A.sum_duplicates()
-    A.sort_indices()
-    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float32)
-    # reverse the column order inside every row
-    col2, val2 = col.copy(), val.copy()
-    for r in range(m):
-        a, b = rp[r], rp[r + 1]
-        col2[a:b] = col[a:b][::-1]
-        val2[a:b] = val[a:b][::-1]
-    M = build(m, n, rp, col2, val2)
-    cp, ri, cv = csc_of(m, n, rp, col, val)
-    for nb in (4, 8):
-        M.A.set_col_blocks(nb)
-        for k in (3, 2000, n):
-            key, v = sa.gen_sparse_vector(n, k, 70 + k)
-            for slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
-                say(f"unsorted rows nb {nb} xnnz {k} slot {slot}")
-                yk, yv, _ = run_slot(M, slot, key, v)
-                check_result(m, cp, ri, cv, key, v, yk, yv, f"unsorted nb={nb} k={k} slot {slot}")
-
-
 def test_autotune_then_parity(rmat12):
     """The autotuner changes team sizes and the row tile width per matrix;
     every slot must stay exact afterwards, for every candidate width."""
