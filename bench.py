@@ -33,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "spmspv-adaptive_amd"))
 HBM_PEAK_GBS = 8000.0
 DENSITIES = [1e-4, 1e-3, 1e-2, 0.1, 0.5, 1.0]
 WEIGHTS = os.path.join(ROOT, "spmspv-adaptive_amd", "selector_weights.npy")
+AUTO_SLOT = 99  # --only-slot value that times the selector-dispatched call
 
 
 def alg_bytes(xnnz, bin_len, ynnz):
@@ -63,11 +64,155 @@ def profile_slot(sa, A, n, args, yk, yv):
         A.set_sparse_x(dk, dv, k)
         A.flush_cache(args.flush_mb << 20)
         A.timer_start()
-        ynnz = A.run(args.only_slot, y_key=yk, y_val=yv)
+        if args.only_slot == AUTO_SLOT:
+            ynnz, _ = A.run_auto(y_key=yk, y_val=yv)
+        else:
+            ynnz = A.run(args.only_slot, y_key=yk, y_val=yv)
         ms.append(A.timer_stop())
-    print(json.dumps({"slot": sa.KERNEL_NAMES[args.only_slot], "density": args.density, "xnnz": k,
+    name = "auto" if args.only_slot == AUTO_SLOT else sa.KERNEL_NAMES[args.only_slot]
+    print(json.dumps({"slot": name, "density": args.density, "xnnz": k,
                       "bin_len": int(bl), "ynnz": int(ynnz), "ms": ms,
                       "alg_bytes": alg_bytes(k, bl, ynnz), "calib_bytes": args.calib_mb << 20}), flush=True)
+
+
+def run_bfs(sa, args, A, rp, col, n):
+    """BFS: fixed slots vs per-level selection, on the SpMSpV graph or on a
+    separate R-MAT of --bfs-scale (the level-synchronous BFS workload:
+    scale 22, 64 sources). Levels of the first --bfs-check sources are
+    compared with the CPU BFS for every policy."""
+    bscale = args.bfs_scale or args.scale
+    if bscale == args.scale:
+        B, brp, bcol, bn = A, rp, col, n
+    else:
+        bn = 1 << bscale
+        brp, bcol, bval = sa.gen_rmat_csr(bscale, 16, 0.57, 0.19, 0.19, seed=0x5EED0004)
+        BM = sa.Matrix(bn, bn, brp, bcol, bval)
+        B = BM.A
+        if os.path.exists(WEIGHTS):
+            B.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
+        else:
+            B.selector_mode(1)
+    deg = np.diff(brp)
+    cand = np.nonzero(deg > 0)[0]
+    rng = np.random.default_rng(0x5EED0003)
+    srcs = rng.choice(cand, size=min(args.bfs_sources, cand.size), replace=False)
+    lev = sa.DeviceArray(bn, np.int32)
+    refs = {}
+    if args.bfs_check > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        for s_ in srcs[:args.bfs_check]:
+            refs[int(s_)] = oracle.bfs(bn, brp, bcol, int(s_))[0]  # symmetric: CSR = CSC
+    res, levels, checked = {}, {}, 0
+    for pol in [-1] + list(range(sa.NUM_SLOTS)):
+        B.bfs(int(srcs[0]), lev, policy=pol, use_graphs=1)  # warm-up + graph capture
+        secs, edges = 0.0, 0.0
+        for s_ in srcs:
+            st = B.bfs(int(s_), lev, policy=pol, use_graphs=1)
+            secs += st[2]
+            edges += st[1]
+            if int(s_) in refs:
+                if not np.array_equal(lev.to_host(), refs[int(s_)]):
+                    raise SystemExit(f"BFS levels differ from the CPU BFS: policy {pol}, source {s_}")
+                checked += 1
+            levels[int(s_)] = int(st[0])
+        res["learned" if pol < 0 else sa.KERNEL_NAMES[pol]] = edges / secs / 1e9
+    fixed_best = max(v for kk, v in res.items() if kk != "learned")
+    bfs = {"graph": f"rmat{bscale}", "sources": int(len(srcs)), "gteps": res,
+           "learned_gteps": res["learned"], "best_fixed_gteps": fixed_best,
+           "levels_checked_vs_cpu": checked,
+           "mean_levels": round(float(np.mean(list(levels.values()))), 2)}
+    return bfs
+
+
+def run_batched_bench(sa, args):
+    """Batched multi-vector SpMSpV: --batched-vectors vectors (density
+    log-uniform in [1e-4, 0.2], random support, U[-1,1]) against one R-MAT
+    of --batched-scale, as one spmspv_run_batched call vs the same vectors
+    as sequential selector-dispatched calls (both warm, wall clock around
+    the whole set, outputs resident on the device). The batched results
+    are compared with the sequential ones (index sets exact, values within
+    1e-5 relative)."""
+    scale, nvec = args.batched_scale, args.batched_vectors
+    n = 1 << scale
+    t0 = time.time()
+    rp, col, val = sa.gen_rmat_csr(scale, 16, 0.57, 0.19, 0.19, seed=0x5EED0005)
+    M = sa.Matrix(n, n, rp, col, val)
+    A = M.A
+    if os.path.exists(WEIGHTS):
+        A.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
+    else:
+        A.selector_mode(1)
+    deg = np.diff(rp).astype(np.int64)
+    rng = np.random.default_rng(0x5EED0006)
+    dens = np.exp(rng.uniform(np.log(1e-4), np.log(0.2), nvec))
+    keys, vals, bls = [], [], []
+    for v in range(nvec):
+        k = max(1, int(round(dens[v] * n)))
+        key, xv = sa.gen_sparse_vector(n, k, 0x5EED1000 + v)
+        keys.append(key)
+        vals.append(xv)
+        bls.append(int(deg[key].sum()))
+    vptr = np.zeros(nvec + 1, np.int32)
+    vptr[1:] = np.cumsum([k.size for k in keys])
+    cap = np.minimum(np.array(bls, np.int64), n)
+    yoff = np.zeros(nvec + 1, np.int64)
+    yoff[1:] = np.cumsum(cap)
+    d_vptr = sa.DeviceArray.from_host(vptr, np.int32)
+    d_vkey = sa.DeviceArray.from_host(np.concatenate(keys), np.int32)
+    d_vval = sa.DeviceArray.from_host(np.concatenate(vals), np.float32)
+    d_yoff = sa.DeviceArray.from_host(yoff, np.int64)
+    d_yk = sa.DeviceArray(int(yoff[-1]), np.int32)
+    d_yv = sa.DeviceArray(int(yoff[-1]), np.float32)
+    d_yn = sa.DeviceArray(nvec, np.int32)
+    s_yk = sa.DeviceArray(int(yoff[-1]), np.int32)
+    s_yv = sa.DeviceArray(int(yoff[-1]), np.float32)
+    setup_s = time.time() - t0
+    # warm-up of both paths
+    A.run_batched(d_vptr, d_vkey, d_vval, d_yoff, d_yk, d_yv, d_yn)
+    bt, st = [], []
+    seq_nnz = np.zeros(nvec, np.int64)
+    seq_used = np.zeros(nvec, np.int32)
+    for rep in range(args.batched_reps):
+        sa.synchronize()
+        t = time.perf_counter()
+        used = A.run_batched(d_vptr, d_vkey, d_vval, d_yoff, d_yk, d_yv, d_yn)
+        sa.synchronize()
+        bt.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        for v in range(nvec):
+            o = int(yoff[v])
+            A.set_sparse_x_ptr(d_vkey.ptr.value + 4 * int(vptr[v]), d_vval.ptr.value + 4 * int(vptr[v]),
+                               int(vptr[v + 1] - vptr[v]))
+            seq_nnz[v], seq_used[v] = A.run_auto_ptr(s_yk.ptr.value + 4 * o, s_yv.ptr.value + 4 * o)
+        sa.synchronize()
+        st.append(time.perf_counter() - t)
+    yn = d_yn.to_host()
+    # compare with the sequential calls: counts, slots and index sets exact;
+    # values agree up to summation order (the atomic slots add in any order)
+    bk, bv, sk, sv = d_yk.to_host(), d_yv.to_host(), s_yk.to_host(), s_yv.to_host()
+    diff_nnz = int(np.sum(yn != seq_nnz))
+    diff_slot = int(np.sum(used != seq_used))
+    diff_idx, max_dv = 0, 0.0
+    for v in range(nvec):
+        o, c = int(yoff[v]), int(min(yn[v], seq_nnz[v]))
+        if not np.array_equal(bk[o:o + c], sk[o:o + c]):
+            diff_idx += 1
+        elif c:
+            max_dv = max(max_dv, float(np.max(np.abs(bv[o:o + c].astype(np.float64) - sv[o:o + c]))))
+    same = diff_nnz == 0 and diff_slot == 0 and diff_idx == 0
+    b_ms, s_ms = 1e3 * float(np.median(bt)), 1e3 * float(np.median(st))
+    prods = int(np.sum(bls))
+    return {"graph": f"rmat{scale}", "nnz": int(M.nnz), "vectors": nvec,
+            "density": "log-uniform [1e-4, 0.2]", "total_bin_len": prods,
+            "batched_ms": round(b_ms, 3), "sequential_ms": round(s_ms, 3),
+            "speedup": round(s_ms / b_ms, 3),
+            "batched_gbs": round(sum(alg_bytes(keys[v].size, bls[v], int(yn[v])) for v in range(nvec))
+                                 / (b_ms * 1e-3) / 1e9, 2),
+            "slots_used": {sa.KERNEL_NAMES[k]: int((used == k).sum()) for k in np.unique(used)},
+            "matches_sequential": same, "vectors_differing": {"ynnz": diff_nnz, "slot": diff_slot,
+                                                              "index_set": diff_idx},
+            "max_abs_value_diff": max_dv, "setup_s": round(setup_s, 1)}
 
 
 def main():
@@ -78,11 +223,23 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--reps-fixed", type=int, default=3)
     ap.add_argument("--bfs-sources", type=int, default=8)
+    ap.add_argument("--bfs-scale", type=int, default=0,
+                    help="R-MAT scale of the BFS graph (0: the SpMSpV graph); 22 with 64 sources is the BFS workload")
+    ap.add_argument("--batched", action="store_true",
+                    help="run only the batched measurement (one JSON line)")
+    ap.add_argument("--batched-scale", type=int, default=24)
+    ap.add_argument("--batched-vectors", type=int, default=256)
+    ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--bfs-only", action="store_true",
+                    help="run only the BFS measurement (prints one JSON line with the BFS result)")
+    ap.add_argument("--bfs-check", type=int, default=2,
+                    help="sources whose levels are compared with the CPU BFS for every policy")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fixed", action="store_true")
     ap.add_argument("--flush-mb", type=int, default=512)
     ap.add_argument("--only-slot", type=int, default=-1,
-                    help="profiling mode: time one slot at --density only (for rocprofv3 runs)")
+                    help="profiling mode: time one slot at --density only (for rocprofv3 runs); "
+                         "99 = the selector-dispatched call")
     ap.add_argument("--density", type=float, default=1.0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--calib-mb", type=int, default=0,
@@ -94,6 +251,17 @@ def main():
         sys.exit(2)
 
     import spmspv_amd as sa
+
+    if args.batched:
+        print(json.dumps({"batched": run_batched_bench(sa, args)}), flush=True)
+        return
+
+    if args.bfs_only:
+        if not args.bfs_scale:
+            args.bfs_scale = args.scale
+        args.scale = -1  # the sweep's graph is not built
+        print(json.dumps({"bfs": run_bfs(sa, args, None, None, None, 0)}), flush=True)
+        return
 
     t_setup = time.time()
     n = 1 << args.scale
@@ -196,26 +364,7 @@ def main():
     dom_bytes = alg_bytes(xs[dom_i][0], meta[dom_i], ynnzs[dom_i])
     dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
 
-    # ---- BFS on the same graph: fixed slots vs per-level selection
-    bfs = None
-    if args.bfs_sources > 0:
-        deg = np.diff(rp)
-        cand = np.nonzero(deg > 0)[0]
-        rng = np.random.default_rng(0x5EED0003)
-        srcs = rng.choice(cand, size=min(args.bfs_sources, cand.size), replace=False)
-        lev = sa.DeviceArray(n, np.int32)
-        res = {}
-        for pol in [-1] + list(range(sa.NUM_SLOTS)):
-            A.bfs(int(srcs[0]), lev, policy=pol, use_graphs=1)  # warm-up + graph capture
-            secs, edges = 0.0, 0.0
-            for s in srcs:
-                st = A.bfs(int(s), lev, policy=pol, use_graphs=1)
-                secs += st[2]
-                edges += st[1]
-            res["learned" if pol < 0 else sa.KERNEL_NAMES[pol]] = edges / secs / 1e9
-        fixed_best = max(v for kk, v in res.items() if kk != "learned")
-        bfs = {"graph": f"rmat{args.scale}", "sources": int(len(srcs)), "gteps": res,
-               "learned_gteps": res["learned"], "best_fixed_gteps": fixed_best}
+    bfs = run_bfs(sa, args, A, rp, col, n) if args.bfs_sources > 0 else None
 
     # ---- CPU baseline: the reference's CPU SpMSpV (oracle port, 1 core)
     cpu = None

@@ -248,12 +248,17 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
                double stats[4]);
 
 /* ---- batched multi-vector SpMSpV (no reference counterpart) ------------- */
-/* nvec sparse vectors in CSR-of-vectors form: d_vptr[nvec+1], d_vkey,
- * d_vval; results in d_yptr[nvec+1] (capacity-based offsets given by the
- * caller), d_ykey, d_yval, d_ynnz[nvec]. One persistent work-queue launch. */
+/* nvec sparse vectors in CSR-of-vectors form: d_vptr[nvec+1] (device),
+ * d_vkey, d_vval. Vector v's sorted result goes to d_ykey/d_yval at the
+ * caller's device offsets d_yoff[v] .. d_yoff[v+1]-1 (capacity must be at
+ * least min(m, bin_len of v), else SPMSPV_INVALID_ARGUMENT before anything
+ * runs); its count to d_ynnz[v]. The selector picks each vector's slot in
+ * one pass over the batch; kernel_used (host, nullable) gets the slots.
+ * Two host synchronisations per batch (one sizing readback, one at the
+ * end) instead of two per vector. The handle's x is unset afterwards. */
 int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int* d_vkey,
                        const float* d_vval, const long long* d_yoff, int* d_ykey, float* d_yval,
-                       int* d_ynnz);
+                       int* d_ynnz, int* kernel_used);
 
 /* ---- utilities (the reference harness called the CUDA runtime directly) - */
 int spmspv_device_count(int* count);

@@ -46,12 +46,6 @@ int set_error(spmspv_handle_t h, int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(h, expr)                                                                     \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess)                                                                    \
-      return set_error(h, SPMSPV_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 static int check_launch(spmspv_handle_t h, const char* what) {
   hipError_t e = hipGetLastError();
@@ -196,7 +190,7 @@ int ensure_mat_features(spmspv_handle_t h) {
   return SPMSPV_SUCCESS;
 }
 
-static int need_csc(spmspv_handle_t h) {
+int need_csc(spmspv_handle_t h) {
   if (!h->csc_colptr) {
     if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "no matrix input");
     int rc = spmspv_to_csc(h);
@@ -442,6 +436,7 @@ int spmspv_destroy(spmspv_handle_t h) {
   if (h->arena) (void)hipFree(h->arena);
   if (h->csc_alloc) (void)hipFree(h->csc_alloc);
   if (h->flush_buf) (void)hipFree(h->flush_buf);
+  if (h->batch_stash) (void)hipFree(h->batch_stash);
   if (h->h_pin) (void)hipHostFree(h->h_pin);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);

@@ -55,6 +55,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
     __syncthreads();
     if (sel.mat) select_in_block<NT>(ctl, sel);
+    if (sel.stash && threadIdx.x == 0) {
+      sel.stash[0] = 0;
+      sel.stash[1] = sel.mat ? ctl->kernel_choice : -1;
+    }
     return;
   }
   if (t >= ntiles) return;
@@ -134,6 +138,11 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   }
   __syncthreads();
   if (sel.mat) select_in_block<NT>(ctl, sel);
+  // the selector's choice and bin_len were written by thread 0 as well
+  if (sel.stash && threadIdx.x == 0) {
+    sel.stash[0] = ctl->bin_len;
+    sel.stash[1] = sel.mat ? ctl->kernel_choice : -1;
+  }
 }
 
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,

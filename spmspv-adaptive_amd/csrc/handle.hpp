@@ -97,16 +97,29 @@ struct spmspv_handle_s {
   long long dev_levels = 0;  // device-epoch sequences run (captured BFS levels)
 
   BfsState* bfs = nullptr;
+  long long* batch_stash = nullptr;  // {bin_len, choice} per vector of a batch
+  int batch_cap = 0;
 
   float* dense_x() { return x_dense_user ? x_dense_user : x_dense_int; }
   unsigned* bits() { return bits_user ? bits_user : bits_int; }
 };
+
+// Returns SPMSPV_DEVICE_ERROR (with the failing call recorded) when a HIP
+// runtime call fails.
+#define HIP_TRY(h, expr)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(h, SPMSPV_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
 
 namespace spmspv {
 int set_error(spmspv_handle_t h, int code, const std::string& msg);
 int ensure_arena(spmspv_handle_t h);
 int ensure_row_tiles(spmspv_handle_t h);
 int ensure_mat_features(spmspv_handle_t h);
+// Builds the handle-owned CSC from the CSR when no CSC was given.
+int need_csc(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
 int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);
 int readback_ynnz(spmspv_handle_t h, int* y_nnz);

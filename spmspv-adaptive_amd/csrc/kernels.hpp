@@ -41,6 +41,7 @@ struct SelectorArgs {
   const MatFeat* mat;
   int mode;              // 0 mlp, 1 tree, 2 fixed
   int fixed;
+  long long* stash;      // when set: {bin_len, kernel choice} of this x are copied here
 };
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
                  const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,

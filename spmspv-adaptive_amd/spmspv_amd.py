@@ -114,7 +114,7 @@ _SIGS = {
     "spmspv_mlp_loss_grad": [_vp, _vp, _vp, _i, ctypes.POINTER(_f), _vp],
     "spmspv_mlp_forward": [_vp, _vp, _i, _vp],
     "spmspv_bfs": [_vp, _i, _vp, _i, _i, _vp],
-    "spmspv_run_batched": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "spmspv_run_batched": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "spmspv_device_count": [ctypes.POINTER(_i)],
     "spmspv_malloc": [ctypes.POINTER(_vp), _sz],
     "spmspv_free": [_vp],
@@ -281,6 +281,10 @@ class SpmspvHandle:
         return self._c(lib().spmspv_set_sparse_x(self.h, key.ptr if key else None,
                                                  val.ptr if val else None, xnnz))
 
+    def set_sparse_x_ptr(self, key_addr, val_addr, xnnz):
+        """set_sparse_x with raw device addresses (slices of larger arrays)."""
+        return self._c(lib().spmspv_set_sparse_x(self.h, _vp(key_addr), _vp(val_addr), xnnz))
+
     def set_x(self, x):
         return self._c(lib().spmspv_set_x(self.h, x.ptr if x else None))
 
@@ -357,6 +361,13 @@ class SpmspvHandle:
                                       ctypes.byref(c), ctypes.byref(k)))
         return c.value, k.value
 
+    def run_auto_ptr(self, ykey_addr, yval_addr):
+        """run_auto writing the sorted result at raw device addresses."""
+        c, k = _i(-1), _i(-1)
+        self._c(lib().spmspv_run_auto(self.h, 1.0, None, _vp(ykey_addr), _vp(yval_addr),
+                                      ctypes.byref(c), ctypes.byref(k)))
+        return c.value, k.value
+
     def autotune(self, reps=3):
         rep = np.zeros(3, np.int32)
         self._c(lib().spmspv_autotune(self.h, reps, _np(rep)))
@@ -378,6 +389,17 @@ class SpmspvHandle:
         st = np.zeros(4, np.float64)
         self._c(lib().spmspv_bfs(self.h, src, levels.ptr, policy, use_graphs, _np(st)))
         return st
+
+    def run_batched(self, vptr, vkey, vval, yoff, ykey, yval, ynnz):
+        """Batched SpMSpV over the vectors vptr/vkey/vval (device arrays);
+        results at yoff (device int64 offsets) in ykey/yval, counts in ynnz.
+        Returns the slot used per vector."""
+        nvec = vptr.count - 1
+        used = np.zeros(max(nvec, 1), np.int32)
+        self._c(lib().spmspv_run_batched(self.h, nvec, vptr.ptr, vkey.ptr if vkey else None,
+                                         vval.ptr if vval else None, yoff.ptr, ykey.ptr, yval.ptr,
+                                         ynnz.ptr, _np(used)))
+        return used[:nvec]
 
     def timer_start(self):
         return self._c(lib().spmspv_timer_start(self.h))
