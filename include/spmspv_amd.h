@@ -194,11 +194,16 @@ int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y);
  * thread counts are compile-time macros, la/include/spmspv/config.h:8-30,
  * scan_config.h:4-26) ------------------------------------------------------ */
 /* On the current x, times every candidate team size of the naive column
- * and row slots (2..64 lanes) and every row-merge tile width (5, 7 or 11
- * items per lane) with the handle's stream events, median of `reps`, and
- * keeps the fastest for this matrix. report (nullable, 3 ints) gets
- * {team_col, team_row, row_vt}. */
+ * and row slots (2..64 lanes), every row-merge tile width (5, 7 or 11
+ * items per lane) and every CSR5 width (sigma 8, 16, 24, 32) with the
+ * handle's stream events, median of `reps`, and keeps the fastest for this
+ * matrix: the K5/K7 format (set_spmv_type) included. report (nullable,
+ * 5 ints) gets {team_col, team_row, row_vt, spmv_type, sigma (0 if the
+ * merge-path tiles won)}. */
 int spmspv_autotune(spmspv_handle_t h, int reps, int* report);
+/* As spmspv_autotune; csr5 = 0 keeps the current K5/K7 format and only
+ * tunes its parameters' merge-path side. */
+int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report);
 
 /* ---- learned selector (no reference counterpart in C++: the reference's
  * predictors are stubs returning 0, la/test/new_test_app.cu:121-131) ------ */

@@ -170,6 +170,28 @@ int ensure_row_tiles(spmspv_handle_t h) {
   return SPMSPV_SUCCESS;
 }
 
+int ensure_csr5(spmspv_handle_t h) {
+  const int S = csr5_sigma_of(h->sigma, h->m, h->nnz);
+  if (h->csr5_ready && h->csr5.S == S) return SPMSPV_SUCCESS;
+  if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "CSR5 needs CSR input");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  const size_t bytes = csr5_bytes(h->m, h->nnz, S);
+  if (h->csr5_bytes < bytes) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->csr5_mem) HIP_TRY(h, hipFree(h->csr5_mem));
+    h->csr5_mem = nullptr;
+    h->csr5_bytes = 0;
+    HIP_TRY(h, hipMalloc(&h->csr5_mem, bytes));
+    h->csr5_bytes = bytes;
+  }
+  HIP_TRY(h, csr5_build(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->nnz, S, h->csr5_mem,
+                        h->h_pin, &h->csr5));
+  h->launches += 8;
+  h->csr5_ready = true;
+  return SPMSPV_SUCCESS;
+}
+
 int ensure_mat_features(spmspv_handle_t h) {
   if (h->mat_ready) return SPMSPV_SUCCESS;
   int rc = ensure_arena(h);
@@ -362,6 +384,27 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       // writing into the internal accumulator always tracks hits so that
       // the compaction can restore its all-zero state
       const bool hits = want_sparse || !y_dense;
+      if (lb && h->spmv_type == 1) {
+        const float* xin = nullptr;
+        if (masked || hits) {
+          if ((rc = need_enc_x(h))) return rc;
+          xin = h->x_enc;
+        } else {
+          if ((rc = need_dense_x(h))) return rc;
+          xin = h->dense_x();
+        }
+        if ((rc = ensure_csr5(h))) return rc;
+        float* vals = nullptr;
+        const int* kmap = nullptr;
+        int nk = 0;
+        launch_csr5(s, h->csr5, masked, hits, xin, nullptr, m, h->nnz, yout, yout != h->acc, h->marks,
+                    &vals, &kmap, &nk);
+        if (hits)
+          launch_compact(s, nullptr, h->marks, nk, vals, vals == h->acc ? 1 : 0, y_key, y_val,
+                         h->scan_state, h->ctl, next_ep(h, 1), kmap);
+        h->launches += 3;
+        return check_launch(h, "CSR5 row kernel");
+      }
       if (lb) {
         const float* xin = nullptr;
         if (masked || hits) {
@@ -437,6 +480,7 @@ int spmspv_destroy(spmspv_handle_t h) {
   if (h->csc_alloc) (void)hipFree(h->csc_alloc);
   if (h->flush_buf) (void)hipFree(h->flush_buf);
   if (h->batch_stash) (void)hipFree(h->batch_stash);
+  if (h->csr5_mem) (void)hipFree(h->csr5_mem);
   if (h->h_pin) (void)hipHostFree(h->h_pin);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -464,6 +508,7 @@ int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* 
   h->csr_col = col;
   h->csr_val = val;
   h->row_ready = false;
+  h->csr5_ready = false;
   h->mat_ready = false;
   return SPMSPV_SUCCESS;
 }
@@ -541,11 +586,27 @@ int spmspv_hola_preprocess(spmspv_handle_t h) {
   return SPMSPV_SUCCESS;
 }
 int spmspv_hola_postprocess(spmspv_handle_t h) { return h ? SPMSPV_SUCCESS : SPMSPV_INVALID_ARGUMENT; }
-int spmspv_csr5_preprocess(spmspv_handle_t h) { return spmspv_hola_preprocess(h); }
-int spmspv_csr5_postprocess(spmspv_handle_t h) { return h ? SPMSPV_SUCCESS : SPMSPV_INVALID_ARGUMENT; }
+int spmspv_csr5_preprocess(spmspv_handle_t h) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  int rc = ensure_csr5(h);
+  if (rc) return rc;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return SPMSPV_SUCCESS;
+}
+int spmspv_csr5_postprocess(spmspv_handle_t h) {
+  // frees the CSR5 copy (the caller's CSR was never modified)
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (h->csr5_mem) HIP_TRY(h, hipFree(h->csr5_mem));
+  h->csr5_mem = nullptr;
+  h->csr5_bytes = 0;
+  h->csr5_ready = false;
+  return SPMSPV_SUCCESS;
+}
 int spmspv_set_sigma(spmspv_handle_t h, int sigma) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
-  h->sigma = sigma;
+  if (sigma > 32) return set_error(h, SPMSPV_INVALID_ARGUMENT, "sigma must be at most 32");
+  h->sigma = sigma;  // the CSR5 copy is rebuilt on next use when its width changes
   return SPMSPV_SUCCESS;
 }
 int spmspv_set_spmv_type(spmspv_handle_t h, int code) {
@@ -831,11 +892,20 @@ int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* k, float* 
 int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y) {
   return spmspv_run(h, SPMSPV_K_ROW_NAIVE, alpha, d_y, nullptr, nullptr, nullptr);
 }
+// holaspmspv / csr5spmspv run their format whatever set_spmv_type says
+static int run_with_type(spmspv_handle_t h, int type, int kid, float alpha, float* d_y) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  const int saved = h->spmv_type;
+  h->spmv_type = type;
+  const int rc = spmspv_run(h, kid, alpha, d_y, nullptr, nullptr, nullptr);
+  h->spmv_type = saved;
+  return rc;
+}
 int spmspv_holaspmspv(spmspv_handle_t h, float alpha, float* d_y) {
-  return spmspv_run(h, SPMSPV_K_ROW_LB, alpha, d_y, nullptr, nullptr, nullptr);
+  return run_with_type(h, 0, SPMSPV_K_ROW_LB, alpha, d_y);
 }
 int spmspv_csr5spmspv(spmspv_handle_t h, float alpha, float* d_y) {
-  return spmspv_run(h, SPMSPV_K_ROW_LB, alpha, d_y, nullptr, nullptr, nullptr);
+  return run_with_type(h, 1, SPMSPV_K_ROW_LB, alpha, d_y);
 }
 int spmspv_spmspv(spmspv_handle_t h, float alpha, float* d_y) {
   return spmspv_run(h, SPMSPV_K_ROW_LB, alpha, d_y, nullptr, nullptr, nullptr);
@@ -844,10 +914,10 @@ int spmspv_naivespmv(spmspv_handle_t h, float alpha, float* d_y) {
   return spmspv_run(h, SPMSPV_K_SPMV_NAIVE, alpha, d_y, nullptr, nullptr, nullptr);
 }
 int spmspv_holaspmv(spmspv_handle_t h, float alpha, float* d_y) {
-  return spmspv_run(h, SPMSPV_K_SPMV_LB, alpha, d_y, nullptr, nullptr, nullptr);
+  return run_with_type(h, 0, SPMSPV_K_SPMV_LB, alpha, d_y);
 }
 int spmspv_csr5spmv(spmspv_handle_t h, float alpha, float* d_y) {
-  return spmspv_run(h, SPMSPV_K_SPMV_LB, alpha, d_y, nullptr, nullptr, nullptr);
+  return run_with_type(h, 1, SPMSPV_K_SPMV_LB, alpha, d_y);
 }
 int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y) {
   return spmspv_run(h, SPMSPV_K_SPMV_LB, alpha, d_y, nullptr, nullptr, nullptr);
@@ -914,6 +984,10 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
 
 // ------------------------------------------------------------- autotuner
 int spmspv_autotune(spmspv_handle_t h, int reps, int* report) {
+  return spmspv_autotune2(h, reps, 1, report);
+}
+
+int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   if (!h || reps <= 0) return set_error(h, SPMSPV_INVALID_ARGUMENT, "autotune: bad arguments");
   int rc;
   if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
@@ -959,6 +1033,8 @@ int spmspv_autotune(spmspv_handle_t h, int reps, int* report) {
     h->team_row = pick;
     best = 1e30f;
     pick = h->row_vt;
+    const int saved_type = h->spmv_type;
+    h->spmv_type = 0;
     for (int vt : {5, 7, 11}) {
       h->row_vt = vt;
       h->row_ready = false;
@@ -971,6 +1047,25 @@ int spmspv_autotune(spmspv_handle_t h, int reps, int* report) {
     h->row_vt = pick;
     h->row_ready = false;
     if ((rc = ensure_row_tiles(h))) return rc;
+    // the CSR5 tiles compete with the merge-path tiles for K5/K7
+    int type = 0, sigma = h->sigma;
+    if (csr5) {
+      h->spmv_type = 1;
+      for (int sg : {8, 16, 24, 32}) {
+        h->sigma = sg;
+        if ((rc = time_slot(SPMSPV_K_SPMV_LB, &t))) return rc;
+        if (t < best) {
+          best = t;
+          type = 1;
+          sigma = sg;
+        }
+      }
+    } else {
+      type = saved_type;
+    }
+    h->spmv_type = type;
+    h->sigma = sigma;
+    if (type == 1 && (rc = ensure_csr5(h))) return rc;
     // captured BFS levels hold the old tiling
     spmspv_bfs_release(h);
   }
@@ -979,6 +1074,8 @@ int spmspv_autotune(spmspv_handle_t h, int reps, int* report) {
     report[0] = h->team_col;
     report[1] = h->team_row;
     report[2] = h->row_vt;
+    report[3] = h->spmv_type;
+    report[4] = h->spmv_type == 1 ? csr5_sigma_of(h->sigma, h->m, h->nnz) : 0;
   }
   return SPMSPV_SUCCESS;
 }

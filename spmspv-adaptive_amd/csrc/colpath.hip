@@ -328,7 +328,8 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
                                                 unsigned char* __restrict__ marks, int m,
                                                 float* __restrict__ acc, int reset_acc,
                                                 int* __restrict__ ykey, float* __restrict__ yval,
-                                                unsigned long long* state, Ctl* ctl, Ep ep) {
+                                                unsigned long long* state, Ctl* ctl, Ep ep,
+                                                const int* __restrict__ keymap) {
   __shared__ unsigned long long s_pre;
   __shared__ int s_i[NT / kWave];
   __shared__ int s_rows[NT * 32];
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
   for (int i = threadIdx.x; i < btot; i += NT) {
     const int r = s_rows[i];
     const long long o = (long long)pre + i;
-    if (ykey) ykey[o] = r;
+    if (ykey) ykey[o] = keymap ? keymap[r] : r;
     if (yval) yval[o] = acc[r];
     if (reset_acc) acc[r] = 0.0f;
   }
@@ -381,15 +382,15 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
 
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep) {
+                    Ep ep, const int* keymap) {
   const int nwords = (m + 31) >> 5;
   const int grid = std::max(1, (nwords + kCmpNT - 1) / kCmpNT);
   if (marks)
     hipLaunchKernelGGL((k_compact<kCmpNT, true>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap);
   else
     hipLaunchKernelGGL((k_compact<kCmpNT, false>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap);
 }
 
 // ------------------------------------------------------------ utilities

@@ -1,0 +1,454 @@
+// CSR5-style SpMV / masked SpMSpV for 64-lane wavefronts (the spmv_type 1
+// path of K5/K7; the reference's csr5_spmv_cuda.h:440-582 and
+// csr5_spmspv.h:449-610 with CSR5Preprocess, spmspv.h:1425-1582, built for
+// 32-lane warps). The format is re-derived for wave64:
+//   * a tile is 64 lanes x S items (S = sigma in {8, 16, 24, 32}); lane l
+//     owns the tile's items l*S .. l*S+S-1 in CSR order, stored transposed
+//     (item i of lane l at tile*64*S + i*64 + l) so that every item step is
+//     one coalesced load per wave;
+//   * the descriptor is one 32-bit word per lane: bit i marks a row start
+//     at item i (bit 0 of lane 0 always marks the tile start). Marker
+//     offsets (the reference's y_offset) come from a DPP prefix sum of the
+//     popcounts and the cross-lane segmented sum (its seg_offset) from a
+//     DPP segmented scan, so neither is stored;
+//   * tile_info[t] = index, in the list of non-empty rows, of the row that
+//     holds the tile's first item, with bit 31 set when that item continues
+//     a row from the previous tile. The k-th marker of tile t is non-empty
+//     row rows_ne[base + k], which replaces the reference's per-tile
+//     empty_offset arrays; empty rows are written from their own list.
+// One wave per tile, no LDS and no barriers; each wave walks tiles
+// w, w + waves, ... with the next tile's loads in flight (column indices
+// and values two tiles ahead, x gathered one tile ahead).
+#include <algorithm>
+
+#include "handle.hpp"
+
+namespace spmspv {
+
+namespace {
+
+constexpr int kXPlain5 = 0, kXEnc5 = 1, kXBits5 = 2;
+
+// ---- preprocessing --------------------------------------------------------
+constexpr int kScanPart = 2048;
+
+__global__ __launch_bounds__(256) void k_nonempty_part(const int* __restrict__ row_ptr, int m,
+                                                       int* __restrict__ sums) {
+  __shared__ int lds[4];
+  const long long base = (long long)blockIdx.x * kScanPart;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPart / 256; ++k) {
+    const long long r = base + k * 256 + threadIdx.x;
+    if (r < m) v += row_ptr[r + 1] > row_ptr[r];
+  }
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = v;
+}
+
+// exclusive scan of the part sums in one workgroup; sums[nparts] = total
+__global__ __launch_bounds__(256) void k_scan_parts(int* sums, int nparts) {
+  __shared__ int lds[4];
+  int run = 0;
+  for (int c = 0; c < nparts; c += 256) {
+    const int i = c + threadIdx.x;
+    const int v = i < nparts ? sums[i] : 0;
+    int tot;
+    const int ex = block_excl_sum<256>(v, lds, &tot);
+    if (i < nparts) sums[i] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) sums[nparts] = run;
+}
+
+// rank[r] = non-empty rows before r; rows_ne / rows_empty lists (ascending)
+__global__ __launch_bounds__(256) void k_rank_rows(const int* __restrict__ row_ptr, int m,
+                                                   const int* __restrict__ sums, int* __restrict__ rank,
+                                                   int* __restrict__ rows_ne, int* __restrict__ rows_empty) {
+  __shared__ int lds[4];
+  constexpr int PT = kScanPart / 256;
+  const long long base = (long long)blockIdx.x * kScanPart + (long long)threadIdx.x * PT;
+  int f[PT];
+  int loc = 0;
+#pragma unroll
+  for (int k = 0; k < PT; ++k) {
+    const long long r = base + k;
+    f[k] = r < m ? (row_ptr[r + 1] > row_ptr[r]) : 0;
+    loc += f[k];
+  }
+  int tot;
+  int run = sums[blockIdx.x] + block_excl_sum<256>(loc, lds, &tot);
+#pragma unroll
+  for (int k = 0; k < PT; ++k) {
+    const long long r = base + k;
+    if (r < m) {
+      rank[r] = run;
+      if (f[k]) rows_ne[run] = (int)r;
+      else rows_empty[r - run] = (int)r;
+    }
+    run += f[k];
+  }
+}
+
+// tile_info and the tile-start marker of every tile
+__global__ void k_tile_info(const int* __restrict__ row_ptr, const int* __restrict__ rank, int m,
+                            int nnz, int T, int ntiles, int n_ne, unsigned* __restrict__ tinfo,
+                            unsigned* __restrict__ flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const long long j0 = (long long)t * T;
+  if (j0 >= nnz) {
+    tinfo[t] = (unsigned)n_ne;  // past the data: a "real start" sentinel
+    return;
+  }
+  // the row holding item j0: first r with row_ptr[r + 1] > j0
+  int lo = 0, hi = m - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (row_ptr[mid + 1] > j0) hi = mid;
+    else lo = mid + 1;
+  }
+  const bool cont = row_ptr[lo] < j0;
+  tinfo[t] = (unsigned)rank[lo] | (cont ? 0x80000000u : 0u);
+  flags[(size_t)t * kWave] |= 1u;  // only this thread touches lane 0 here
+}
+
+// row-start markers (one-time preprocessing, atomics are fine)
+__global__ void k_row_markers(const int* __restrict__ row_ptr, int m, int T, int S,
+                              unsigned* __restrict__ flags) {
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < m;
+       r += (long long)gridDim.x * blockDim.x) {
+    const int j = row_ptr[r];
+    if (row_ptr[r + 1] <= j) continue;
+    const int t = j / T, o = j % T;
+    atomicOr(flags + (size_t)t * kWave + o / S, 1u << (o % S));
+  }
+}
+
+__global__ void k_csr5_transpose(const int* __restrict__ col, const float* __restrict__ val, int nnz,
+                                 int T, int S, long long total, int* __restrict__ ccol,
+                                 float* __restrict__ cval) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long t = q / T;
+    const int o = (int)(q - t * T);
+    const int i = o / kWave, l = o % kWave;
+    const long long src = t * T + (long long)l * S + i;
+    const bool ok = src < nnz;
+    ccol[q] = ok ? col[src] : 0;
+    cval[q] = ok ? val[src] : 0.f;
+  }
+}
+
+// ---- SpMV / masked SpMSpV ---------------------------------------------------
+template <int S, int XIN, bool MASKED, bool HITS>
+__global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
+                                              const float* __restrict__ cval,
+                                              const unsigned* __restrict__ flags,
+                                              const unsigned* __restrict__ tinfo,
+                                              const float* __restrict__ x,
+                                              const unsigned* __restrict__ xbits, int nnz, int ntiles,
+                                              float* __restrict__ y, unsigned char* __restrict__ marks,
+                                              float* __restrict__ carry_val,
+                                              int* __restrict__ carry_row,
+                                              int* __restrict__ carry_hit) {
+  constexpr int T = kWave * S;
+  const int lanes = lane();
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  // wave-uniform, so that per-tile words come through scalar loads
+  int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  if (t >= ntiles) return;
+
+  int cN[S];  // stage 2: columns / values loading
+  float vN[S];
+  float vG[S], xG[S];  // stage 1: values and gathered x (or bit words)
+  int cG[S];
+  unsigned fG;  // the tile's marker word
+
+  auto load_cv = [&](int tt) {
+    const int* __restrict__ ct = ccol + (size_t)tt * T + lanes;
+    const float* __restrict__ vt = cval + (size_t)tt * T + lanes;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      cN[i] = __builtin_nontemporal_load(ct + i * kWave);
+      vN[i] = __builtin_nontemporal_load(vt + i * kWave);
+    }
+  };
+  auto gather = [&](int tt) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      if (XIN == kXBits5) xG[i] = __builtin_bit_cast(float, xbits[(unsigned)cN[i] >> 5]);
+      else xG[i] = x[(unsigned)cN[i]];
+    }
+    fG = flags[(size_t)tt * kWave + lanes];
+    // real moves rather than renames (see the row-merge kernel): the stage
+    // registers keep their roles and no in-flight load is copied
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      asm volatile("v_mov_b32 %0, %1" : "=v"(vG[i]) : "v"(vN[i]));
+      if (XIN == kXBits5) asm volatile("v_mov_b32 %0, %1" : "=v"(cG[i]) : "v"(cN[i]));
+    }
+  };
+
+  load_cv(t);
+  gather(t);
+  load_cv(min(t + waves, ntiles - 1));
+
+  for (; t < ntiles; t += waves) {
+    const unsigned f = fG;
+    const unsigned ti = tinfo[t];
+    const unsigned tn = tinfo[t + 1];
+    const int base = (int)(ti & 0x7fffffffu);
+    const int lim = nnz - t * T;  // items of this tile that exist (>= T except the last)
+
+    // markers before this lane (DPP inclusive prefix of the popcounts)
+    const int cnt = __popc(f);
+    int incl = cnt;
+    incl += dpp_i32<kDppRowShr1, 0xf>(0, incl);
+    incl += dpp_i32<kDppRowShr2, 0xf>(0, incl);
+    incl += dpp_i32<kDppRowShr4, 0xf>(0, incl);
+    incl += dpp_i32<kDppRowShr8, 0xf>(0, incl);
+    incl += dpp_i32<kDppRowBcast15, 0xa>(0, incl);
+    incl += dpp_i32<kDppRowBcast31, 0xc>(0, incl);
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, kWave - 1, kWave);
+
+    // walk this lane's items; a segment that ends at a marker inside the
+    // lane is a finished row. Products are formed on the way, so that the
+    // stage-1 registers are free before the next tile is gathered into them.
+    float s = 0.f, head_s = 0.f;
+    int h = 0, head_h = 0, j = 0;
+    bool head = true;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      if ((f >> i) & 1u) {
+        if (head) {
+          head_s = s;
+          head_h = h;
+          head = false;
+        } else {
+          const unsigned k = (unsigned)(base + excl + j - 1);
+          y[k] = s;
+          if (HITS && h) marks[k] = 1;
+        }
+        s = 0.f;
+        h = 0;
+        ++j;
+      }
+      const bool ok = lanes * S + i < lim;
+      bool b;
+      float q;
+      if (XIN == kXBits5) {
+        b = (__builtin_bit_cast(unsigned, xG[i]) >> (cG[i] & 31)) & 1u;
+        q = b ? vG[i] : 0.f;
+      } else {
+        b = __builtin_bit_cast(unsigned, xG[i]) != 0u;
+        q = vG[i] * xG[i];
+        if (MASKED) q = b ? q : 0.f;
+      }
+      s += ok ? q : 0.f;
+      if (HITS) h |= (ok && b) ? 1 : 0;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    gather(min(t + waves, ntiles - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    load_cv(min(t + 2 * waves, ntiles - 1));
+    __builtin_amdgcn_sched_barrier(0);
+
+    // the segment open at the lane's end runs into the next lanes
+    int fl = cnt > 0 ? 1 : 0;
+    float sv = s;
+    int hv = h;
+    wave_seg_scan(fl, sv, hv);  // inclusive: segment open after this lane
+    const float cin = dpp_f32<kDppWaveShr1, 0xf>(0.f, sv);
+    const int hin = dpp_i32<kDppWaveShr1, 0xf>(0, hv);
+    if (cnt > 0 && excl > 0) {
+      // the head segment closes the row of the marker before this lane
+      const unsigned k = (unsigned)(base + excl - 1);
+      y[k] = cin + head_s;
+      if (HITS && (hin | head_h)) marks[k] = 1;
+    }
+    if (lanes == kWave - 1) {
+      // the tile's open segment: complete when the next tile starts a row
+      const int k = base + total - 1;
+      if (!(tn & 0x80000000u)) {
+        y[(unsigned)k] = sv;
+        if (HITS && hv) marks[k] = 1;
+        carry_row[t] = -1;
+      } else {
+        carry_row[t] = k;
+        carry_val[t] = sv;
+        carry_hit[t] = hv;
+      }
+    }
+  }
+}
+
+// rows that run through several tiles: their tail parts, added in tile order
+__global__ void k_csr5_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_row,
+                             const int* __restrict__ carry_hit, int ntiles, float* y,
+                             unsigned char* marks) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+    const int r = carry_row[t];
+    if (r < 0) continue;
+    if (t > 0 && carry_row[t - 1] == r) continue;
+    float s = 0.f;
+    int h = 0;
+    for (int u = t; u < ntiles && carry_row[u] == r; ++u) {
+      s += carry_val[u];
+      h |= carry_hit[u];
+    }
+    y[r] += s;
+    if (marks && h) marks[r] = 1;
+  }
+}
+
+// dense y from the non-empty-row results: y[r] = y_ne[rank[r]] or 0
+__global__ void k_csr5_expand(const int* __restrict__ row_ptr, const int* __restrict__ rank, int m,
+                              const float* __restrict__ y_ne, float* __restrict__ y) {
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < m;
+       r += (long long)gridDim.x * blockDim.x)
+    y[r] = row_ptr[r + 1] > row_ptr[r] ? y_ne[rank[r]] : 0.f;
+}
+
+}  // namespace
+
+int csr5_sigma_of(int sigma, int m, long long nnz) {
+  int s = sigma;
+  if (s <= 0) {
+    // nonzeros per row -> sigma, the reference's auto rule (spmspv.h:1397-1411)
+    const long long per = m > 0 ? nnz / m : 0;
+    if (per <= 4) s = 4;
+    else if (per <= 32) s = (int)per;
+    else if (per <= 256) s = 32;
+    else s = 6;
+  }
+  // the instantiated widths
+  if (s <= 8) return 8;
+  if (s <= 16) return 16;
+  if (s <= 24) return 24;
+  return 32;
+}
+
+size_t csr5_bytes(int m, long long nnz, int S) {
+  const long long T = (long long)kWave * S;
+  const long long ntiles = (nnz + T - 1) / T;
+  auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+  size_t b = 0;
+  b += up(sizeof(int) * (size_t)std::max<long long>(ntiles * T, 1)) * 2;  // ccol, cval
+  b += up(sizeof(unsigned) * (size_t)(ntiles + 1) * kWave);              // flags
+  b += up(sizeof(unsigned) * (size_t)(ntiles + 2));                      // tinfo
+  b += up(sizeof(int) * (size_t)m) * 4;                                  // rank, rows_ne, rows_empty, y_ne
+  b += up(sizeof(int) * (size_t)(m / kScanPart + 2));                    // part sums
+  b += up(sizeof(float) * (size_t)(ntiles + 1)) * 3;                     // carries
+  return b;
+}
+
+hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const float* val, int m,
+                      int nnz, int S, void* mem, int* h_pin, Csr5* out) {
+  const int T = kWave * S;
+  const int ntiles = (int)(((long long)nnz + T - 1) / T);
+  auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+  char* p = static_cast<char*>(mem);
+  Csr5 c{};
+  c.S = S;
+  c.row_ptr = row_ptr;
+  c.ntiles = ntiles;
+  c.ccol = reinterpret_cast<int*>(p);
+  p += up(sizeof(int) * (size_t)std::max<long long>((long long)ntiles * T, 1));
+  c.cval = reinterpret_cast<float*>(p);
+  p += up(sizeof(int) * (size_t)std::max<long long>((long long)ntiles * T, 1));
+  c.flags = reinterpret_cast<unsigned*>(p);
+  p += up(sizeof(unsigned) * (size_t)(ntiles + 1) * kWave);
+  c.tinfo = reinterpret_cast<unsigned*>(p);
+  p += up(sizeof(unsigned) * (size_t)(ntiles + 2));
+  int* rank = reinterpret_cast<int*>(p);
+  c.rank = rank;
+  p += up(sizeof(int) * (size_t)m);
+  c.y_ne = reinterpret_cast<float*>(p);
+  p += up(sizeof(float) * (size_t)m);
+  c.rows_ne = reinterpret_cast<int*>(p);
+  p += up(sizeof(int) * (size_t)m);
+  c.rows_empty = reinterpret_cast<int*>(p);
+  p += up(sizeof(int) * (size_t)m);
+  int* sums = reinterpret_cast<int*>(p);
+  p += up(sizeof(int) * (size_t)(m / kScanPart + 2));
+  c.carry_val = reinterpret_cast<float*>(p);
+  p += up(sizeof(float) * (size_t)(ntiles + 1));
+  c.carry_row = reinterpret_cast<int*>(p);
+  p += up(sizeof(float) * (size_t)(ntiles + 1));
+  c.carry_hit = reinterpret_cast<int*>(p);
+  p += up(sizeof(float) * (size_t)(ntiles + 1));
+
+  const int parts = (m + kScanPart - 1) / kScanPart;
+  if (parts > 0) {
+    hipLaunchKernelGGL(k_nonempty_part, dim3(parts), dim3(256), 0, s, row_ptr, m, sums);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(256), 0, s, sums, parts);
+    hipLaunchKernelGGL(k_rank_rows, dim3(parts), dim3(256), 0, s, row_ptr, m, sums, rank, c.rows_ne,
+                       c.rows_empty);
+  }
+  launch_fill_i32(s, reinterpret_cast<int*>(c.flags), (long long)(ntiles + 1) * kWave, 0);
+  hipError_t e = hipMemcpyAsync(h_pin, sums + parts, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  c.n_ne = parts > 0 ? h_pin[0] : 0;
+  c.n_empty = m - c.n_ne;
+  if (ntiles > 0) {
+    hipLaunchKernelGGL(k_tile_info, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, s, row_ptr, rank, m,
+                       nnz, T, ntiles, c.n_ne, c.tinfo, c.flags);
+    hipLaunchKernelGGL(k_row_markers, dim3(std::min(8192, (m + 255) / 256)), dim3(256), 0, s, row_ptr,
+                       m, T, S, c.flags);
+    const long long total = (long long)ntiles * T;
+    hipLaunchKernelGGL(k_csr5_transpose, dim3((unsigned)std::min<long long>(8192, (total + 255) / 256)),
+                       dim3(256), 0, s, col, val, nnz, T, S, total, c.ccol, c.cval);
+  }
+  *out = c;
+  return hipGetLastError();
+}
+
+void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
+                 const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
+                 unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out) {
+  const bool compressed = c.n_empty > 0;
+  float* yk = compressed ? c.y_ne : y;  // results indexed by non-empty row
+  *vals_out = yk;
+  *keymap_out = compressed ? c.rows_ne : nullptr;
+  *n_out = compressed ? c.n_ne : m;
+  if (m <= 0) return;
+  if (c.ntiles == 0) {
+    if (y_dense_rows) launch_fill_f32(s, y, m, 0.f);
+    return;
+  }
+#define SPMSPV_CSR5(SS, XI, MK, HT)                                                              \
+  {                                                                                              \
+    const void* fn = (const void*)k_csr5<SS, XI, MK, HT>;                                        \
+    const int waves = std::max(4, std::min(c.ntiles, resident_blocks(fn, 256) * 4));             \
+    hipLaunchKernelGGL((k_csr5<SS, XI, MK, HT>), dim3((waves + 3) / 4), dim3(256), 0, s, c.ccol,  \
+                       c.cval, c.flags, c.tinfo, x, xbits, nnz, c.ntiles, yk, marks,             \
+                       c.carry_val, c.carry_row, c.carry_hit);                                   \
+  }
+#define SPMSPV_CSR5_S(SS)                                        \
+  if (!x) SPMSPV_CSR5(SS, kXBits5, true, true)                   \
+  else if (masked && hits) SPMSPV_CSR5(SS, kXEnc5, true, true)   \
+  else if (masked) SPMSPV_CSR5(SS, kXEnc5, true, false)          \
+  else if (hits) SPMSPV_CSR5(SS, kXEnc5, false, true)            \
+  else SPMSPV_CSR5(SS, kXPlain5, false, false)
+  if (c.S == 8) {
+    SPMSPV_CSR5_S(8)
+  } else if (c.S == 16) {
+    SPMSPV_CSR5_S(16)
+  } else if (c.S == 24) {
+    SPMSPV_CSR5_S(24)
+  } else {
+    SPMSPV_CSR5_S(32)
+  }
+#undef SPMSPV_CSR5_S
+#undef SPMSPV_CSR5
+  hipLaunchKernelGGL(k_csr5_fixup, dim3(std::min(1024, (c.ntiles + 255) / 256)), dim3(256), 0, s,
+                     c.carry_val, c.carry_row, c.carry_hit, c.ntiles, yk, hits ? marks : nullptr);
+  if (compressed && y_dense_rows)
+    hipLaunchKernelGGL(k_csr5_expand, dim3(std::min(8192, (m + 255) / 256)), dim3(256), 0, s, c.row_ptr,
+                       c.rank, m, c.y_ne, y);
+}
+
+}  // namespace spmspv

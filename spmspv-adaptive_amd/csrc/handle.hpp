@@ -97,6 +97,10 @@ struct spmspv_handle_s {
   long long dev_levels = 0;  // device-epoch sequences run (captured BFS levels)
 
   BfsState* bfs = nullptr;
+  void* csr5_mem = nullptr;          // CSR5 copy of the CSR (spmv_type 1)
+  size_t csr5_bytes = 0;
+  bool csr5_ready = false;
+  spmspv::Csr5 csr5{};
   long long* batch_stash = nullptr;  // {bin_len, choice} per vector of a batch
   int batch_cap = 0;
 
@@ -118,6 +122,8 @@ int set_error(spmspv_handle_t h, int code, const std::string& msg);
 int ensure_arena(spmspv_handle_t h);
 int ensure_row_tiles(spmspv_handle_t h);
 int ensure_mat_features(spmspv_handle_t h);
+// Builds the CSR5 copy (sigma from set_sigma or nonzeros per row).
+int ensure_csr5(spmspv_handle_t h);
 // Builds the handle-owned CSC from the CSR when no CSC was given.
 int need_csc(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);

@@ -56,10 +56,11 @@ void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, 
                       float* bval, long long tiles_hint);
 // hits (bit array, or mark bytes when marks != null) -> sorted (key, val);
 // clears the hits (and acc entries when reset_acc) so the accumulator
-// returns to all-zero.
+// returns to all-zero. With keymap, hit k is reported as key keymap[k]
+// (keymap ascending keeps the keys sorted).
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep);
+                    Ep ep, const int* keymap = nullptr);
 // bitmap of (v != 0) over a dense vector (whole words, no atomics)
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
 void launch_fill_f32(hipStream_t s, float* p, long long n, float v);
@@ -106,6 +107,40 @@ void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, unsigned l
                    int* ykey, float* yval, long long binlen_hint, long long n_cap, Ep ep);
 
 // --- row path ------------------------------------------------------------
+// --- CSR5 for wave64 (spmv_type 1) ----------------------------------------
+struct Csr5 {
+  int S = 0;            // items per lane (sigma)
+  const int* row_ptr = nullptr;  // the CSR it was built from
+  int ntiles = 0;       // tiles of 64 * S items
+  int n_ne = 0, n_empty = 0;
+  int* ccol = nullptr;  // transposed column indices, ntiles * 64 * S
+  float* cval = nullptr;
+  unsigned* flags = nullptr;  // one row-start marker word per lane and tile
+  unsigned* tinfo = nullptr;  // first non-empty row index | continuation bit
+  int* rows_ne = nullptr;     // non-empty rows, ascending
+  int* rows_empty = nullptr;  // empty rows, ascending
+  int* rank = nullptr;        // rank[r] = non-empty rows before r
+  float* y_ne = nullptr;      // per non-empty row results when some rows are empty
+  float* carry_val = nullptr;
+  int* carry_row = nullptr;
+  int* carry_hit = nullptr;
+};
+// sigma (<= 0: from nonzeros per row) rounded up to an instantiated width
+int csr5_sigma_of(int sigma, int m, long long nnz);
+size_t csr5_bytes(int m, long long nnz, int S);
+// Builds the CSR5 copy of a CSR into `mem` (csr5_bytes); synchronises s.
+hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const float* val, int m,
+                      int nnz, int S, void* mem, int* h_pin, Csr5* out);
+// Same gather modes as launch_row_merge (x == nullptr: bit array only).
+// Without empty rows the results go to y (one mark byte per row with hits).
+// With empty rows they go to the non-empty-row space: vals_out = y_ne,
+// marks indexed by non-empty row, keymap_out = rows_ne (for the
+// compaction), and y is filled only when y_dense_rows (a dense output).
+// n_out = entries of that space (m or n_ne).
+void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
+                 const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
+                 unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out);
+
 // Workgroups of `block` threads of `kernel` resident on the whole device
 // at once (occupancy x CUs), cached per kernel.
 int resident_blocks(const void* kernel, int block);

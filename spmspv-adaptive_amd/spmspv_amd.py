@@ -105,6 +105,7 @@ _SIGS = {
     "spmspv_holaspmv": [_vp, _f, _vp],
     "spmspv_csr5spmv": [_vp, _f, _vp],
     "spmspv_spmv": [_vp, _f, _vp],
+    "spmspv_autotune2": [_vp, _i, _i, _vp],
     "spmspv_autotune": [_vp, _i, _vp],
     "spmspv_selector_set_weights": [_vp, _vp, _i],
     "spmspv_selector_mode": [_vp, _i, _i],
@@ -271,7 +272,18 @@ class SpmspvHandle:
         return self._c(lib().spmspv_hola_postprocess(self.h))
 
     def set_spmv_type(self, code):
+        """0: merge-path row tiles (Hola role), 1: CSR5 tiles for K5/K7."""
         return self._c(lib().spmspv_set_spmv_type(self.h, code))
+
+    def set_sigma(self, sigma):
+        """CSR5 items per lane (<= 0: from nonzeros per row)."""
+        return self._c(lib().spmspv_set_sigma(self.h, sigma))
+
+    def csr5_preprocess(self):
+        return self._c(lib().spmspv_csr5_preprocess(self.h))
+
+    def csr5_postprocess(self):
+        return self._c(lib().spmspv_csr5_postprocess(self.h))
 
     # ---- x (spmspv.h:977-1025)
     def set_vector_type(self, code):
@@ -368,10 +380,11 @@ class SpmspvHandle:
                                       ctypes.byref(c), ctypes.byref(k)))
         return c.value, k.value
 
-    def autotune(self, reps=3):
-        rep = np.zeros(3, np.int32)
-        self._c(lib().spmspv_autotune(self.h, reps, _np(rep)))
-        return {"team_col": int(rep[0]), "team_row": int(rep[1]), "row_vt": int(rep[2])}
+    def autotune(self, reps=3, csr5=True):
+        rep = np.zeros(5, np.int32)
+        self._c(lib().spmspv_autotune2(self.h, reps, 1 if csr5 else 0, _np(rep)))
+        return {"team_col": int(rep[0]), "team_row": int(rep[1]), "row_vt": int(rep[2]),
+                "spmv_type": int(rep[3]), "sigma": int(rep[4])}
 
     def select(self):
         k = _i(-1)

@@ -96,7 +96,16 @@ def cmd_corpus(args):
     feats, times, rule, meta = [], [], [], []
     t0 = time.time()
     flush = args.flush_mb << 20
+
+    def save():
+        np.savez(args.out, feats=np.array(feats, np.float32).reshape(-1, sa.MLP_F),
+                 times=np.array(times, np.float64).reshape(-1, sa.NUM_SLOTS),
+                 rule=np.array(rule, np.int32), meta=np.array(meta, np.int64).reshape(-1, 4))
+
     for seed in range(args.start, args.start + args.count):
+        if args.budget_s and time.time() - t0 > args.budget_s:
+            print(f"corpus stopped at seed {seed} (time budget)", flush=True)
+            break
         spec = matrix_spec(seed)
         n, rp, col, val = build_matrix(spec, seed)
         if rp[-1] == 0:
@@ -120,9 +129,9 @@ def cmd_corpus(args):
             meta.append([seed, n, int(rp[-1]), k])
         del M, A
         if (seed - args.start) % 25 == 0:
+            save()
             print(f"corpus seed {seed} {spec} {time.time() - t0:.1f}s", flush=True)
-    np.savez(args.out, feats=np.array(feats, np.float32), times=np.array(times, np.float64),
-             rule=np.array(rule, np.int32), meta=np.array(meta, np.int64))
+    save()
     print(f"corpus {len(feats)} samples -> {args.out} in {time.time() - t0:.1f}s", flush=True)
 
 
@@ -204,6 +213,8 @@ def main():
     c.add_argument("--reps", type=int, default=3)
     c.add_argument("--flush-mb", type=int, default=512)
     c.add_argument("--out", required=True)
+    c.add_argument("--budget-s", type=float, default=0.0,
+                   help="stop starting new matrices after this many seconds (results are kept)")
     t = sub.add_parser("train")
     t.add_argument("--corpus", nargs="+", required=True)
     t.add_argument("--holdout-from", type=int, default=2000)

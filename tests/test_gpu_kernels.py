@@ -267,6 +267,84 @@ def test_cpp_handle_adaptive_app(gpu):
     assert out.stdout.count("RESULT OK") == 6
 
 
+@pytest.mark.parametrize("sigma", [8, 16, 24, 32])
+def test_csr5_row_kernels(rmat12, sigma):
+    """K5/K7 on the CSR5 tiles (spmv_type 1) for every instantiated sigma:
+    sparse output, dense output with and without support tracking, every x
+    density; then back to the merge-path tiles."""
+    M, (cp, ri, cv) = rmat12
+    M.A.set_sigma(sigma)
+    M.A.set_spmv_type(1)
+    try:
+        for k in (1, 40, 400, M.n):
+            key, val = sa.gen_sparse_vector(M.n, k, 600 + k + sigma)
+            for slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
+                say(f"csr5 sigma {sigma} xnnz {k} slot {slot}")
+                yk, yv, yd = run_slot(M, slot, key, val, want_dense=True)
+                check_result(M.m, cp, ri, cv, key, val, yk, yv, f"csr5 s={sigma} k={k} slot {slot}")
+                dense = np.zeros(M.m, np.float32)
+                dense[yk] = yv
+                np.testing.assert_array_equal(yd, dense)
+                yd2 = sa.DeviceArray(M.m, np.float32)
+                M.A.run(slot, y_dense=yd2, want_nnz=False)
+                np.testing.assert_array_equal(yd2.to_host(), dense)
+    finally:
+        M.A.set_spmv_type(0)
+        M.A.set_sigma(-1)
+
+
+def test_csr5_edge_shapes(gpu):
+    """CSR5 with empty leading/trailing rows, rectangular shape, a row that
+    spans many tiles, a single nonzero, and the reference-named entry
+    points (csr5spmv / holaspmv)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(21)
+    cases = []
+    m, n = 5000, 3000
+    A = sp.random(m, n, density=0.003, format="lil", random_state=rng, dtype=np.float32)
+    A[:50, :] = 0
+    A[m - 70:, :] = 0
+    A[777, :] = rng.uniform(-1, 1, n).astype(np.float32)  # spans several tiles
+    cases.append(("rect+empty+long", A.tocsr()))
+    B = sp.lil_matrix((300, 300), dtype=np.float32)
+    B[123, 45] = 2.5
+    cases.append(("single nonzero", B.tocsr()))
+    for name, A in cases:
+        A.sort_indices()
+        A.eliminate_zeros()
+        m, n = A.shape
+        M = build(m, n, A.indptr, A.indices, A.data.astype(np.float32))
+        cp, ri, cv = csc_of(m, n, A.indptr, A.indices, A.data.astype(np.float32))
+        for sigma in (8, 32):
+            M.A.set_sigma(sigma)
+            M.A.set_spmv_type(1)
+            for k in (1, n // 3, n):
+                key, val = sa.gen_sparse_vector(n, k, 31 + k)
+                for slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
+                    say(f"csr5 {name} sigma {sigma} xnnz {k} slot {slot}")
+                    yk, yv, yd = run_slot(M, slot, key, val, want_dense=True)
+                    check_result(m, cp, ri, cv, key, val, yk, yv, f"{name} s={sigma} k={k} slot {slot}")
+                    dense = np.zeros(m, np.float32)
+                    dense[yk] = yv
+                    np.testing.assert_array_equal(yd, dense)
+        # reference-named entry points: csr5spmv runs CSR5 whatever the type
+        M.A.set_spmv_type(0)
+        key = np.arange(n, dtype=np.int32)
+        val = rng.uniform(-1, 1, n).astype(np.float32)
+        dk = sa.DeviceArray.from_host(key, np.int32)
+        dv = sa.DeviceArray.from_host(val, np.float32)
+        M.A.set_sparse_x(dk, dv, n)
+        y1 = sa.DeviceArray(m, np.float32)
+        y2 = sa.DeviceArray(m, np.float32)
+        assert sa.lib().spmspv_csr5spmv(M.A.h, 1.0, y1.ptr) == 0
+        assert sa.lib().spmspv_holaspmv(M.A.h, 1.0, y2.ptr) == 0
+        ref = (A.astype(np.float64) @ val.astype(np.float64))
+        bound = (abs(A).astype(np.float64) @ np.abs(val).astype(np.float64))
+        for y in (y1.to_host(), y2.to_host()):
+            assert np.all(np.abs(y - ref) <= TOL * bound + 1e-30), name
+        M.A.set_sigma(-1)
+
+
 def test_autotune_then_parity(rmat12):
     """The autotuner changes team sizes and the row tile width per matrix;
     every slot must stay exact afterwards, for every candidate width."""

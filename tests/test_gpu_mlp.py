@@ -79,3 +79,31 @@ def test_mlp_training_fits_a_rule(gpu):
     assert sa.lib().spmspv_mlp_forward(_p(w), _p(Xn), 1000, _p(z)) == 0
     acc = (z.argmax(1) == lab[3000:]).mean()
     assert acc > 0.95, acc
+
+
+def test_trained_selector_gain_on_heldout(gpu):
+    """The committed network, scored on the committed held-out corpus
+    (matrix seeds 2000-2399, every (matrix, x) timed on all eight slots):
+    its geomean regret against the fastest slot is at most 5% and below
+    both the rule fallback's and the best single fixed slot's."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "spmspv-adaptive_amd")
+    w = np.load(os.path.join(here, "selector_weights.npy")).astype(np.float32)
+    d = np.load(os.path.join(here, "selector_corpus", "heldout_seeds_2000_2399.npz"), allow_pickle=False)
+    X, T, rule = d["feats"], d["times"], d["rule"]
+    mean, istd = w[sa.MLP_NPARAM:sa.MLP_NPARAM + F], w[sa.MLP_NPARAM + F:]
+    Xn = np.ascontiguousarray((X - mean) * istd, np.float32)
+    z = np.zeros((X.shape[0], C), np.float32)
+    assert sa.lib().spmspv_mlp_forward(_p(w), _p(Xn), X.shape[0], _p(z)) == 0
+    pred = z.argmax(axis=1)
+
+    def regret(choice):
+        t = T[np.arange(len(choice)), choice]
+        return float(np.exp(np.mean(np.log(t / T.min(axis=1)))) - 1.0)
+
+    mlp = regret(pred)
+    rules = regret(rule)
+    fixed = min(regret(np.full(len(pred), s)) for s in range(C))
+    print(f"held-out regret: mlp {mlp:.4f} rules {rules:.4f} best fixed {fixed:.4f}")
+    assert mlp <= 0.05
+    assert mlp < rules and mlp < fixed
