@@ -58,6 +58,9 @@ def profile_slot(sa, A, n, args, yk, yv):
     dv = sa.DeviceArray.from_host(xv, np.float32)
     A.set_sparse_x(dk, dv, k)
     bl = A.computeBinlen()
+    if args.spmv_type >= 0:  # the K5/K7 format the sweep's autotuner picked
+        A.set_spmv_type(args.spmv_type)
+        A.set_sigma(args.sigma)
     ms = []
     ynnz = 0
     for _ in range(args.reps):
@@ -236,6 +239,10 @@ def main():
                     help="sources whose levels are compared with the CPU BFS for every policy")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fixed", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--spmv-type", type=int, default=-1,
+                    help="profiling mode: K5/K7 format (0 merge-path, 1 CSR5), as reported under autotuned")
+    ap.add_argument("--sigma", type=int, default=-1, help="profiling mode: CSR5 sigma")
     ap.add_argument("--flush-mb", type=int, default=512)
     ap.add_argument("--only-slot", type=int, default=-1,
                     help="profiling mode: time one slot at --density only (for rocprofv3 runs); "
@@ -292,6 +299,11 @@ def main():
         A.set_sparse_x(dk, dv, k)
         bl = A.computeBinlen()
         meta.append(bl)
+    # per-matrix tuning (team sizes, row tile width, K5/K7 format) on the
+    # densest x, as a user would run it once per matrix
+    k, dk, dv = xs[-1][:3]
+    A.set_sparse_x(dk, dv, k)
+    tuned = A.autotune(reps=3) if not args.no_autotune else None
     setup_s = time.time() - t_setup
 
     def timed_call(i, slot=None):
@@ -359,10 +371,31 @@ def main():
     dom_i = int(np.argmax([np.median(v) for v in per_density]))
     dom_slot = chosen[dom_i]
     reps = 10
-    dts = [timed_call(dom_i, dom_slot)[0] for _ in range(reps)]
-    dom_ms = float(np.median(dts))
     dom_bytes = alg_bytes(xs[dom_i][0], meta[dom_i], ynnzs[dom_i])
+    kernel_ms = None
+    if dom_slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
+        # the tile kernel alone, events recorded on its own stream around it
+        A.time_main_kernel(True)
+        kms = []
+        for _ in range(reps):
+            timed_call(dom_i, dom_slot)
+            kms.append(A.main_kernel_ms())
+        A.time_main_kernel(False)
+        kernel_ms = float(np.median(kms))
+    dts = [timed_call(dom_i, dom_slot)[0] for _ in range(reps)]
+    call_ms = float(np.median(dts))
+    dom_ms = kernel_ms if kernel_ms is not None else call_ms
     dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
+    fmt = ""
+    if dom_slot in (sa.K_ROW_LB, sa.K_SPMV_LB) and tuned and tuned["spmv_type"] == 1:
+        fmt = "-csr5"
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", f"{sa.KERNEL_NAMES[dom_slot]}{fmt}_rmat{args.scale}_d{DENSITIES[dom_i]}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = tj["read_bytes"] + tj["write_bytes"]
+        traffic_src = os.path.relpath(tpath, ROOT)
 
     bfs = run_bfs(sa, args, A, rp, col, n) if args.bfs_sources > 0 else None
 
@@ -404,9 +437,12 @@ def main():
                    "timing": "cold per call (512 MB MALL flush before each call, outside the events)"},
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 5),
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": sa.KERNEL_NAMES[dom_slot], "density": DENSITIES[dom_i],
-                     "launch_ms": round(dom_ms, 5), "alg_bytes": dom_bytes},
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": sa.KERNEL_NAMES[dom_slot] + fmt, "density": DENSITIES[dom_i],
+                     "launch_ms": round(dom_ms, 5), "call_ms": round(call_ms, 5),
+                     "timed": "tile kernel alone" if kernel_ms is not None else "whole call",
+                     "alg_bytes": dom_bytes},
         "cpu_baseline": cpu,
         "warm_value": round(warm_value, 3),
         "per_density": [{"density": DENSITIES[i], "xnnz": xs[i][0], "bin_len": int(meta[i]), "ynnz": int(ynnzs[i]),
@@ -414,6 +450,7 @@ def main():
                          "gbs": round(alg_bytes(xs[i][0], meta[i], ynnzs[i]) / (np.median(per_density[i]) * 1e-3) / 1e9, 2)}
                         for i in range(len(DENSITIES))],
         "setup_s": round(setup_s, 2),
+        "autotuned": tuned,
     }
     if fixed is not None:
         out["fixed_ms"] = {sa.KERNEL_NAMES[s]: [round(float(v), 5) for v in fixed[s]] for s in range(sa.NUM_SLOTS)}

@@ -277,6 +277,12 @@ int spmspv_synchronize(void);
 /* Stream timing: events recorded on the handle's stream. */
 int spmspv_timer_start(spmspv_handle_t h);
 int spmspv_timer_stop(spmspv_handle_t h, float* ms);
+/* No counterpart in the reference (measurement): with on != 0, the next
+ * K5/K7 calls record events around their tile kernel alone (not the
+ * fix-up, compaction or x conversion); spmspv_main_kernel_ms returns the
+ * last such kernel's duration. */
+int spmspv_time_main_kernel(spmspv_handle_t h, int on);
+int spmspv_main_kernel_ms(spmspv_handle_t h, float* ms);
 /* Writes `bytes` through a scratch buffer to evict the 256 MB MALL. */
 int spmspv_flush_cache(spmspv_handle_t h, size_t bytes);
 

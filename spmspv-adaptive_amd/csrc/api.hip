@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+
 #include "handle.hpp"
 #include "selector.hpp"
 
@@ -398,7 +400,9 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         const int* kmap = nullptr;
         int nk = 0;
         launch_csr5(s, h->csr5, masked, hits, xin, nullptr, m, h->nnz, yout, yout != h->acc, h->marks,
-                    &vals, &kmap, &nk);
+                    &vals, &kmap, &nk, h->time_main ? h->evk0 : nullptr,
+                    h->time_main ? h->evk1 : nullptr);
+        h->main_timed = h->time_main;
         if (hits)
           launch_compact(s, nullptr, h->marks, nk, vals, vals == h->acc ? 1 : 0, y_key, y_val,
                          h->scan_state, h->ctl, next_ep(h, 1), kmap);
@@ -417,7 +421,9 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         if ((rc = ensure_row_tiles(h))) return rc;
         launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, xin, nullptr, m, h->nnz,
                          h->row_split, h->n_row_tiles, h->row_vt, yout, h->bitmap, h->row_carry_val,
-                         h->row_carry_row, h->row_carry_hit);
+                         h->row_carry_row, h->row_carry_hit, h->time_main ? h->evk0 : nullptr,
+                         h->time_main ? h->evk1 : nullptr);
+        h->main_timed = h->time_main;
       } else {
         if ((rc = need_dense_x(h))) return rc;
         if ((masked || hits) && (rc = need_bits(h))) return rc;
@@ -435,10 +441,33 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
   }
 }
 
-int readback_ynnz(spmspv_handle_t h, int* y_nnz) {
-  HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->ynnz, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+int wait_signal(spmspv_handle_t h, int idx, long long* out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int spin = 0;; ++spin) {
+    const long long v = __atomic_load_n(h->sig + idx, __ATOMIC_ACQUIRE);
+    if (v != kSigPending) {
+      *out = v;
+      return SPMSPV_SUCCESS;
+    }
+    if ((spin & 255) == 255 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000))
+      break;  // a long kernel: block on the stream instead of spinning
+    __builtin_ia32_pause();
+  }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  *y_nnz = h->h_pin[0];
+  *out = __atomic_load_n(h->sig + idx, __ATOMIC_ACQUIRE);
+  if (*out == kSigPending) return set_error(h, SPMSPV_DEVICE_ERROR, "device result never arrived");
+  return SPMSPV_SUCCESS;
+}
+
+int readback_ynnz(spmspv_handle_t h, int* y_nnz) {
+  __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
+  launch_signal_int(h->stream, &h->ctl->ynnz, h->sig_dev + 3);
+  int rc = check_launch(h, "result count");
+  if (rc) return rc;
+  long long v = 0;
+  if ((rc = wait_signal(h, 3, &v))) return rc;
+  *y_nnz = (int)v;
   return SPMSPV_SUCCESS;
 }
 
@@ -460,12 +489,20 @@ int spmspv_create(spmspv_handle_t* out, int m, int n, int nnz) {
   h->team_col = std::min(64, std::max(4, pow2_team((double)nnz / n)));
   h->team_row = std::min(64, std::max(2, pow2_team((double)nnz / m)));
   hipError_t e = hipHostMalloc((void**)&h->h_pin, 64 * sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&h->sig, 16 * sizeof(long long), hipHostMallocCoherent | hipHostMallocMapped);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&h->sig_dev, h->sig, 0);
   if (e != hipSuccess) {
+    if (h->h_pin) (void)hipHostFree(h->h_pin);
+  if (h->sig) (void)hipHostFree(h->sig);
+    if (h->sig) (void)hipHostFree(h->sig);
     delete h;
     return set_error(nullptr, SPMSPV_DEVICE_ERROR, std::string("pinned buffer: ") + hipGetErrorString(e));
   }
   (void)hipEventCreate(&h->ev0);
   (void)hipEventCreate(&h->ev1);
+  (void)hipEventCreate(&h->evk0);
+  (void)hipEventCreate(&h->evk1);
   *out = h;
   return SPMSPV_SUCCESS;
 }
@@ -484,6 +521,8 @@ int spmspv_destroy(spmspv_handle_t h) {
   if (h->h_pin) (void)hipHostFree(h->h_pin);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->evk0) (void)hipEventDestroy(h->evk0);
+  if (h->evk1) (void)hipEventDestroy(h->evk1);
   delete h;
   return SPMSPV_SUCCESS;
 }
@@ -952,15 +991,15 @@ int spmspv_select(spmspv_handle_t h, int* kernel_id) {
   if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = ensure_mat_features(h)) ||
       (rc = need_sparse_x(h)))
     return rc;
-  SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed};
+  // the selecting workgroup writes {bin_len, choice} straight to host words
+  SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
+                   h->sig_dev};
+  __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
   if ((rc = run_prep(h, true, &sel))) return rc;
-  HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->bin_len, 24, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipMemcpyAsync(h->h_pin + 8, &h->ctl->kernel_choice, 4, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  long long bl;
-  std::memcpy(&bl, h->h_pin, sizeof(bl));
-  h->binlen_host = bl;
-  *kernel_id = h->h_pin[8];
+  long long choice = 0;
+  if ((rc = wait_signal(h, 1, &choice))) return rc;
+  h->binlen_host = __atomic_load_n(h->sig, __ATOMIC_ACQUIRE);
+  *kernel_id = (int)choice;
   return SPMSPV_SUCCESS;
 }
 
@@ -980,6 +1019,22 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   if (rc) return rc;
   if (kernel_used) *kernel_used = kid;
   return spmspv_run(h, kid, alpha, d_y_dense, d_y_key, d_y_val, y_nnz);
+}
+
+int spmspv_time_main_kernel(spmspv_handle_t h, int on) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->time_main = on != 0;
+  h->main_timed = false;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_main_kernel_ms(spmspv_handle_t h, float* ms) {
+  if (!h || !ms) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  if (!h->main_timed)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "no timed main kernel (row load-balanced slots only)");
+  HIP_TRY(h, hipEventSynchronize(h->evk1));
+  HIP_TRY(h, hipEventElapsedTime(ms, h->evk0, h->evk1));
+  return SPMSPV_SUCCESS;
 }
 
 // ------------------------------------------------------------- autotuner

@@ -38,9 +38,44 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   __shared__ unsigned long long s_pre;
   __shared__ long long s_l[NT / kWave];
   __shared__ int s_i[NT / kWave];
+  __shared__ float s_w[kNWeights];  // selector weights (selecting workgroup only)
+  __shared__ MatFeat s_mf;
   const int xnnz = xnnz_dev ? *xnnz_dev : xnnz_arg;
   const unsigned ntiles = (unsigned)((xnnz + NT * IPT - 1) / (NT * IPT));
   const unsigned t = blockIdx.x;
+  // The workgroup that will run the selector loads its weights and the
+  // matrix features now, so that their memory latency overlaps the scan
+  // instead of following it.
+  const bool selects = sel.mat != nullptr && t == (ntiles == 0 ? 0u : ntiles - 1);
+  constexpr int kWPT = (kNWeights + NT - 1) / NT;
+  float wr[kWPT];
+  MatFeat mfr;
+  if (selects) {
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        wr[i] = idx < kNWeights ? sel.weights[idx] : 0.f;
+      }
+    }
+    if (threadIdx.x == 0) mfr = *sel.mat;
+  }
+  auto run_selector = [&]() {
+    if (!selects) return;
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        if (idx < kNWeights) s_w[idx] = wr[i];
+      }
+    }
+    if (threadIdx.x == 0) s_mf = mfr;
+    __syncthreads();
+    SelectorArgs sl = sel;
+    if (sel.weights) sl.weights = s_w;
+    sl.mat = &s_mf;
+    select_in_block<NT>(ctl, sl);
+  };
   if (ntiles == 0) {
     // empty x: block 0 alone writes the results
     if (t != 0) return;
@@ -54,9 +89,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
       if (split) split[0] = 0;
     }
     __syncthreads();
-    if (sel.mat) select_in_block<NT>(ctl, sel);
+    run_selector();
     if (sel.stash && threadIdx.x == 0) {
       sel.stash[0] = 0;
+      __threadfence_system();  // bin_len lands before the choice the host polls
       sel.stash[1] = sel.mat ? ctl->kernel_choice : -1;
     }
     return;
@@ -137,10 +173,11 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
   }
   __syncthreads();
-  if (sel.mat) select_in_block<NT>(ctl, sel);
+  run_selector();
   // the selector's choice and bin_len were written by thread 0 as well
   if (sel.stash && threadIdx.x == 0) {
     sel.stash[0] = ctl->bin_len;
+    __threadfence_system();  // bin_len lands before the choice the host polls
     sel.stash[1] = sel.mat ? ctl->kernel_choice : -1;
   }
 }
@@ -370,12 +407,32 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < btot; i += NT) {
-    const int r = s_rows[i];
-    const long long o = (long long)pre + i;
-    if (ykey) ykey[o] = keymap ? keymap[r] : r;
-    if (yval) yval[o] = acc[r];
-    if (reset_acc) acc[r] = 0.0f;
+  // U rows per thread per round, their loads issued together
+  constexpr int U = 8;
+  for (int i0 = threadIdx.x; i0 < btot; i0 += NT * U) {
+    int r[U], key[U];
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NT;
+      r[u] = i < btot ? s_rows[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] >= 0) {
+        key[u] = keymap ? keymap[r[u]] : r[u];
+        v[u] = yval ? acc[r[u]] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] >= 0) {
+        const long long o = (long long)pre + i0 + u * NT;
+        if (ykey) ykey[o] = key[u];
+        if (yval) yval[o] = v[u];
+        if (reset_acc) acc[r[u]] = 0.0f;
+      }
+    }
   }
   if (t == ntiles - 1 && threadIdx.x == 0) ctl->ynnz = (int)(pre + btot);
 }
@@ -394,6 +451,15 @@ void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m
 }
 
 // ------------------------------------------------------------ utilities
+__global__ void k_signal_int(const int* src, long long* dst) {
+  const long long v = *src;
+  __threadfence_system();
+  *dst = v;
+}
+void launch_signal_int(hipStream_t s, const int* src, long long* dst) {
+  hipLaunchKernelGGL(k_signal_int, dim3(1), dim3(1), 0, s, src, dst);
+}
+
 __global__ void k_bitmap_from_dense(const float* __restrict__ v, int n, unsigned* __restrict__ bits) {
   // one wave per 64 elements; element e -> bit e of words 2w, 2w+1
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;

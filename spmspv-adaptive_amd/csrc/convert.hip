@@ -84,6 +84,12 @@ __global__ __launch_bounds__(NT) void k_mat_stats(const int* __restrict__ row_pt
   __shared__ double s_d[NT / kWave];
   __shared__ int s_i[NT / kWave];
   __shared__ unsigned s_flag;
+  // degree histogram: small degrees in LDS (flushed once per workgroup),
+  // the rare large ones straight to memory
+  constexpr int HB = 2048;
+  __shared__ int s_hist[HB];
+  for (int k = threadIdx.x; k < HB; k += NT) s_hist[k] = 0;
+  __syncthreads();
   const double avg = (double)nnz / (double)m;
   const int avg_trunc = nnz / m;
   int mx = 0, mn = INT_MAX;
@@ -99,8 +105,13 @@ __global__ __launch_bounds__(NT) void k_mat_stats(const int* __restrict__ row_pt
       const double p = d / (1.0 * nnz);
       eq += -p * log(p);
     }
-    atomicAdd(hist + min(d, n), 1);  // rows with duplicate columns stay in range
+    const int b = min(d, n);  // rows with duplicate columns stay in range
+    if (b < HB) atomicAdd(&s_hist[b], 1);
+    else atomicAdd(hist + b, 1);
   }
+  __syncthreads();
+  for (int k = threadIdx.x; k < HB && k <= n; k += NT)
+    if (s_hist[k]) atomicAdd(hist + k, s_hist[k]);
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
   sq = block_sum<NT, double>(sq, s_d);

@@ -408,7 +408,8 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
 
 void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
                  const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
-                 unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out) {
+                 unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out,
+                 hipEvent_t ev_begin, hipEvent_t ev_end) {
   const bool compressed = c.n_empty > 0;
   float* yk = compressed ? c.y_ne : y;  // results indexed by non-empty row
   *vals_out = yk;
@@ -433,6 +434,7 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
   else if (masked) SPMSPV_CSR5(SS, kXEnc5, true, false)          \
   else if (hits) SPMSPV_CSR5(SS, kXEnc5, false, true)            \
   else SPMSPV_CSR5(SS, kXPlain5, false, false)
+  if (ev_begin) (void)hipEventRecord(ev_begin, s);
   if (c.S == 8) {
     SPMSPV_CSR5_S(8)
   } else if (c.S == 16) {
@@ -444,6 +446,7 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
   }
 #undef SPMSPV_CSR5_S
 #undef SPMSPV_CSR5
+  if (ev_end) (void)hipEventRecord(ev_end, s);
   hipLaunchKernelGGL(k_csr5_fixup, dim3(std::min(1024, (c.ntiles + 255) / 256)), dim3(256), 0, s,
                      c.carry_val, c.carry_row, c.carry_hit, c.ntiles, yk, hits ? marks : nullptr);
   if (compressed && y_dense_rows)

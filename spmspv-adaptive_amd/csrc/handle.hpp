@@ -91,7 +91,16 @@ struct spmspv_handle_s {
 
   // host mirrors
   int* h_pin = nullptr;       // small pinned readback area (64 ints)
+  // Host-coherent words the device writes results into directly (the host
+  // marks a word kSigPending before the launch and polls it): no copy
+  // kernel and no stream wait between the selector and the chosen kernel.
+  long long* sig = nullptr;
+  long long* sig_dev = nullptr;  // the same words, device address
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // events around the slot's main kernel alone (spmspv_time_main_kernel)
+  hipEvent_t evk0 = nullptr, evk1 = nullptr;
+  bool time_main = false;
+  bool main_timed = false;
   long long launches = 0;
   unsigned epoch_host = 1;   // next host-side look-back epoch (bit 23 clear)
   long long dev_levels = 0;  // device-epoch sequences run (captured BFS levels)
@@ -117,6 +126,8 @@ struct spmspv_handle_s {
       return set_error(h, SPMSPV_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+constexpr long long kSigPending = -0x5A5A5A5A5A5ALL;
+
 namespace spmspv {
 int set_error(spmspv_handle_t h, int code, const std::string& msg);
 int ensure_arena(spmspv_handle_t h);
@@ -129,6 +140,9 @@ int need_csc(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
 int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);
 int readback_ynnz(spmspv_handle_t h, int* y_nnz);
+// Waits for the device to write signal word idx (polls, then blocks on the
+// stream for long kernels).
+int wait_signal(spmspv_handle_t h, int idx, long long* out);
 // Reserves k consecutive host epochs for the look-back kernels of one call.
 Ep next_ep(spmspv_handle_t h, unsigned k);
 // Reads bin_len of the current x back to the host (one synchronisation).

@@ -64,6 +64,8 @@ void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m
 // bitmap of (v != 0) over a dense vector (whole words, no atomics)
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
 void launch_fill_f32(hipStream_t s, float* p, long long n, float v);
+// *dst = *src once every earlier launch on s has finished (dst may be host-mapped)
+void launch_signal_int(hipStream_t s, const int* src, long long* dst);
 void launch_fill_i32(hipStream_t s, int* p, long long n, int v);
 // dense[key[i]] = val[i]; with `encode`, a zero value is stored as -0.0
 // (the support encoding: +0.0 outside the support, nonzero bits inside).
@@ -137,9 +139,11 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
 // marks indexed by non-empty row, keymap_out = rows_ne (for the
 // compaction), and y is filled only when y_dense_rows (a dense output).
 // n_out = entries of that space (m or n_ne).
+// ev_begin / ev_end (nullable) are recorded around the tile kernel alone.
 void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
                  const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
-                 unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out);
+                 unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out,
+                 hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // Workgroups of `block` threads of `kernel` resident on the whole device
 // at once (occupancy x CUs), cached per kernel.
@@ -154,7 +158,8 @@ void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* sp
 void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
                       const int* split, int ntiles, int vt, float* y, unsigned* ybits,
-                      float* carry_val, int* carry_row, int* carry_hit);
+                      float* carry_val, int* carry_row, int* carry_hit,
+                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int team,
                       float* y, unsigned* ybits);

@@ -373,7 +373,8 @@ int resident_blocks(const void* kernel, int block) {
 void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
                       const int* split, int ntiles, int vt, float* y, unsigned* ybits,
-                      float* carry_val, int* carry_row, int* carry_hit) {
+                      float* carry_val, int* carry_row, int* carry_hit, hipEvent_t ev_begin,
+                      hipEvent_t ev_end) {
   if (m <= 0 || ntiles <= 0) return;
   if (nnz == 0) {  // every row is empty; no hits
     launch_fill_f32(s, y, m, 0.f);
@@ -394,6 +395,7 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
   else if (masked) SPMSPV_ROW_MERGE(V, kXEnc, true, false)       \
   else if (hits) SPMSPV_ROW_MERGE(V, kXEnc, false, true)         \
   else SPMSPV_ROW_MERGE(V, kXPlain, false, false)
+  if (ev_begin) (void)hipEventRecord(ev_begin, s);
   if (vt == 5) {
     SPMSPV_ROW_MERGE_VT(5)
   } else if (vt == 11) {
@@ -403,6 +405,7 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
   }
 #undef SPMSPV_ROW_MERGE_VT
 #undef SPMSPV_ROW_MERGE
+  if (ev_end) (void)hipEventRecord(ev_end, s);
   const int fgrid = std::min((ntiles + 255) / 256, 1024);
   if (hits)
     hipLaunchKernelGGL(k_row_fixup<true>, dim3(fgrid), dim3(256), 0, s, carry_val, carry_row,

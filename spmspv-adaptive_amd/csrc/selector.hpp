@@ -14,6 +14,9 @@ namespace spmspv {
 
 constexpr int kF = 16, kH1 = 64, kH2 = 64, kC = 8;
 constexpr int kNParam = kH1 * kF + kH1 + kH2 * kH1 + kH2 + kC * kH2 + kC;
+constexpr int kNWeights = kNParam + 2 * kF;  // + the feature normaliser (mean, 1/std)
+static_assert((kF & (kF - 1)) == 0 && (kH1 & (kH1 - 1)) == 0 && (kH2 & (kH2 - 1)) == 0,
+              "layer widths are powers of two (rotated row reads)");
 
 __host__ __device__ inline void build_features(double m, double n, double nnz, double max_row,
                                                double avg_row, double std_row, double equality,
@@ -91,22 +94,36 @@ __device__ void select_in_block(Ctl* ctl, const SelectorArgs& sel) {
   const float* istd = mean + kF;
   if (threadIdx.x < kF) s_x[threadIdx.x] = (s_x[threadIdx.x] - mean[threadIdx.x]) * istd[threadIdx.x];
   __syncthreads();
+  // rows are read at a rotated index (i + j) so that, with the weights in
+  // LDS, the 64 threads of a layer hit 64 different banks
   if (threadIdx.x < kH1) {
-    float a = b1[threadIdx.x];
-    for (int i = 0; i < kF; ++i) a += W1[threadIdx.x * kF + i] * s_x[i];
-    s_h1[threadIdx.x] = a > 0.f ? a : 0.f;
+    const int j = threadIdx.x;
+    float a = b1[j];
+    for (int q = 0; q < kF; ++q) {
+      const int i = (q + j) & (kF - 1);
+      a += W1[j * kF + i] * s_x[i];
+    }
+    s_h1[j] = a > 0.f ? a : 0.f;
   }
   __syncthreads();
   if (threadIdx.x < kH2) {
-    float a = b2[threadIdx.x];
-    for (int i = 0; i < kH1; ++i) a += W2[threadIdx.x * kH1 + i] * s_h1[i];
-    s_h2[threadIdx.x] = a > 0.f ? a : 0.f;
+    const int j = threadIdx.x;
+    float a = b2[j];
+    for (int q = 0; q < kH1; ++q) {
+      const int i = (q + j) & (kH1 - 1);
+      a += W2[j * kH1 + i] * s_h1[i];
+    }
+    s_h2[j] = a > 0.f ? a : 0.f;
   }
   __syncthreads();
   if (threadIdx.x < kC) {
-    float a = b3[threadIdx.x];
-    for (int i = 0; i < kH2; ++i) a += W3[threadIdx.x * kH2 + i] * s_h2[i];
-    s_z[threadIdx.x] = a;
+    const int j = threadIdx.x;
+    float a = b3[j];
+    for (int q = 0; q < kH2; ++q) {
+      const int i = (q + j) & (kH2 - 1);
+      a += W3[j * kH2 + i] * s_h2[i];
+    }
+    s_z[j] = a;
   }
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -106,6 +106,8 @@ _SIGS = {
     "spmspv_csr5spmv": [_vp, _f, _vp],
     "spmspv_spmv": [_vp, _f, _vp],
     "spmspv_autotune2": [_vp, _i, _i, _vp],
+    "spmspv_time_main_kernel": [_vp, _i],
+    "spmspv_main_kernel_ms": [_vp, _vp],
     "spmspv_autotune": [_vp, _i, _vp],
     "spmspv_selector_set_weights": [_vp, _vp, _i],
     "spmspv_selector_mode": [_vp, _i, _i],
@@ -413,6 +415,14 @@ class SpmspvHandle:
                                          vval.ptr if vval else None, yoff.ptr, ykey.ptr, yval.ptr,
                                          ynnz.ptr, _np(used)))
         return used[:nvec]
+
+    def time_main_kernel(self, on=True):
+        return self._c(lib().spmspv_time_main_kernel(self.h, 1 if on else 0))
+
+    def main_kernel_ms(self):
+        v = _f(0)
+        self._c(lib().spmspv_main_kernel_ms(self.h, ctypes.byref(v)))
+        return v.value
 
     def timer_start(self):
         return self._c(lib().spmspv_timer_start(self.h))

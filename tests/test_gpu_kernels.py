@@ -254,6 +254,40 @@ def test_selector_tree_mode_and_run_auto(rmat12):
         assert np.all(np.isfinite(f))
 
 
+def test_fused_mlp_selection_matches_oracle(rmat12):
+    """The MLP inference fused into the scan kernel (committed weights)
+    picks the slot the FP64 oracle network picks on the device-computed
+    features (near-ties, top-2 logit margin < 1e-3, are skipped)."""
+    import os
+    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "spmspv-adaptive_amd",
+                             "selector_weights.npy")).astype(np.float32)
+    M, _ = rmat12
+    M.A.selector_set_weights(w)
+    M.A.selector_mode(0)
+    F = sa.MLP_F
+    mean, istd = w[sa.MLP_NPARAM:sa.MLP_NPARAM + F].astype(np.float64), w[sa.MLP_NPARAM + F:].astype(np.float64)
+    checked = 0
+    try:
+        for k in (1, 3, 40, 400, 1500, 4096):
+            key, val = sa.gen_sparse_vector(M.n, k, 70 + k)
+            dk = sa.DeviceArray.from_host(key, np.int32)
+            dv = sa.DeviceArray.from_host(val, np.float32)
+            M.A.set_sparse_x(dk, dv, k)
+            used = M.A.select()
+            f = M.A.features().astype(np.float64)
+            z = oracle.mlp_forward(F, 64, 64, 8, w[:sa.MLP_NPARAM].astype(np.float64),
+                                   ((f - mean) * istd)[None, :])[0]
+            top = np.sort(z)[::-1]
+            say(f"xnnz {k}: device slot {used}, oracle slot {int(np.argmax(z))}, margin {top[0] - top[1]:.4f}")
+            if top[0] - top[1] < 1e-3:
+                continue
+            assert used == int(np.argmax(z))
+            checked += 1
+    finally:
+        M.A.selector_mode(1)
+    assert checked >= 3
+
+
 def test_cpp_handle_adaptive_app(gpu):
     """The reference's adaptive-app flow written against the C++
     SpmspvHandle shim (examples/adaptive_app.cpp), checked against the
