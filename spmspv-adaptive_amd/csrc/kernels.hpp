@@ -14,7 +14,7 @@ constexpr int kPrepNT = 256, kPrepIPT = 8, kPrepTile = kPrepNT * kPrepIPT;
 // odd items per thread: a lane walking VT consecutive LDS words then hits
 // a different bank than its neighbours (stride 7 is coprime with 32 banks)
 constexpr int kLbNT = 256, kLbVT = 7, kLbTile = kLbNT * kLbVT;
-constexpr int kCmpNT = 256;  // bitmap words per compaction tile
+constexpr int kCmpNT = 64;  // bitmap words per compaction tile (one wave: many small tiles keep every CU busy)
 constexpr int kSortNT = 256, kSortIPT = 12, kSortTile = kSortNT * kSortIPT;
 constexpr int kRedNT = 256, kRedIPT = 7, kRedTile = kRedNT * kRedIPT;
 constexpr int kRowNT = 256, kRowVT = 7, kRowTile = kRowNT * kRowVT;

@@ -183,9 +183,26 @@ __global__ __launch_bounds__(NT) void k_sort_hist(const unsigned* __restrict__ k
   if (n <= small_cap) return;
   for (int k = threadIdx.x; k < 4 * 256; k += NT) s_h[k] = 0u;
   __syncthreads();
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
-    const unsigned k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (shift0 + 8 * p)) & 255u)], 1u);
+  // lanes holding the same digit are found with 8 ballots and counted by
+  // one of them, so skewed keys (R-MAT hub rows) do not serialise on one
+  // LDS counter; the loop bound is wave-uniform for the ballots
+  const uint64_t below = lane() == 0 ? 0ull : (~0ull >> (kWave - lane()));
+  for (long long i0 = (long long)blockIdx.x * NT + (threadIdx.x & ~(kWave - 1)); i0 < n;
+       i0 += (long long)gridDim.x * NT) {
+    const long long i = i0 + lane();
+    const bool ok = i < n;
+    const unsigned k = ok ? keys[i] : 0u;
+    for (int p = 0; p < passes; ++p) {
+      const unsigned dg = (k >> (shift0 + 8 * p)) & 255u;
+      uint64_t peers = __ballot(ok);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (dg >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      if (ok && (peers & below) == 0ull) atomicAdd(&s_h[p * 256 + dg], (unsigned)__popcll(peers));
+    }
   }
   __syncthreads();
   for (int k = threadIdx.x; k < passes * 256; k += NT)
@@ -247,20 +264,40 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
     st_relaxed(st, pack_state(t == 0 ? 2u : 1u, ep, tot));
     unsigned long long excl = 0;
     if (t > 0) {
+      // look back LB predecessors at a time: their state words are loaded
+      // together, so a chain of aggregates costs one memory latency per LB
+      // tiles instead of one per tile
+      constexpr int LB = 8;
       long long k = (long long)t - 1;
       while (true) {
-        unsigned long long s;
-        unsigned f;
-        while (true) {
-          s = ld_relaxed(state + (size_t)k * 256 + d);
-          f = (unsigned)(s >> 62);
-          if (((s >> 38) & ((1u << kEpochBits) - 1)) != ep) f = 0;
-          if (f) break;
-          __builtin_amdgcn_s_sleep(1);
+        unsigned long long sv[LB];
+        unsigned fv[LB];
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+          const long long idx = k - q;
+          sv[q] = idx >= 0 ? ld_relaxed(state + (size_t)idx * 256 + d) : pack_state(2u, ep, 0ull);
         }
-        excl += s & kValMask;
-        if (f == 2) break;
-        --k;
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+          fv[q] = (unsigned)(sv[q] >> 62);
+          if (((sv[q] >> 38) & ((1u << kEpochBits) - 1)) != ep) fv[q] = 0;
+          while (fv[q] == 0) {  // not published yet (rare: older tiles run first)
+            __builtin_amdgcn_s_sleep(1);
+            sv[q] = ld_relaxed(state + (size_t)(k - q) * 256 + d);
+            fv[q] = (unsigned)(sv[q] >> 62);
+            if (((sv[q] >> 38) & ((1u << kEpochBits) - 1)) != ep) fv[q] = 0;
+          }
+        }
+        bool done = false;
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+          if (!done) {
+            excl += sv[q] & kValMask;
+            if (fv[q] == 2) done = true;
+          }
+        }
+        if (done) break;
+        k -= LB;
       }
       st_relaxed(st, pack_state(2u, ep, excl + tot));
     }
