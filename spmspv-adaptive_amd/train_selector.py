@@ -6,9 +6,10 @@ The reference labels each (matrix, x) pair with the fastest of its kernels
 (model/1-test_dtree.py:170-196), scoring them by loss_ratio =
 (t_pred - t_best) / t_best (model/1-test_dtree.py:300-317). Here:
 
-  corpus  builds synthetic matrices on the host, times all eight slots on
-          the GPU (cold cache, median of reps) and records the 16 device
-          features and the rule fallback's choice for each (matrix, x):
+  corpus  builds synthetic matrices on the host, autotunes each one (as a
+          user would once per matrix), times all eight slots on the GPU
+          (cold cache, median of reps) and records the 16 device features
+          and the rule fallback's choice for each (matrix, x):
             uniform random (n = 2^14..2^21 log-uniform, mean nnz/col 2..64),
             R-MAT (a in [0.45, 0.65], d = 0.05, b = c),
             banded (bandwidth 3..201), block-diagonal (dense 32x32 blocks);
@@ -114,6 +115,14 @@ def cmd_corpus(args):
         A = M.A
         yk = sa.DeviceArray(n, np.int32)
         yv = sa.DeviceArray(n, np.float32)
+        if args.autotune:
+            # the per-matrix tuning a user runs once (team sizes, row tile
+            # width, K5/K7 format), on a dense x, before any label is timed
+            key = np.arange(n, dtype=np.int32)
+            dk = sa.DeviceArray.from_host(key, np.int32)
+            dv = sa.DeviceArray.from_host(np.ones(n, np.float32), np.float32)
+            A.set_sparse_x(dk, dv, n)
+            A.autotune(reps=2)
         rng = np.random.default_rng(seed + 7)
         for j in range(4):
             d = float(np.exp(rng.uniform(np.log(1e-5), 0.0)))
@@ -213,6 +222,8 @@ def main():
     c.add_argument("--reps", type=int, default=3)
     c.add_argument("--flush-mb", type=int, default=512)
     c.add_argument("--out", required=True)
+    c.add_argument("--no-autotune", dest="autotune", action="store_false",
+                   help="time the slots with the default widths instead of the autotuned ones")
     c.add_argument("--budget-s", type=float, default=0.0,
                    help="stop starting new matrices after this many seconds (results are kept)")
     t = sub.add_parser("train")
