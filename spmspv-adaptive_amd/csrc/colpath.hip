@@ -48,8 +48,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   // instead of following it.
   const bool selects = sel.mat != nullptr && t == (ntiles == 0 ? 0u : ntiles - 1);
   constexpr int kWPT = (kNWeights + NT - 1) / NT;
+  constexpr int kMF = sizeof(MatFeat) / sizeof(double);
+  static_assert(sizeof(MatFeat) % sizeof(double) == 0 && kMF <= NT, "MatFeat is copied as doubles");
   float wr[kWPT];
-  MatFeat mfr;
+  double mfr = 0.0;  // field threadIdx.x of MatFeat (lanes, not a struct copy)
   if (selects) {
     if (sel.weights) {
 #pragma unroll
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
         wr[i] = idx < kNWeights ? sel.weights[idx] : 0.f;
       }
     }
-    if (threadIdx.x == 0) mfr = *sel.mat;
+    if (threadIdx.x < kMF) mfr = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
   }
   auto run_selector = [&]() {
     if (!selects) return;
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
         if (idx < kNWeights) s_w[idx] = wr[i];
       }
     }
-    if (threadIdx.x == 0) s_mf = mfr;
+    if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
     __syncthreads();
     SelectorArgs sl = sel;
     if (sel.weights) sl.weights = s_w;
@@ -408,29 +410,36 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
   }
   __syncthreads();
   // U rows per thread per round, their loads issued together
+  // (the loads are unconditional, on a valid row, so that the compiler
+  // issues them back to back instead of waiting for each in a branch)
   constexpr int U = 8;
   for (int i0 = threadIdx.x; i0 < btot; i0 += NT * U) {
     int r[U], key[U];
     float v[U];
+    unsigned ok = 0u;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = i0 + u * NT;
-      r[u] = i < btot ? s_rows[i] : -1;
+      const bool in = i < btot;
+      r[u] = s_rows[in ? i : 0];
+      ok |= in ? (1u << u) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (r[u] >= 0) {
-        key[u] = keymap ? keymap[r[u]] : r[u];
-        v[u] = yval ? acc[r[u]] : 0.f;
-      }
+      key[u] = keymap ? keymap[(unsigned)r[u]] : r[u];
+      v[u] = yval ? acc[(unsigned)r[u]] : 0.f;
     }
+    // one wait for all loads here; otherwise every store behind a branch
+    // below waits for the whole memory counter, stores included
+#pragma unroll
+    for (int u = 0; u < U; ++u) asm volatile("" ::"v"(key[u]), "v"(v[u]));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (r[u] >= 0) {
+      if ((ok >> u) & 1u) {
         const long long o = (long long)pre + i0 + u * NT;
         if (ykey) ykey[o] = key[u];
         if (yval) yval[o] = v[u];
-        if (reset_acc) acc[r[u]] = 0.0f;
+        if (reset_acc) acc[(unsigned)r[u]] = 0.0f;
       }
     }
   }
