@@ -991,15 +991,15 @@ int spmspv_select(spmspv_handle_t h, int* kernel_id) {
   if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = ensure_mat_features(h)) ||
       (rc = need_sparse_x(h)))
     return rc;
-  // the selecting workgroup writes {bin_len, choice} straight to host words
+  // the selecting workgroup writes the selector word straight to a host word
   SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
-                   h->sig_dev};
+                   h->sig_dev + 1};
   __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
   if ((rc = run_prep(h, true, &sel))) return rc;
-  long long choice = 0;
-  if ((rc = wait_signal(h, 1, &choice))) return rc;
-  h->binlen_host = __atomic_load_n(h->sig, __ATOMIC_ACQUIRE);
-  *kernel_id = (int)choice;
+  long long word = 0;
+  if ((rc = wait_signal(h, 1, &word))) return rc;
+  h->binlen_host = sel_word_binlen(word);
+  *kernel_id = sel_word_choice(word);
   return SPMSPV_SUCCESS;
 }
 

@@ -23,7 +23,7 @@ int ensure_batch_stash(spmspv_handle_t h, int nvec) {
   if (h->batch_stash) HIP_TRY(h, hipFree(h->batch_stash));
   h->batch_stash = nullptr;
   h->batch_cap = 0;
-  HIP_TRY(h, hipMalloc(&h->batch_stash, sizeof(long long) * 2 * (size_t)nvec));
+  HIP_TRY(h, hipMalloc(&h->batch_stash, sizeof(long long) * (size_t)nvec));
   h->batch_cap = nvec;
   return SPMSPV_SUCCESS;
 }
@@ -65,31 +65,32 @@ int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int
                                   const_cast<float*>(d_vval) + vptr[v], xn)))
       return rc;
     SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
-                     h->batch_stash + 2 * v};
+                     h->batch_stash + v};
     if ((rc = run_prep(h, false, &sel))) return rc;
   }
-  std::vector<long long> st(2 * (size_t)nvec);
-  HIP_TRY(h, hipMemcpyAsync(st.data(), h->batch_stash, sizeof(long long) * 2 * nvec,
+  std::vector<long long> st((size_t)nvec);
+  HIP_TRY(h, hipMemcpyAsync(st.data(), h->batch_stash, sizeof(long long) * nvec,
                             hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   for (int v = 0; v < nvec; ++v) {
-    const long long need = std::min<long long>(h->m, st[2 * v]);
+    const long long need = std::min<long long>(h->m, sel_word_binlen(st[v]));
     if (yoff[v + 1] - yoff[v] < need)
       return set_error(h, SPMSPV_INVALID_ARGUMENT,
                        "run_batched: output capacity of vector " + std::to_string(v) + " is " +
                            std::to_string(yoff[v + 1] - yoff[v]) + ", needs " + std::to_string(need));
-    if (st[2 * v + 1] < 0 || st[2 * v + 1] >= SPMSPV_NUM_SLOTS)
+    const int kid = sel_word_choice(st[v]);
+    if (kid < 0 || kid >= SPMSPV_NUM_SLOTS)
       return set_error(h, SPMSPV_DEVICE_ERROR, "run_batched: selector returned no slot");
   }
   // 2. queued pipelines
   for (int v = 0; v < nvec; ++v) {
     const int xn = vptr[v + 1] - vptr[v];
-    const int kid = (int)st[2 * v + 1];
+    const int kid = sel_word_choice(st[v]);
     if (kernel_used) kernel_used[v] = kid;
     if ((rc = spmspv_set_sparse_x(h, const_cast<int*>(d_vkey) + vptr[v],
                                   const_cast<float*>(d_vval) + vptr[v], xn)))
       return rc;
-    h->binlen_host = st[2 * v];
+    h->binlen_host = sel_word_binlen(st[v]);
     if ((rc = run_kernel(h, kid, nullptr, d_ykey + yoff[v], d_yval + yoff[v], true))) return rc;
     hipLaunchKernelGGL(k_copy_ynnz, dim3(1), dim3(1), 0, h->stream, h->ctl, d_ynnz + v);
   }

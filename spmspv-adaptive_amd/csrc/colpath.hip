@@ -38,7 +38,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   __shared__ unsigned long long s_pre;
   __shared__ long long s_l[NT / kWave];
   __shared__ int s_i[NT / kWave];
-  __shared__ float s_w[kNWeights];  // selector weights (selecting workgroup only)
+  __shared__ __align__(16) float s_w[kNWeights];  // selector weights (selecting workgroup only)
   __shared__ MatFeat s_mf;
   const int xnnz = xnnz_dev ? *xnnz_dev : xnnz_arg;
   const unsigned ntiles = (unsigned)((xnnz + NT * IPT - 1) / (NT * IPT));
@@ -62,13 +62,13 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
     if (threadIdx.x < kMF) mfr = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
   }
-  auto run_selector = [&]() {
+  auto run_selector = [&](long long xn, long long bl, int mxl) {
     if (!selects) return;
     if (sel.weights) {
 #pragma unroll
       for (int i = 0; i < kWPT; ++i) {
         const int idx = threadIdx.x + i * NT;
-        if (idx < kNWeights) s_w[idx] = wr[i];
+        if (idx < kNWeights) s_w[wlds_index(idx)] = wr[i];
       }
     }
     if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     SelectorArgs sl = sel;
     if (sel.weights) sl.weights = s_w;
     sl.mat = &s_mf;
-    select_in_block<NT>(ctl, sl);
+    select_in_wave(ctl, sl, (double)xn, (double)bl, (double)mxl);
   };
   if (ntiles == 0) {
     // empty x: block 0 alone writes the results
@@ -91,12 +91,8 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
       if (split) split[0] = 0;
     }
     __syncthreads();
-    run_selector();
-    if (sel.stash && threadIdx.x == 0) {
-      sel.stash[0] = 0;
-      __threadfence_system();  // bin_len lands before the choice the host polls
-      sel.stash[1] = sel.mat ? ctl->kernel_choice : -1;
-    }
+    run_selector(0, 0, 0);
+    if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(0, sel.mat ? ctl->kernel_choice : -1);
     return;
   }
   if (t >= ntiles) return;
@@ -105,19 +101,30 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   int len[IPT], beg[IPT];
   long long tsum = 0;
   int mx = 0, mn = INT_MAX;
+  // all keys, then all column bounds, are loaded unconditionally (indices
+  // clamped) so that each group is one round trip to memory, not one per item
+  int c[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const long long i = base + j;
-    len[j] = 0;
-    beg[j] = 0;
-    if (i < xnnz) {
-      const int c = xkey[i];
-      // an out-of-range key contributes nothing instead of a wild read
-      const bool ok = (unsigned)c < (unsigned)ncols;
-      const int b = ok ? colptr[c] : 0;
-      const int e = ok ? colptr[c + 1] : 0;
-      beg[j] = b;
-      len[j] = e - b;
+    c[j] = xkey[i < xnnz ? i : (long long)xnnz - 1];
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    // an out-of-range key contributes nothing instead of a wild read
+    // (index masked arithmetically: a select here becomes a branch with a
+    // scalar load on the other side, and every item then waits for memory)
+    const int ok = (base + j < xnnz && (unsigned)c[j] < (unsigned)ncols) ? 1 : 0;
+    const int ci = c[j] & -ok;
+    beg[j] = colptr[ci];
+    len[j] = colptr[ci + ok];
+    c[j] = ok;
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    len[j] = c[j] ? len[j] - beg[j] : 0;
+    beg[j] = c[j] ? beg[j] : 0;
+    if (c[j]) {
       mx = max(mx, len[j]);
       mn = min(mn, len[j]);
     }
@@ -126,9 +133,12 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
   if (threadIdx.x == 0) {
-    tile_mm[2 * t] = mx;
-    tile_mm[2 * t + 1] = mn;
-    __threadfence();  // visible before this tile's aggregate is
+    // device-coherent stores, complete before this tile's aggregate is
+    // published; the last tile reads them with device-coherent loads once
+    // the look-back has seen every earlier tile (no cache write-back fence)
+    __hip_atomic_store(&tile_mm[2 * t], mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&tile_mm[2 * t + 1], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   }
   long long btot;
   const long long bex = block_excl_sum<NT, long long>(tsum, s_l, &btot);
@@ -151,8 +161,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
   }
   if (t != ntiles - 1) return;
-  // last tile: every earlier tile has published, so its tile_mm is visible
-  __threadfence();
+  // last tile: every earlier tile has published, so its tile_mm is in place
   int gmx = 0, gmn = INT_MAX;
   for (unsigned u = threadIdx.x; u < ntiles; u += NT) {
     gmx = max(gmx, __hip_atomic_load(&tile_mm[2 * u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -175,13 +184,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
   }
   __syncthreads();
-  run_selector();
-  // the selector's choice and bin_len were written by thread 0 as well
-  if (sel.stash && threadIdx.x == 0) {
-    sel.stash[0] = ctl->bin_len;
-    __threadfence_system();  // bin_len lands before the choice the host polls
-    sel.stash[1] = sel.mat ? ctl->kernel_choice : -1;
-  }
+  run_selector(xnnz, (long long)pre + btot, gmx);
+  // the selector's choice was written by thread 0 (lane 0 of wave 0)
+  if (sel.stash && threadIdx.x == 0)
+    *sel.stash = sel_word((long long)pre + btot, sel.mat ? ctl->kernel_choice : -1);
 }
 
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,

@@ -317,4 +317,36 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Carry runs of a tiled segmented reduction: tile t left the partial sum
+// cv[t] (hit flag chit[t], nullable) of row crow[t] (-1: none), and
+// consecutive tiles with the same row form one run. For the first tile of
+// a run, sums the run in tile order into (s, h) and returns true. The
+// loads of the tile, its neighbours and its own carry are issued together:
+// one round trip to memory for the common run of one or two tiles.
+template <typename Idx>
+__device__ __forceinline__ bool carry_run(const float* __restrict__ cv, const int* __restrict__ crow,
+                                          const int* __restrict__ chit, Idx t, Idx ntiles, int& r,
+                                          float& s, int& h) {
+  const Idx tp = t > 0 ? t - 1 : 0;
+  const Idx tn = t + 1 < ntiles ? t + 1 : t;
+  r = crow[t];
+  const int rp = crow[tp];
+  const int rn = crow[tn];
+  const float v = cv[t];
+  h = chit ? chit[t] : 0;
+  // keeps the compiler from sinking the loads of v, rn and h below the test
+  asm volatile("" ::"v"(v), "v"(rn), "v"(h), "v"(rp));
+  if (r < 0 || (t > 0 && rp == r)) return false;
+  s = 0.f + v;
+  Idx u = t + 1;
+  int ru = u < ntiles ? rn : -1;
+  while (ru == r) {
+    s += cv[u];
+    if (chit) h |= chit[u];
+    ++u;
+    ru = u < ntiles ? crow[u] : -1;
+  }
+  return true;
+}
+
 }  // namespace spmspv

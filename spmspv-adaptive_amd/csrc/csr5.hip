@@ -289,15 +289,9 @@ __global__ void k_csr5_fixup(const float* __restrict__ carry_val, const int* __r
                              const int* __restrict__ carry_hit, int ntiles, float* y,
                              unsigned char* marks) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
-    const int r = carry_row[t];
-    if (r < 0) continue;
-    if (t > 0 && carry_row[t - 1] == r) continue;
-    float s = 0.f;
-    int h = 0;
-    for (int u = t; u < ntiles && carry_row[u] == r; ++u) {
-      s += carry_val[u];
-      h |= carry_hit[u];
-    }
+    int r, h;
+    float s;
+    if (!carry_run<int>(carry_val, carry_row, carry_hit, t, ntiles, r, s, h)) continue;
     y[r] += s;
     if (marks && h) marks[r] = 1;
   }

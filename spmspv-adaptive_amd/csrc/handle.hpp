@@ -110,7 +110,7 @@ struct spmspv_handle_s {
   size_t csr5_bytes = 0;
   bool csr5_ready = false;
   spmspv::Csr5 csr5{};
-  long long* batch_stash = nullptr;  // {bin_len, choice} per vector of a batch
+  long long* batch_stash = nullptr;  // selector word (sel_word) per vector of a batch
   int batch_cap = 0;
 
   float* dense_x() { return x_dense_user ? x_dense_user : x_dense_int; }

@@ -41,8 +41,18 @@ struct SelectorArgs {
   const MatFeat* mat;
   int mode;              // 0 mlp, 1 tree, 2 fixed
   int fixed;
-  long long* stash;      // when set: {bin_len, kernel choice} of this x are copied here
+  long long* stash;      // when set: the selector word of this x (sel_word) is stored here
 };
+// {bin_len, kernel choice} in one 64-bit word, so that a single store
+// publishes both (no ordering fence); never negative, choice 0xff = none.
+__host__ __device__ inline long long sel_word(long long bin_len, int choice) {
+  return (bin_len << 8) | (long long)(choice & 0xff);
+}
+__host__ __device__ inline long long sel_word_binlen(long long w) { return w >> 8; }
+__host__ __device__ inline int sel_word_choice(long long w) {
+  const int c = (int)(w & 0xff);
+  return c == 0xff ? -1 : c;
+}
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
                  const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
                  int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep);

@@ -338,15 +338,10 @@ template <bool HITS>
 __global__ void k_row_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_row,
                             const int* __restrict__ carry_hit, int ntiles, float* y, unsigned* ybits) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
-    const int r = carry_row[t];
-    if (r < 0) continue;
-    if (t > 0 && carry_row[t - 1] == r) continue;
-    float s = 0.f;
-    int h = 0;
-    for (int u = t; u < ntiles && carry_row[u] == r; ++u) {
-      s += carry_val[u];
-      h |= carry_hit[u];
-    }
+    int r, h;
+    float s;
+    if (!carry_run<int>(carry_val, carry_row, HITS ? carry_hit : nullptr, t, ntiles, r, s, h))
+      continue;
     y[r] += s;
     if (HITS && h) atomicOr(ybits + (r >> 5), 1u << (r & 31));
   }

@@ -1,5 +1,5 @@
-// Selector features and in-workgroup inference (fused into the last
-// workgroup of k_prep, so one launch does scan + features + MLP).
+// Selector features and one-wave inference (fused into the last workgroup
+// of k_prep, so one launch does scan + features + MLP).
 //
 // The reference's selector inputs are the 23 columns of its .info files
 // (model/1-test_dtree.py:63): m, n, nnz, maxRow, minRow, avgRow, range,
@@ -18,29 +18,73 @@ constexpr int kNWeights = kNParam + 2 * kF;  // + the feature normaliser (mean, 
 static_assert((kF & (kF - 1)) == 0 && (kH1 & (kH1 - 1)) == 0 && (kH2 & (kH2 - 1)) == 0,
               "layer widths are powers of two (rotated row reads)");
 
+// Feature i of the selector input (0 <= i < kF). One function per feature
+// index, so that the device computes the 16 features in 16 lanes at once
+// (one double log2 per lane) and the host gets the same arithmetic.
+__host__ __device__ inline float feature_value(int i, double m, double n, double nnz, double max_row,
+                                               double avg_row, double std_row, double equality,
+                                               double gini, double xnnz, double bin_len,
+                                               double max_len) {
+  if (i == 5) return (float)(std_row / (avg_row + 1e-9));
+  if (i == 6) return (float)equality;
+  if (i == 7) return (float)gini;
+  const double gm1 = 4.0 * (3.0 * xnnz + 3.0 * bin_len);                   // spmspv.h:2727-2731
+  const double gm2 = 4.0 * ((m + 1) + 2.0 * bin_len + nnz + m);            // spmspv.h:2737-2740
+  const double gm3 = 4.0 * ((m + 1) + 3.0 * nnz + m);                      // spmspv.h:2746-2749
+  double a;
+  switch (i) {
+    case 0: a = m > 1 ? m : 1.0; break;
+    case 1: a = n > 1 ? n : 1.0; break;
+    case 2: a = 1.0 + nnz; break;
+    case 3: a = 1.0 + max_row; break;
+    case 4: a = 1.0 + avg_row; break;
+    case 8: a = 1.0 + xnnz; break;
+    case 9: a = 1.0 + bin_len; break;
+    case 10: a = 1.0 + max_len; break;
+    case 11: a = xnnz / (n > 0 ? n : 1.0) + 1e-7; break;
+    case 12: a = bin_len / (nnz > 0 ? nnz : 1.0) + 1e-7; break;
+    case 13: a = 1.0 + bin_len / (xnnz > 1 ? xnnz : 1.0); break;
+    case 14: a = (gm1 + 1.0) / gm3; break;
+    default: a = gm2 / gm3; break;
+  }
+  return (float)log2(a);
+}
+
 __host__ __device__ inline void build_features(double m, double n, double nnz, double max_row,
                                                double avg_row, double std_row, double equality,
                                                double gini, double xnnz, double bin_len,
                                                double max_len, float* f) {
-  const double gm1 = 4.0 * (3.0 * xnnz + 3.0 * bin_len);                   // spmspv.h:2727-2731
-  const double gm2 = 4.0 * ((m + 1) + 2.0 * bin_len + nnz + m);            // spmspv.h:2737-2740
-  const double gm3 = 4.0 * ((m + 1) + 3.0 * nnz + m);                      // spmspv.h:2746-2749
-  f[0] = (float)log2(m > 1 ? m : 1.0);
-  f[1] = (float)log2(n > 1 ? n : 1.0);
-  f[2] = (float)log2(1.0 + nnz);
-  f[3] = (float)log2(1.0 + max_row);
-  f[4] = (float)log2(1.0 + avg_row);
-  f[5] = (float)(std_row / (avg_row + 1e-9));
-  f[6] = (float)equality;
-  f[7] = (float)gini;
-  f[8] = (float)log2(1.0 + xnnz);
-  f[9] = (float)log2(1.0 + bin_len);
-  f[10] = (float)log2(1.0 + max_len);
-  f[11] = (float)log2(xnnz / (n > 0 ? n : 1.0) + 1e-7);
-  f[12] = (float)log2(bin_len / (nnz > 0 ? nnz : 1.0) + 1e-7);
-  f[13] = (float)log2(1.0 + bin_len / (xnnz > 1 ? xnnz : 1.0));
-  f[14] = (float)log2((gm1 + 1.0) / gm3);
-  f[15] = (float)log2(gm2 / gm3);
+  for (int i = 0; i < kF; ++i)
+    f[i] = feature_value(i, m, n, nnz, max_row, avg_row, std_row, equality, gini, xnnz, bin_len,
+                         max_len);
+}
+
+// Weights as the inference reads them from LDS. The flat order (W1, b1,
+// W2, b2, W3, b3, mean, 1/std; W row-major [out][in]) is kept per region,
+// but each matrix is permuted so that the one wave running the network
+// reads its rows in 16-byte pieces lying side by side across lanes (no
+// bank conflicts): W1 and W2 as [in/4][out][4], W3 as [in/8][class][8]
+// (lane = part * kC + class, 8 inputs per part).
+constexpr int kOB1 = kH1 * kF, kOW2 = kOB1 + kH1, kOB2 = kOW2 + kH2 * kH1, kOW3 = kOB2 + kH2;
+constexpr int kOB3 = kOW3 + kC * kH2, kOMean = kOB3 + kC, kOIstd = kOMean + kF;
+constexpr int kW3Part = 8;
+static_assert(kH1 == kWave && kH2 == kWave && kC * (kH2 / kW3Part) == kWave && kF <= kWave,
+              "one lane per hidden unit; the last layer fills the wave with (part, class)");
+static_assert(kOW2 % 4 == 0 && kOW3 % 4 == 0, "16-byte aligned matrices");
+__host__ __device__ inline int wlds_index(int idx) {
+  if (idx < kOB1) {
+    const int j = idx / kF, k = idx % kF;
+    return ((k >> 2) * kH1 + j) * 4 + (k & 3);
+  }
+  if (idx >= kOW2 && idx < kOB2) {
+    const int r = idx - kOW2, j = r / kH1, k = r % kH1;
+    return kOW2 + ((k >> 2) * kH2 + j) * 4 + (k & 3);
+  }
+  if (idx >= kOW3 && idx < kOB3) {
+    const int r = idx - kOW3, c = r / kH2, k = r % kH2;
+    return kOW3 + ((k / kW3Part) * kC + c) * kW3Part + (k % kW3Part);
+  }
+  return idx;
 }
 
 // Hand-written fallback rules (the role of the reference's 3-way rule stubs,
@@ -56,82 +100,85 @@ __host__ __device__ inline int tree_select(double n, double nnz, double avg_row,
   return skewed ? 2 : 0;
 }
 
-// Runs in one workgroup of NT >= 64 threads after ctl has bin_len/max/min.
-template <int NT>
-__device__ void select_in_block(Ctl* ctl, const SelectorArgs& sel) {
-  __shared__ float s_x[kF];
-  __shared__ float s_h1[kH1];
-  __shared__ float s_h2[kH2];
-  __shared__ float s_z[kC];
+__device__ __forceinline__ float lane_bcast(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// Runs in wave 0 of the calling workgroup (the other waves return at
+// once; no barrier inside). sel.weights, when set, points at the weights in
+// LDS in the wlds_index layout. Writes ctl->feat and ctl->kernel_choice
+// (lane 0 writes the choice).
+__device__ void select_in_wave(Ctl* ctl, const SelectorArgs& sel, double xnnz, double bin_len,
+                               double max_len) {
+  if (threadIdx.x >= kWave) return;
+  const int l = threadIdx.x;
   const MatFeat* mf = sel.mat;
-  if (threadIdx.x == 0) {
-    float f[kF];
-    build_features(mf->m, mf->n, mf->nnz, mf->max_row, mf->avg_row, mf->std_row, mf->equality,
-                   mf->gini, (double)ctl->xnnz, (double)ctl->bin_len, (double)ctl->max_len, f);
-    for (int i = 0; i < kF; ++i) {
-      ctl->feat[i] = f[i];
-      s_x[i] = f[i];
-    }
-  }
-  __syncthreads();
+  const float f = feature_value(l & (kF - 1), mf->m, mf->n, mf->nnz, mf->max_row, mf->avg_row,
+                                mf->std_row, mf->equality, mf->gini, xnnz, bin_len, max_len);
+  if (l < kF) ctl->feat[l] = f;
   if (sel.mode == 2) {
-    if (threadIdx.x == 0) ctl->kernel_choice = sel.fixed;
+    if (l == 0) ctl->kernel_choice = sel.fixed;
     return;
   }
   if (sel.mode == 1 || !sel.weights) {
-    if (threadIdx.x == 0)
-      ctl->kernel_choice = tree_select(mf->n, mf->nnz, mf->avg_row, mf->std_row, (double)ctl->xnnz,
-                                       (double)ctl->bin_len, (double)ctl->max_len);
+    if (l == 0)
+      ctl->kernel_choice = tree_select(mf->n, mf->nnz, mf->avg_row, mf->std_row, xnnz, bin_len, max_len);
     return;
   }
-  const float* W1 = sel.weights;
-  const float* b1 = W1 + kH1 * kF;
-  const float* W2 = b1 + kH1;
-  const float* b2 = W2 + kH2 * kH1;
-  const float* W3 = b2 + kH2;
-  const float* b3 = W3 + kC * kH2;
-  const float* mean = b3 + kC;
-  const float* istd = mean + kF;
-  if (threadIdx.x < kF) s_x[threadIdx.x] = (s_x[threadIdx.x] - mean[threadIdx.x]) * istd[threadIdx.x];
-  __syncthreads();
-  // rows are read at a rotated index (i + j) so that, with the weights in
-  // LDS, the 64 threads of a layer hit 64 different banks
-  if (threadIdx.x < kH1) {
-    const int j = threadIdx.x;
-    float a = b1[j];
-    for (int q = 0; q < kF; ++q) {
-      const int i = (q + j) & (kF - 1);
-      a += W1[j * kF + i] * s_x[i];
+  const float* w = sel.weights;
+  const int fi = l & (kF - 1);
+  const float xn = (f - w[kOMean + fi]) * w[kOIstd + fi];
+  // layer 1: lane j = hidden unit j; inputs broadcast from their lanes
+  float a = w[kOB1 + l];
+#pragma unroll
+  for (int kk = 0; kk < kF / 4; ++kk) {
+    const float4 q = *reinterpret_cast<const float4*>(w + ((kk * kH1 + l) << 2));
+    a += q.x * lane_bcast(xn, 4 * kk);
+    a += q.y * lane_bcast(xn, 4 * kk + 1);
+    a += q.z * lane_bcast(xn, 4 * kk + 2);
+    a += q.w * lane_bcast(xn, 4 * kk + 3);
+  }
+  const float h1 = a > 0.f ? a : 0.f;
+  // layer 2
+  a = w[kOB2 + l];
+#pragma unroll
+  for (int kk = 0; kk < kH1 / 4; ++kk) {
+    const float4 q = *reinterpret_cast<const float4*>(w + kOW2 + ((kk * kH2 + l) << 2));
+    a += q.x * lane_bcast(h1, 4 * kk);
+    a += q.y * lane_bcast(h1, 4 * kk + 1);
+    a += q.z * lane_bcast(h1, 4 * kk + 2);
+    a += q.w * lane_bcast(h1, 4 * kk + 3);
+  }
+  const float h2 = a > 0.f ? a : 0.f;
+  // layer 3: lane = part * kC + class; 8 inputs per lane, then the parts
+  // are summed across lanes
+  const int c = l & (kC - 1), part = l / kC;
+  const float4 u0 = *reinterpret_cast<const float4*>(w + kOW3 + l * kW3Part);
+  const float4 u1 = *reinterpret_cast<const float4*>(w + kOW3 + l * kW3Part + 4);
+  const int k0 = part * kW3Part;
+  float z = u0.x * __shfl(h2, k0);
+  z += u0.y * __shfl(h2, k0 + 1);
+  z += u0.z * __shfl(h2, k0 + 2);
+  z += u0.w * __shfl(h2, k0 + 3);
+  z += u1.x * __shfl(h2, k0 + 4);
+  z += u1.y * __shfl(h2, k0 + 5);
+  z += u1.z * __shfl(h2, k0 + 6);
+  z += u1.w * __shfl(h2, k0 + 7);
+#pragma unroll
+  for (int d = kC; d < kWave; d <<= 1) z += __shfl_xor(z, d);
+  z += w[kOB3 + c];
+  // argmax over the classes in lanes 0..kC-1 (the first maximum wins)
+  int best = 0;
+  float bz = lane_bcast(z, 0);
+#pragma unroll
+  for (int k = 1; k < kC; ++k) {
+    const float zk = lane_bcast(z, k);
+    if (zk > bz) {
+      bz = zk;
+      best = k;
     }
-    s_h1[j] = a > 0.f ? a : 0.f;
   }
-  __syncthreads();
-  if (threadIdx.x < kH2) {
-    const int j = threadIdx.x;
-    float a = b2[j];
-    for (int q = 0; q < kH1; ++q) {
-      const int i = (q + j) & (kH1 - 1);
-      a += W2[j * kH1 + i] * s_h1[i];
-    }
-    s_h2[j] = a > 0.f ? a : 0.f;
-  }
-  __syncthreads();
-  if (threadIdx.x < kC) {
-    const int j = threadIdx.x;
-    float a = b3[j];
-    for (int q = 0; q < kH2; ++q) {
-      const int i = (q + j) & (kH2 - 1);
-      a += W3[j * kH2 + i] * s_h2[i];
-    }
-    s_z[j] = a;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int best = 0;
-    for (int c = 1; c < kC; ++c)
-      if (s_z[c] > s_z[best]) best = c;
-    ctl->kernel_choice = best;
-  }
+  if (l == 0) ctl->kernel_choice = best;
 }
 
 }  // namespace spmspv

@@ -454,11 +454,9 @@ __global__ void k_carry_fixup(const float* __restrict__ carry_val, const int* __
   const long long ntiles = (n + tile - 1) / tile;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntiles;
        t += (long long)gridDim.x * blockDim.x) {
-    const int idx = carry_idx[t];
-    if (idx < 0) continue;
-    if (t > 0 && carry_idx[t - 1] == idx) continue;
-    float s = 0.f;
-    for (long long u = t; u < ntiles && carry_idx[u] == idx; ++u) s += carry_val[u];
+    int idx, h;
+    float s;
+    if (!carry_run<long long>(carry_val, carry_idx, nullptr, t, ntiles, idx, s, h)) continue;
     yval[idx] += s;
   }
 }
