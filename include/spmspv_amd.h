@@ -223,9 +223,20 @@ int spmspv_selector_set_weights(spmspv_handle_t h, const float* w, int count);
 int spmspv_selector_mode(spmspv_handle_t h, int mode, int fixed_kernel);
 /* Features + MLP inference in one device launch; returns the chosen slot. */
 int spmspv_select(spmspv_handle_t h, int* kernel_id);
-/* spmspv_select then spmspv_run of the chosen slot. */
+/* spmspv_select then spmspv_run of the chosen slot (the reference's
+ * predict-then-run flow, la/test/new_test_app.cu:121-131, with its stub
+ * predictors replaced by the selector). With speculation on (the
+ * default), the slot the selector chose last time for an x of the same
+ * size class (xnnz within a power of two) is queued behind the selector
+ * before its choice reaches the host; its launches do nothing on the device
+ * unless the choice matches. Results are the same either way. */
 int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_key,
                     float* d_y_val, int* y_nnz, int* kernel_used);
+/* No counterpart in the reference: switches the speculative queueing of
+ * spmspv_run_auto on (1) or off (0), and reports its right and wrong
+ * guesses so far (either pointer may be null). */
+int spmspv_set_speculation(spmspv_handle_t h, int on);
+int spmspv_speculation_stats(spmspv_handle_t h, long long* hits, long long* misses);
 
 /* Training on the GPU with the library's own forward/backward kernels.
  * X[B*16] raw features (host), targets[B*8] soft labels (host; rows sum to

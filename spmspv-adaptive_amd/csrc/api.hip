@@ -87,9 +87,9 @@ static size_t carve(spmspv_handle_t h, char* base) {
   sb.valA = c.take<float>(nnz);
   sb.keyB = c.take<unsigned>(nnz);
   sb.valB = c.take<float>(nnz);
-  sb.hist = c.take<unsigned>(4 * 256);
-  sb.digoff = c.take<unsigned>(4 * 256);
-  sb.state = c.take<unsigned long long>(sort_tiles * 256);
+  const size_t dscan_tiles = sort_tiles * 256 / kDScanTile + 2;
+  sb.dcount = c.take<unsigned>(sort_tiles * 256);
+  sb.dstate = c.take<unsigned long long>(dscan_tiles);
   sb.red_state = c.take<unsigned long long>(red_tiles);
   sb.carry_val = c.take<float>(red_tiles);
   sb.carry_idx = c.take<int>(red_tiles);
@@ -108,7 +108,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
   float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
   if (base) {
     h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
-    h->sort_words = sort_tiles * 256;
+    h->sort_words = dscan_tiles;
     h->red_words = red_tiles;
     h->ctl = ctl;
     h->mat = mat;
@@ -270,7 +270,7 @@ Ep next_ep(spmspv_handle_t h, unsigned k) {
   // host epochs live in [1, 2^23); before they wrap, every look-back state
   // array is cleared so that no stale word can carry a reused epoch
   if (h->epoch_host + k >= (1u << 23) - 1) {
-    launch_fill_i32(h->stream, reinterpret_cast<int*>(h->sb.state), (long long)h->sort_words * 2, 0);
+    launch_fill_i32(h->stream, reinterpret_cast<int*>(h->sb.dstate), (long long)h->sort_words * 2, 0);
     launch_fill_i32(h->stream, reinterpret_cast<int*>(h->sb.red_state), (long long)h->red_words * 2, 0);
     launch_fill_i32(h->stream, reinterpret_cast<int*>(h->scan_state), (long long)h->scan_words * 2, 0);
     h->epoch_host = 1;
@@ -320,17 +320,18 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
     case SPMSPV_K_COL_LB_ATOMIC: {
       if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
       float* acc = y_dense ? y_dense : h->acc;
-      if (y_dense) launch_fill_f32(s, y_dense, m, 0.f);
+      if (y_dense) launch_fill_f32(s, y_dense, m, 0.f, h->gate);
       if (kid == SPMSPV_K_COL_NAIVE_ATOMIC) {
         launch_naive_col(s, 0, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, nullptr,
-                         h->n, h->xnnz, nullptr, h->team_col, acc, h->marks, nullptr, nullptr);
+                         h->n, h->xnnz, nullptr, h->team_col, acc, h->marks, nullptr, nullptr,
+                         h->gate);
       } else {
         if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
         launch_lb_gather(s, 0, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
-                         h->xnnz, nullptr, acc, h->marks, nullptr, nullptr, lb_tiles_hint(h));
+                         h->xnnz, nullptr, acc, h->marks, nullptr, nullptr, lb_tiles_hint(h), h->gate);
       }
       launch_compact(s, nullptr, h->marks, m, acc, y_dense ? 0 : 1, y_key, y_val, h->scan_state, h->ctl,
-                     next_ep(h, 1));
+                     next_ep(h, 1), nullptr, h->gate);
       h->launches += 2;
       return check_launch(h, "column atomic kernel");
     }
@@ -401,11 +402,11 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         int nk = 0;
         launch_csr5(s, h->csr5, masked, hits, xin, nullptr, m, h->nnz, yout, yout != h->acc, h->marks,
                     &vals, &kmap, &nk, h->time_main ? h->evk0 : nullptr,
-                    h->time_main ? h->evk1 : nullptr);
+                    h->time_main ? h->evk1 : nullptr, h->gate);
         h->main_timed = h->time_main;
         if (hits)
           launch_compact(s, nullptr, h->marks, nk, vals, vals == h->acc ? 1 : 0, y_key, y_val,
-                         h->scan_state, h->ctl, next_ep(h, 1), kmap);
+                         h->scan_state, h->ctl, next_ep(h, 1), kmap, h->gate);
         h->launches += 3;
         return check_launch(h, "CSR5 row kernel");
       }
@@ -422,7 +423,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         launch_row_merge(s, masked, hits, h->csr_row, h->csr_col, h->csr_val, xin, nullptr, m, h->nnz,
                          h->row_split, h->n_row_tiles, h->row_vt, yout, h->bitmap, h->row_carry_val,
                          h->row_carry_row, h->row_carry_hit, h->time_main ? h->evk0 : nullptr,
-                         h->time_main ? h->evk1 : nullptr);
+                         h->time_main ? h->evk1 : nullptr, h->gate);
         h->main_timed = h->time_main;
       } else {
         if ((rc = need_dense_x(h))) return rc;
@@ -432,7 +433,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       }
       if (hits)
         launch_compact(s, h->bitmap, nullptr, m, yout, yout == h->acc ? 1 : 0, y_key, y_val,
-                       h->scan_state, h->ctl, next_ep(h, 1));
+                       h->scan_state, h->ctl, next_ep(h, 1), nullptr, h->gate);
       h->launches += 2;
       return check_launch(h, "row kernel");
     }
@@ -1012,13 +1013,83 @@ int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]) {
   return SPMSPV_SUCCESS;
 }
 
+// Speculative dispatch. The selector's choice reaches the host one round
+// trip after the selector kernel; instead of leaving the device idle for
+// it, the slot this handle's selector chose last time for an x of the same
+// size class (xnnz within a power of two) is queued right behind the
+// selector, every launch carrying a gate on the device-side choice. A right
+// guess costs nothing more; a wrong one costs the gated launches (each exits
+// at once) and the chosen slot is queued as usual. Only slots whose launches
+// need nothing from the host about x (no bin_len readback) are guessed: the
+// atomic column slots and the load-balanced row slots.
+static bool speculable(spmspv_handle_t h, int kid) {
+  if (kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_LB_ATOMIC) return h->prep_valid;
+  if (kid == SPMSPV_K_ROW_LB || kid == SPMSPV_K_SPMV_LB)
+    return h->csr_row && (h->spmv_type == 1 ? h->csr5_ready : h->row_ready);
+  return false;
+}
+
+static int size_class(int xnnz) { return xnnz > 0 ? 32 - __builtin_clz((unsigned)xnnz) : 0; }
+
 int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_key, float* d_y_val,
                     int* y_nnz, int* kernel_used) {
-  int kid = 0;
-  int rc = spmspv_select(h, &kid);
-  if (rc) return rc;
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  int rc;
+  if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = ensure_mat_features(h)) ||
+      (rc = need_sparse_x(h)))
+    return rc;
+  SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
+                   h->sig_dev + 1};
+  __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
+  if ((rc = run_prep(h, true, &sel))) return rc;
+  const int cls = size_class(h->xnnz);
+  const int guess = h->spec_last[cls] - 1;
+  const bool spec = h->speculate && guess >= 0 && speculable(h, guess);
+  if (spec) {
+    struct Ungate {
+      spmspv_handle_t h;
+      ~Ungate() { h->gate = Gate{}; }
+    } ungate{h};
+    h->gate = Gate{&h->ctl->kernel_choice, guess};
+    if (y_nnz) __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
+    if ((rc = run_kernel(h, guess, d_y_dense, d_y_key, d_y_val, y_nnz != nullptr))) return rc;
+    if (y_nnz) {
+      launch_signal_int(h->stream, &h->ctl->ynnz, h->sig_dev + 3, h->gate);
+      if ((rc = check_launch(h, "result count"))) return rc;
+    }
+  }
+  long long word = 0;
+  if ((rc = wait_signal(h, 1, &word))) return rc;
+  h->binlen_host = sel_word_binlen(word);
+  const int kid = sel_word_choice(word);
+  if (kid < 0 || kid >= SPMSPV_NUM_SLOTS)
+    return set_error(h, SPMSPV_DEVICE_ERROR, "selector returned no slot");
+  h->spec_last[cls] = (signed char)(kid + 1);
   if (kernel_used) *kernel_used = kid;
+  if (spec && kid == guess) {
+    h->spec_hits++;
+    if (y_nnz) {
+      long long v = 0;
+      if ((rc = wait_signal(h, 3, &v))) return rc;
+      *y_nnz = (int)v;
+    }
+    return SPMSPV_SUCCESS;
+  }
+  if (spec) h->spec_misses++;
   return spmspv_run(h, kid, alpha, d_y_dense, d_y_key, d_y_val, y_nnz);
+}
+
+int spmspv_set_speculation(spmspv_handle_t h, int on) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->speculate = on != 0;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_speculation_stats(spmspv_handle_t h, long long* hits, long long* misses) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (hits) *hits = h->spec_hits;
+  if (misses) *misses = h->spec_misses;
+  return SPMSPV_SUCCESS;
 }
 
 int spmspv_time_main_kernel(spmspv_handle_t h, int on) {

@@ -521,7 +521,7 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
   // before (ctl->epoch + offset) could wrap onto a stale word
   h->dev_levels += nlev;
   if (h->dev_levels > 200000) {
-    launch_fill_i32(s, reinterpret_cast<int*>(h->sb.state), (long long)h->sort_words * 2, 0);
+    launch_fill_i32(s, reinterpret_cast<int*>(h->sb.dstate), (long long)h->sort_words * 2, 0);
     launch_fill_i32(s, reinterpret_cast<int*>(h->sb.red_state), (long long)h->red_words * 2, 0);
     launch_fill_i32(s, reinterpret_cast<int*>(h->scan_state), (long long)h->scan_words * 2, 0);
     h->dev_levels = 0;

@@ -5,8 +5,9 @@
 // out (#if 0, :999-1071) and its atomic variant truncates values to int
 // (:684). Here:
 //   1. merge-path load-balanced gather writes every product (row, a*x);
-//   2. one stable onesweep partition pass on the top 8 bits of the row
-//      groups the products into 256 row buckets (the count/scan/extract);
+//   2. one stable counted partition pass (per-tile digit counts, their
+//      scan, scatter) on the top 8 bits of the row groups the products into
+//      256 row buckets (the count/scan/extract);
 //   3. one workgroup per (bucket, row window of 8192 rows) accumulates its
 //      products into an LDS row window with LDS float atomics and a hit
 //      bit array, then writes the window's structural support in row order
@@ -24,7 +25,7 @@ constexpr int kWinBits = 13;  // 8192 rows per LDS window (32 KB of floats)
 template <int NT>
 __global__ __launch_bounds__(NT) void k_bucket_acc(const unsigned* __restrict__ keys,
                                                    const float* __restrict__ vals,
-                                                   const unsigned* __restrict__ hist,
+                                                   const unsigned* __restrict__ dstart,
                                                    const long long* __restrict__ n_dev, int m,
                                                    int bshift, int wbits, Ctl* ctl,
                                                    unsigned long long* state, Ep epa,
@@ -48,20 +49,12 @@ __global__ __launch_bounds__(NT) void k_bucket_acc(const unsigned* __restrict__ 
     for (int k = threadIdx.x; k < win; k += NT) s_acc[k] = 0.f;
     for (int k = threadIdx.x; k < (win + 31) / 32; k += NT) s_hit[k] = 0u;
     __syncthreads();
-    // bucket b's items: [sum of hist[0..b), + hist[b]) of the partition pass
-    __shared__ long long s_ib[2];
-    if (threadIdx.x < kWave) {
-      unsigned long long pre = 0;
-      for (int d = lane(); d < b; d += kWave) pre += hist[d];
-      pre = wave_sum(pre);
-      if (threadIdx.x == 0) {
-        s_ib[0] = (long long)pre;
-        s_ib[1] = (long long)pre + hist[b];
-      }
-    }
-    __syncthreads();
-    const long long i0 = s_ib[0];
-    const long long i1 = s_ib[1];
+    // bucket b's items: the partition pass left the start of digit d at
+    // dstart[d * stiles] (stiles = its tiles)
+    const long long n = *n_dev;
+    const long long stiles = (n + kSortTile - 1) / kSortTile;
+    const long long i0 = n > 0 ? (long long)dstart[(size_t)b * stiles] : 0;
+    const long long i1 = n > 0 && b + 1 < 256 ? (long long)dstart[(size_t)(b + 1) * stiles] : n;
     for (long long i = i0 + threadIdx.x; i < i1; i += NT) {
       const int r = (int)keys[i] - r0;
       if ((unsigned)r < (unsigned)win) {
@@ -119,7 +112,7 @@ void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, unsigned l
   launch_partition_pass(s, b, bshift, n_dev, ctl, binlen_hint, n_cap, ep);
   const int ntiles = (((m - 1) >> bshift) + 1) << (bshift - wbits);
   const Ep e2{ep.v + 1u, ep.dev < 0 ? -1 : ep.dev + 1};
-  hipLaunchKernelGGL(k_bucket_acc<256>, dim3(ntiles), dim3(256), 0, s, b.keyB, b.valB, b.hist, n_dev, m,
+  hipLaunchKernelGGL(k_bucket_acc<256>, dim3(ntiles), dim3(256), 0, s, b.keyB, b.valB, b.dcount, n_dev, m,
                      bshift, wbits, ctl, state, e2, ykey, yval);
 }
 

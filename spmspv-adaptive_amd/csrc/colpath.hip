@@ -213,7 +213,9 @@ __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr
                                                   const int* __restrict__ ptr, int ncols, int xnnz_arg,
                                                   const int* __restrict__ xnnz_dev, int lg_team,
                                                   float* acc, unsigned char* marks,
-                                                  int* __restrict__ bkey, float* __restrict__ bval) {
+                                                  int* __restrict__ bkey, float* __restrict__ bval,
+                                                  Gate g) {
+  if (gate_shut(g)) return;
   const int xnnz = xnnz_dev ? *xnnz_dev : xnnz_arg;
   const int team = 1 << lg_team;
   const int tpb = NT >> lg_team;
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
                       const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,
-                      float* bval) {
+                      float* bval, Gate g) {
   int lg = 0;
   while ((1 << lg) < team && lg < 6) ++lg;
   const int tpb = 256 >> lg;
@@ -253,10 +255,10 @@ void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* cro
   if (grid < 1) grid = 1;
   if (mode == 0)
     hipLaunchKernelGGL((k_naive_col<256, 0>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
-                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval);
+                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval, g);
   else
     hipLaunchKernelGGL((k_naive_col<256, 1>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
-                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval);
+                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval, g);
 }
 
 // ------------------------------------------------------------ LB gather
@@ -273,7 +275,8 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
                                                   const Ctl* __restrict__ ctl, int xnnz_arg,
                                                   const int* __restrict__ xnnz_dev, float* acc,
                                                   unsigned char* marks, int* __restrict__ bkey,
-                                                  float* __restrict__ bval) {
+                                                  float* __restrict__ bval, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int T = NT * VT;
   __shared__ int s_ptr[T + 2];
   __shared__ int s_beg[T + 2];
@@ -345,15 +348,15 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
 void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval,
                       const int* split, const int* crow, const float* cval, const Ctl* ctl,
                       int xnnz, const int* xnnz_dev, float* acc, unsigned char* marks, int* bkey,
-                      float* bval, long long tiles_hint) {
+                      float* bval, long long tiles_hint, Gate g) {
   long long grid = tiles_hint > 0 ? std::min<long long>(tiles_hint, 8192) : 2048;
   if (grid < 1) grid = 1;
   if (mode == 0)
     hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 0>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
-                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval);
+                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval, g);
   else
     hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 1>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
-                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval);
+                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval, g);
 }
 
 // -------------------------------------------------------------- compact
@@ -374,7 +377,8 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
                                                 float* __restrict__ acc, int reset_acc,
                                                 int* __restrict__ ykey, float* __restrict__ yval,
                                                 unsigned long long* state, Ctl* ctl, Ep ep,
-                                                const int* __restrict__ keymap) {
+                                                const int* __restrict__ keymap, Gate g) {
+  if (gate_shut(g)) return;
   __shared__ unsigned long long s_pre;
   __shared__ int s_i[NT / kWave];
   __shared__ int s_rows[NT * 32];
@@ -454,25 +458,26 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
 
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep, const int* keymap) {
+                    Ep ep, const int* keymap, Gate g) {
   const int nwords = (m + 31) >> 5;
   const int grid = std::max(1, (nwords + kCmpNT - 1) / kCmpNT);
   if (marks)
     hipLaunchKernelGGL((k_compact<kCmpNT, true>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep, keymap);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap, g);
   else
     hipLaunchKernelGGL((k_compact<kCmpNT, false>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep, keymap);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap, g);
 }
 
 // ------------------------------------------------------------ utilities
-__global__ void k_signal_int(const int* src, long long* dst) {
+__global__ void k_signal_int(const int* src, long long* dst, Gate g) {
+  if (gate_shut(g)) return;
   const long long v = *src;
   __threadfence_system();
   *dst = v;
 }
-void launch_signal_int(hipStream_t s, const int* src, long long* dst) {
-  hipLaunchKernelGGL(k_signal_int, dim3(1), dim3(1), 0, s, src, dst);
+void launch_signal_int(hipStream_t s, const int* src, long long* dst, Gate g) {
+  hipLaunchKernelGGL(k_signal_int, dim3(1), dim3(1), 0, s, src, dst, g);
 }
 
 __global__ void k_bitmap_from_dense(const float* __restrict__ v, int n, unsigned* __restrict__ bits) {
@@ -492,7 +497,8 @@ void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bi
   hipLaunchKernelGGL(k_bitmap_from_dense, dim3(grid), dim3(256), 0, s, v, n, bits);
 }
 
-__global__ void k_fill_f32(float* p, long long n, float v) {
+__global__ void k_fill_f32(float* p, long long n, float v, Gate g) {
+  if (gate_shut(g)) return;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -502,10 +508,10 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
        i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
 }
-void launch_fill_f32(hipStream_t s, float* p, long long n, float v) {
+void launch_fill_f32(hipStream_t s, float* p, long long n, float v, Gate g) {
   if (n <= 0) return;
   const long long grid = std::min<long long>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)grid), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)grid), dim3(256), 0, s, p, n, v, g);
 }
 void launch_fill_i32(hipStream_t s, int* p, long long n, int v) {
   if (n <= 0) return;

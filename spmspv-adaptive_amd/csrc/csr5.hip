@@ -151,7 +151,8 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
                                               float* __restrict__ y, unsigned char* __restrict__ marks,
                                               float* __restrict__ carry_val,
                                               int* __restrict__ carry_row,
-                                              int* __restrict__ carry_hit) {
+                                              int* __restrict__ carry_hit, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int T = kWave * S;
   const int lanes = lane();
   const int waves = gridDim.x * (blockDim.x / kWave);
@@ -287,7 +288,8 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
 // rows that run through several tiles: their tail parts, added in tile order
 __global__ void k_csr5_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_row,
                              const int* __restrict__ carry_hit, int ntiles, float* y,
-                             unsigned char* marks) {
+                             unsigned char* marks, Gate g) {
+  if (gate_shut(g)) return;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
     int r, h;
     float s;
@@ -299,7 +301,8 @@ __global__ void k_csr5_fixup(const float* __restrict__ carry_val, const int* __r
 
 // dense y from the non-empty-row results: y[r] = y_ne[rank[r]] or 0
 __global__ void k_csr5_expand(const int* __restrict__ row_ptr, const int* __restrict__ rank, int m,
-                              const float* __restrict__ y_ne, float* __restrict__ y) {
+                              const float* __restrict__ y_ne, float* __restrict__ y, Gate g) {
+  if (gate_shut(g)) return;
   for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < m;
        r += (long long)gridDim.x * blockDim.x)
     y[r] = row_ptr[r + 1] > row_ptr[r] ? y_ne[rank[r]] : 0.f;
@@ -403,7 +406,7 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
 void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
                  const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
                  unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out,
-                 hipEvent_t ev_begin, hipEvent_t ev_end) {
+                 hipEvent_t ev_begin, hipEvent_t ev_end, Gate g) {
   const bool compressed = c.n_empty > 0;
   float* yk = compressed ? c.y_ne : y;  // results indexed by non-empty row
   *vals_out = yk;
@@ -411,7 +414,7 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
   *n_out = compressed ? c.n_ne : m;
   if (m <= 0) return;
   if (c.ntiles == 0) {
-    if (y_dense_rows) launch_fill_f32(s, y, m, 0.f);
+    if (y_dense_rows) launch_fill_f32(s, y, m, 0.f, g);
     return;
   }
 #define SPMSPV_CSR5(SS, XI, MK, HT)                                                              \
@@ -420,7 +423,7 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
     const int waves = std::max(4, std::min(c.ntiles, resident_blocks(fn, 256) * 4));             \
     hipLaunchKernelGGL((k_csr5<SS, XI, MK, HT>), dim3((waves + 3) / 4), dim3(256), 0, s, c.ccol,  \
                        c.cval, c.flags, c.tinfo, x, xbits, nnz, c.ntiles, yk, marks,             \
-                       c.carry_val, c.carry_row, c.carry_hit);                                   \
+                       c.carry_val, c.carry_row, c.carry_hit, g);                                \
   }
 #define SPMSPV_CSR5_S(SS)                                        \
   if (!x) SPMSPV_CSR5(SS, kXBits5, true, true)                   \
@@ -442,10 +445,10 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
 #undef SPMSPV_CSR5
   if (ev_end) (void)hipEventRecord(ev_end, s);
   hipLaunchKernelGGL(k_csr5_fixup, dim3(std::min(1024, (c.ntiles + 255) / 256)), dim3(256), 0, s,
-                     c.carry_val, c.carry_row, c.carry_hit, c.ntiles, yk, hits ? marks : nullptr);
+                     c.carry_val, c.carry_row, c.carry_hit, c.ntiles, yk, hits ? marks : nullptr, g);
   if (compressed && y_dense_rows)
     hipLaunchKernelGGL(k_csr5_expand, dim3(std::min(8192, (m + 255) / 256)), dim3(256), 0, s, c.row_ptr,
-                       c.rank, m, c.y_ne, y);
+                       c.rank, m, c.y_ne, y, g);
 }
 
 }  // namespace spmspv

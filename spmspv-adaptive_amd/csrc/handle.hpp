@@ -111,6 +111,13 @@ struct spmspv_handle_s {
   bool csr5_ready = false;
   spmspv::Csr5 csr5{};
   long long* batch_stash = nullptr;  // selector word (sel_word) per vector of a batch
+  // speculative dispatch (spmspv_run_auto): the gate launches carry while
+  // the selector's choice is still on its way, and per x-size class
+  // (log2 of xnnz) the last choice + 1 (0: none yet)
+  spmspv::Gate gate{};
+  bool speculate = true;
+  signed char spec_last[33] = {};
+  long long spec_hits = 0, spec_misses = 0;
   int batch_cap = 0;
 
   float* dense_x() { return x_dense_user ? x_dense_user : x_dense_int; }

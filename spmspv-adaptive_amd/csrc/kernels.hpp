@@ -16,6 +16,7 @@ constexpr int kPrepNT = 256, kPrepIPT = 8, kPrepTile = kPrepNT * kPrepIPT;
 constexpr int kLbNT = 256, kLbVT = 7, kLbTile = kLbNT * kLbVT;
 constexpr int kCmpNT = 64;  // bitmap words per compaction tile (one wave: many small tiles keep every CU busy)
 constexpr int kSortNT = 256, kSortIPT = 12, kSortTile = kSortNT * kSortIPT;
+constexpr int kDScanNT = 256, kDScanIPT = 8, kDScanTile = kDScanNT * kDScanIPT;
 constexpr int kRedNT = 256, kRedIPT = 7, kRedTile = kRedNT * kRedIPT;
 constexpr int kRowNT = 256, kRowVT = 7, kRowTile = kRowNT * kRowVT;
 // row-merge items per thread the autotuner may pick (odd: conflict-free)
@@ -31,6 +32,17 @@ struct MatFeat {  // device-resident matrix features (computeMatFeture_serial)
   double max_row, min_row, avg_row, range, std_row, equality, gini, std_trunc;
   double m, n, nnz;
 };
+
+// Device-side switch for work queued before the host knows whether it is
+// wanted: a launch whose gate has a word does nothing unless *word == want
+// (read as the kernel starts; an earlier launch on the stream wrote it).
+struct Gate {
+  const int* word = nullptr;
+  int want = 0;
+};
+__device__ __forceinline__ bool gate_shut(const Gate& g) {
+  return g.word != nullptr && *g.word != g.want;
+}
 
 // --- column path ---------------------------------------------------------
 // Column-length gather + exclusive scan + (bin_len, max, min) + merge-path
@@ -59,23 +71,23 @@ void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, i
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
                       const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,
-                      float* bval);
+                      float* bval, Gate g = {});
 void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval,
                       const int* split, const int* crow, const float* cval, const Ctl* ctl,
                       int xnnz, const int* xnnz_dev, float* acc, unsigned char* marks, int* bkey,
-                      float* bval, long long tiles_hint);
+                      float* bval, long long tiles_hint, Gate g = {});
 // hits (bit array, or mark bytes when marks != null) -> sorted (key, val);
 // clears the hits (and acc entries when reset_acc) so the accumulator
 // returns to all-zero. With keymap, hit k is reported as key keymap[k]
 // (keymap ascending keeps the keys sorted).
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep, const int* keymap = nullptr);
+                    Ep ep, const int* keymap = nullptr, Gate g = {});
 // bitmap of (v != 0) over a dense vector (whole words, no atomics)
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
-void launch_fill_f32(hipStream_t s, float* p, long long n, float v);
+void launch_fill_f32(hipStream_t s, float* p, long long n, float v, Gate g = {});
 // *dst = *src once every earlier launch on s has finished (dst may be host-mapped)
-void launch_signal_int(hipStream_t s, const int* src, long long* dst);
+void launch_signal_int(hipStream_t s, const int* src, long long* dst, Gate g = {});
 void launch_fill_i32(hipStream_t s, int* p, long long n, int v);
 // dense[key[i]] = val[i]; with `encode`, a zero value is stored as -0.0
 // (the support encoding: +0.0 outside the support, nonzero bits inside).
@@ -96,9 +108,8 @@ struct SortBufs {
   float* valA;
   unsigned* keyB;
   float* valB;
-  unsigned* hist;       // 4*256 digit counts (zeroed before each histogram)
-  unsigned* digoff;     // 4*256 (spare)
-  unsigned long long* state;      // per (tile, digit)
+  unsigned* dcount;     // 256 per sort tile: per-tile digit counts, then their scan
+  unsigned long long* dstate;     // look-back words of the digit-count scan
   unsigned long long* red_state;  // per reduce tile
   float* carry_val;
   int* carry_idx;
@@ -153,7 +164,7 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
 void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
                  const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
                  unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out,
-                 hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                 hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, Gate g = {});
 
 // Workgroups of `block` threads of `kernel` resident on the whole device
 // at once (occupancy x CUs), cached per kernel.
@@ -169,7 +180,7 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
                       const int* split, int ntiles, int vt, float* y, unsigned* ybits,
                       float* carry_val, int* carry_row, int* carry_hit,
-                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                      hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, Gate g = {});
 void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int team,
                       float* y, unsigned* ybits);

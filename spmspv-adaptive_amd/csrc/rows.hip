@@ -142,7 +142,8 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
                                                   float* __restrict__ y, unsigned* ybits,
                                                   float* __restrict__ carry_val,
                                                   int* __restrict__ carry_row,
-                                                  int* __restrict__ carry_hit) {
+                                                  int* __restrict__ carry_hit, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int T = NT * VT;
   constexpr int NW = NT / kWave;
   __shared__ float s_p[T];
@@ -336,7 +337,9 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
 
 template <bool HITS>
 __global__ void k_row_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_row,
-                            const int* __restrict__ carry_hit, int ntiles, float* y, unsigned* ybits) {
+                            const int* __restrict__ carry_hit, int ntiles, float* y, unsigned* ybits,
+                            Gate g) {
+  if (gate_shut(g)) return;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
     int r, h;
     float s;
@@ -369,10 +372,10 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                       const float* val, const float* x, const unsigned* xbits, int m, int nnz,
                       const int* split, int ntiles, int vt, float* y, unsigned* ybits,
                       float* carry_val, int* carry_row, int* carry_hit, hipEvent_t ev_begin,
-                      hipEvent_t ev_end) {
+                      hipEvent_t ev_end, Gate g) {
   if (m <= 0 || ntiles <= 0) return;
   if (nnz == 0) {  // every row is empty; no hits
-    launch_fill_f32(s, y, m, 0.f);
+    launch_fill_f32(s, y, m, 0.f, g);
     return;
   }
   // one resident wave of workgroups, each walking many tiles
@@ -382,7 +385,7 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
     hipLaunchKernelGGL((k_row_merge<kRowNT, V, XI, MK, HT>),                                    \
                        dim3(std::min(ntiles, resident_blocks(fn, kRowNT))), dim3(kRowNT), 0, s, \
                        row_ptr, col, val, x, xbits, m, nnz, split, ntiles, y, ybits, carry_val, \
-                       carry_row, carry_hit);                                                   \
+                       carry_row, carry_hit, g);                                                \
   }
 #define SPMSPV_ROW_MERGE_VT(V)                                   \
   if (!x) SPMSPV_ROW_MERGE(V, kXBits, true, true)                \
@@ -404,10 +407,10 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
   const int fgrid = std::min((ntiles + 255) / 256, 1024);
   if (hits)
     hipLaunchKernelGGL(k_row_fixup<true>, dim3(fgrid), dim3(256), 0, s, carry_val, carry_row,
-                       carry_hit, ntiles, y, ybits);
+                       carry_hit, ntiles, y, ybits, g);
   else
     hipLaunchKernelGGL(k_row_fixup<false>, dim3(fgrid), dim3(256), 0, s, carry_val, carry_row,
-                       carry_hit, ntiles, y, ybits);
+                       carry_hit, ntiles, y, ybits, g);
 }
 
 template <int NT, bool MASKED, bool HITS>

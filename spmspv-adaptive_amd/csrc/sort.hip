@@ -5,9 +5,9 @@
 // 1030-1152), written for wave64:
 //  * keys are ranked inside a tile with 8 wave ballots per key (peers with
 //    the same 8-bit digit) and per-wave digit counters in LDS (stable);
-//  * one global histogram pass gives every pass's digit offsets; each pass
-//    is then a single kernel whose tiles find their per-digit global offset
-//    by decoupled look-back (the "onesweep" structure);
+//  * each pass counts digits per tile, scans the digit-major count array
+//    (one small look-back scan over 256 x tiles entries) and scatters, so no
+//    tile waits on a chain of earlier tiles;
 //  * tiles are re-ordered in LDS before the store so that runs of equal
 //    digits leave as contiguous, coalesced writes;
 //  * n <= one tile takes a single-workgroup path that sorts all passes in
@@ -174,51 +174,104 @@ __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const floa
 // ------------------------------------------------------------- histogram
 // Digit counts of every pass into hist[pass][256] (zeroed by the caller's
 // preceding fill); each pass kernel scans its row of it.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_sort_hist(const unsigned* __restrict__ keys,
-                                                  const long long* __restrict__ n_dev, int shift0,
-                                                  int passes, unsigned* hist, int small_cap) {
-  __shared__ unsigned s_h[4 * 256];
-  const long long n = *n_dev;
-  if (n <= small_cap) return;
-  for (int k = threadIdx.x; k < 4 * 256; k += NT) s_h[k] = 0u;
-  __syncthreads();
-  // lanes holding the same digit are found with 8 ballots and counted by
-  // one of them, so skewed keys (R-MAT hub rows) do not serialise on one
-  // LDS counter; the loop bound is wave-uniform for the ballots
-  const uint64_t below = lane() == 0 ? 0ull : (~0ull >> (kWave - lane()));
-  for (long long i0 = (long long)blockIdx.x * NT + (threadIdx.x & ~(kWave - 1)); i0 < n;
-       i0 += (long long)gridDim.x * NT) {
-    const long long i = i0 + lane();
-    const bool ok = i < n;
-    const unsigned k = ok ? keys[i] : 0u;
-    for (int p = 0; p < passes; ++p) {
-      const unsigned dg = (k >> (shift0 + 8 * p)) & 255u;
-      uint64_t peers = __ballot(ok);
+// ------------------------------------------------------------ one pass
+// A pass is three launches, with no look-back chain between tiles:
+//   k_digit_count  per-tile digit counts, digit-major: cnt[d * ntiles + t];
+//   k_digit_scan   exclusive scan of that array in place, so that
+//                  cnt[d * ntiles + t] becomes the global position of the
+//                  first item of digit d in tile t (stable: digit, then tile);
+//   k_sort_scatter ranks the tile again and writes every item there.
+// (With every tile resident at once, a per-digit decoupled look-back
+// spends most of the pass waiting for the prefix to crawl along the tile
+// chain; the scan of 256 x ntiles counts costs a few microseconds.)
+
+// Keys of tile t in the wave-major layout of wave_rank.
+template <int NT, int IPT>
+__device__ __forceinline__ void load_tile_keys(const unsigned* __restrict__ kin, long long n,
+                                               long long base, unsigned (&key)[IPT],
+                                               bool (&valid)[IPT]) {
+  const int w = threadIdx.x / kWave;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const bool bit = (dg >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        peers &= bit ? bb : ~bb;
-      }
-      if (ok && (peers & below) == 0ull) atomicAdd(&s_h[p * 256 + dg], (unsigned)__popcll(peers));
-    }
+  for (int j = 0; j < IPT; ++j) {
+    const long long gi = base + (long long)w * kWave * IPT + j * kWave + lane();
+    valid[j] = gi < n;
+    key[j] = kin[valid[j] ? gi : n - 1];
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < passes * 256; k += NT)
-    if (s_h[k]) atomicAdd(&hist[k], s_h[k]);
 }
 
-// ------------------------------------------------------------ one pass
 template <int NT, int IPT>
-__global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ kin,
-                                                  const float* __restrict__ vin,
-                                                  unsigned* __restrict__ kout,
-                                                  float* __restrict__ vout,
-                                                  const long long* __restrict__ n_dev, int shift,
-                                                  const unsigned* __restrict__ hist,
-                                                  unsigned long long* state, Ctl* ctl, Ep epa,
-                                                  int small_cap) {
+__global__ __launch_bounds__(NT) void k_digit_count(const unsigned* __restrict__ kin,
+                                                    const long long* __restrict__ n_dev, int shift,
+                                                    unsigned* __restrict__ cnt, int small_cap) {
+  constexpr int TILE = NT * IPT;
+  constexpr int NW = NT / kWave;
+  static_assert(NT == 256, "one thread per digit");
+  __shared__ unsigned s_cnt[NW * 256];
+  const long long n = *n_dev;
+  if (n <= small_cap) return;
+  const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
+  const unsigned t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int w = threadIdx.x / kWave;
+  unsigned key[IPT];
+  bool valid[IPT];
+  int rank[IPT];
+  unsigned dig[IPT];
+  load_tile_keys<NT, IPT>(kin, n, (long long)t * TILE, key, valid);
+  wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
+  __syncthreads();
+  unsigned tot = 0;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) tot += s_cnt[q * 256 + threadIdx.x];
+  cnt[(size_t)threadIdx.x * ntiles + t] = tot;
+}
+
+// In-place exclusive scan of cnt[0 .. 256 * ntiles) (ntiles from n and the
+// sort tile), ordered tiles of NT * IPT entries with decoupled look-back.
+template <int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_digit_scan(unsigned* __restrict__ cnt,
+                                                   const long long* __restrict__ n_dev, int sort_tile,
+                                                   int small_cap, unsigned long long* state, Ctl* ctl,
+                                                   Ep epa) {
+  __shared__ unsigned long long s_pre;
+  __shared__ unsigned s_t[NT / kWave];
+  const long long n = *n_dev;
+  if (n <= small_cap) return;
+  const long long E = 256 * ((n + sort_tile - 1) / sort_tile);
+  const unsigned nb = (unsigned)((E + NT * IPT - 1) / (NT * IPT));
+  const unsigned t = blockIdx.x;
+  if (t >= nb) return;
+  const unsigned epoch = ep_of(ctl, epa);
+  const long long base = (long long)t * NT * IPT + (long long)threadIdx.x * IPT;
+  unsigned v[IPT];
+  unsigned sum = 0;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const long long i = base + j;
+    v[j] = cnt[i < E ? i : E - 1];
+    v[j] = i < E ? v[j] : 0u;
+    sum += v[j];
+  }
+  unsigned btot;
+  const unsigned bex = block_excl_sum<NT, unsigned>(sum, s_t, &btot);
+  const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
+  unsigned run = (unsigned)pre + bex;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const long long i = base + j;
+    if (i < E) cnt[i] = run;
+    run += v[j];
+  }
+}
+
+template <int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned* __restrict__ kin,
+                                                     const float* __restrict__ vin,
+                                                     unsigned* __restrict__ kout,
+                                                     float* __restrict__ vout,
+                                                     const long long* __restrict__ n_dev, int shift,
+                                                     const unsigned* __restrict__ pos0,
+                                                     int small_cap) {
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
   static_assert(NT == 256, "one thread per digit");
@@ -233,93 +286,45 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const unsigned* __restrict__ k
   const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
   const unsigned t = blockIdx.x;
   if (t >= ntiles) return;
-  const unsigned ep = ep_of(ctl, epa);
   const int w = threadIdx.x / kWave;
-  {
-    // global start of every digit: exclusive scan of this pass's histogram
-    unsigned dtot0;
-    const unsigned doff = block_excl_sum<NT, unsigned>(hist[threadIdx.x], s_t, &dtot0);
-    const long long base = (long long)t * TILE;
-    unsigned key[IPT];
-    float val[IPT];
-    bool valid[IPT];
-    int rank[IPT];
-    unsigned dig[IPT];
+  const int d = threadIdx.x;
+  const long long base = (long long)t * TILE;
+  // this tile's global start of digit d, from the scan (issued with the keys)
+  const unsigned gb = pos0[(size_t)d * ntiles + t];
+  unsigned key[IPT];
+  float val[IPT];
+  bool valid[IPT];
+  int rank[IPT];
+  unsigned dig[IPT];
+  load_tile_keys<NT, IPT>(kin, n, base, key, valid);
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const long long gi = base + w * kWave * IPT + j * kWave + lane();
-      valid[j] = gi < n;
-      key[j] = valid[j] ? kin[gi] : 0u;
-      val[j] = valid[j] ? vin[gi] : 0.f;
+  for (int j = 0; j < IPT; ++j) {
+    const long long gi = base + (long long)w * kWave * IPT + j * kWave + lane();
+    val[j] = vin[valid[j] ? gi : n - 1];
+  }
+  wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
+  __syncthreads();
+  const unsigned tot = digit_wave_offsets<NW>(s_cnt);
+  unsigned dtot;
+  s_dstart[d] = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
+  s_gbase[d] = gb;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if (valid[j]) {
+      const unsigned p = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
+      s_k[p] = key[j];
+      s_v[p] = val[j];
     }
-    wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
-    __syncthreads();
-    const int d = threadIdx.x;
-    const unsigned tot = digit_wave_offsets<NW>(s_cnt);
-    unsigned dtot;
-    const unsigned dex = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
-    s_dstart[d] = dex;
-    // per-digit decoupled look-back
-    unsigned long long* st = state + (size_t)t * 256 + d;
-    st_relaxed(st, pack_state(t == 0 ? 2u : 1u, ep, tot));
-    unsigned long long excl = 0;
-    if (t > 0) {
-      // look back LB predecessors at a time: their state words are loaded
-      // together, so a chain of aggregates costs one memory latency per LB
-      // tiles instead of one per tile
-      constexpr int LB = 8;
-      long long k = (long long)t - 1;
-      while (true) {
-        unsigned long long sv[LB];
-        unsigned fv[LB];
-#pragma unroll
-        for (int q = 0; q < LB; ++q) {
-          const long long idx = k - q;
-          sv[q] = idx >= 0 ? ld_relaxed(state + (size_t)idx * 256 + d) : pack_state(2u, ep, 0ull);
-        }
-#pragma unroll
-        for (int q = 0; q < LB; ++q) {
-          fv[q] = (unsigned)(sv[q] >> 62);
-          if (((sv[q] >> 38) & ((1u << kEpochBits) - 1)) != ep) fv[q] = 0;
-          while (fv[q] == 0) {  // not published yet (rare: older tiles run first)
-            __builtin_amdgcn_s_sleep(1);
-            sv[q] = ld_relaxed(state + (size_t)(k - q) * 256 + d);
-            fv[q] = (unsigned)(sv[q] >> 62);
-            if (((sv[q] >> 38) & ((1u << kEpochBits) - 1)) != ep) fv[q] = 0;
-          }
-        }
-        bool done = false;
-#pragma unroll
-        for (int q = 0; q < LB; ++q) {
-          if (!done) {
-            excl += sv[q] & kValMask;
-            if (fv[q] == 2) done = true;
-          }
-        }
-        if (done) break;
-        k -= LB;
-      }
-      st_relaxed(st, pack_state(2u, ep, excl + tot));
-    }
-    s_gbase[d] = doff + (unsigned)excl;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      if (valid[j]) {
-        const unsigned pos = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
-        s_k[pos] = key[j];
-        s_v[pos] = val[j];
-      }
-    }
-    __syncthreads();
-    const int cnt = (int)min((long long)TILE, n - base);
-    for (int i = threadIdx.x; i < cnt; i += NT) {
-      const unsigned k = s_k[i];
-      const unsigned dd = (k >> shift) & 255u;
-      const unsigned g = s_gbase[dd] + (i - s_dstart[dd]);
-      kout[g] = k;
-      vout[g] = s_v[i];
-    }
+  }
+  __syncthreads();
+  const int cnt = (int)min((long long)TILE, n - base);
+  for (int i = threadIdx.x; i < cnt; i += NT) {
+    const unsigned k = s_k[i];
+    const unsigned dd = (k >> shift) & 255u;
+    const unsigned g = s_gbase[dd] + (i - s_dstart[dd]);
+    kout[g] = k;
+    vout[g] = s_v[i];
   }
 }
 
@@ -474,6 +479,22 @@ static int tiles_grid(long long n_hint, long long n_cap, int tile) {
   return (int)std::max<long long>(1, (n + tile - 1) / tile);
 }
 
+// One counted pass (count, scan, scatter) over *n_dev items; tiles beyond
+// the device-side count exit at once. small_cap: passes do nothing when
+// n <= small_cap (the single-workgroup path handled them).
+static void launch_digit_pass(hipStream_t s, const SortBufs& b, const unsigned* kin, const float* vin,
+                              unsigned* kout, float* vout, int shift, const long long* n_dev, Ctl* ctl,
+                              long long n_hint, long long n_cap, int small_cap, Ep e) {
+  const int pgrid = tiles_grid(n_hint, n_cap, kSortTile);
+  const int sgrid = (int)std::max<long long>(1, (256LL * pgrid + kDScanTile - 1) / kDScanTile);
+  hipLaunchKernelGGL((k_digit_count<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, n_dev,
+                     shift, b.dcount, small_cap);
+  hipLaunchKernelGGL((k_digit_scan<kDScanNT, kDScanIPT>), dim3(sgrid), dim3(kDScanNT), 0, s, b.dcount,
+                     n_dev, kSortTile, small_cap, b.dstate, ctl, e);
+  hipLaunchKernelGGL((k_sort_scatter<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, vin, kout,
+                     vout, n_dev, shift, b.dcount, small_cap);
+}
+
 // Sorts (keyA, valA)[*n_dev] by key < range; returns 0 when the result is
 // in A, 1 when in B. ep.dev >= 0 (captured graphs) uses device epochs
 // ep.dev + pass; otherwise ep.v + pass.
@@ -489,34 +510,23 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
                        final_b ? b.valB : b.valA);
   }
   if (small_known) return final_b;
-  launch_fill_i32(s, reinterpret_cast<int*>(b.hist), 4 * 256, 0);
-  const int hgrid = std::min(1024, tiles_grid(n_hint, n_cap, 4096));
-  hipLaunchKernelGGL(k_sort_hist<256>, dim3(hgrid), dim3(256), 0, s, b.keyA, n_dev, 0, passes, b.hist,
-                     kSmallSortCap);
-  const int pgrid = tiles_grid(n_hint, n_cap, kSortTile);
   for (int p = 0; p < passes; ++p) {
     const unsigned* kin = (p & 1) ? b.keyB : b.keyA;
     const float* vin = (p & 1) ? b.valB : b.valA;
     unsigned* kout = (p & 1) ? b.keyA : b.keyB;
     float* vout = (p & 1) ? b.valA : b.valB;
     const Ep e{ep.v + (unsigned)p, ep.dev < 0 ? -1 : ep.dev + p};
-    hipLaunchKernelGGL((k_sort_pass<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, vin,
-                       kout, vout, n_dev, 8 * p, b.hist + p * 256, b.state, ctl, e, kSmallSortCap);
+    launch_digit_pass(s, b, kin, vin, kout, vout, 8 * p, n_dev, ctl, n_hint, n_cap, kSmallSortCap, e);
   }
   return final_b;
 }
 
 // One stable partition pass of (keyA, valA) into (keyB, valB) on the 8-bit
-// digit (key >> shift) & 255; the digit starts are the exclusive scan of
-// b.hist[0..255].
+// digit (key >> shift) & 255. Afterwards b.dcount[d * ntiles] (ntiles =
+// ceil(n / kSortTile)) is the start of digit d in the output.
 void launch_partition_pass(hipStream_t s, const SortBufs& b, int shift, const long long* n_dev, Ctl* ctl,
                            long long n_hint, long long n_cap, Ep ep) {
-  launch_fill_i32(s, reinterpret_cast<int*>(b.hist), 256, 0);
-  const int hgrid = std::min(1024, tiles_grid(n_hint, n_cap, 4096));
-  hipLaunchKernelGGL(k_sort_hist<256>, dim3(hgrid), dim3(256), 0, s, b.keyA, n_dev, shift, 1, b.hist, -1);
-  hipLaunchKernelGGL((k_sort_pass<kSortNT, kSortIPT>), dim3(tiles_grid(n_hint, n_cap, kSortTile)),
-                     dim3(kSortNT), 0, s, b.keyA, b.valA, b.keyB, b.valB, n_dev, shift, b.hist, b.state,
-                     ctl, ep, -1);
+  launch_digit_pass(s, b, b.keyA, b.valA, b.keyB, b.valB, shift, n_dev, ctl, n_hint, n_cap, -1, ep);
 }
 
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,

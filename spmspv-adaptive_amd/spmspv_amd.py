@@ -107,6 +107,8 @@ _SIGS = {
     "spmspv_spmv": [_vp, _f, _vp],
     "spmspv_autotune2": [_vp, _i, _i, _vp],
     "spmspv_time_main_kernel": [_vp, _i],
+    "spmspv_set_speculation": [_vp, _i],
+    "spmspv_speculation_stats": [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "spmspv_main_kernel_ms": [_vp, _vp],
     "spmspv_autotune": [_vp, _i, _vp],
     "spmspv_selector_set_weights": [_vp, _vp, _i],
@@ -415,6 +417,16 @@ class SpmspvHandle:
                                          vval.ptr if vval else None, yoff.ptr, ykey.ptr, yval.ptr,
                                          ynnz.ptr, _np(used)))
         return used[:nvec]
+
+    def set_speculation(self, on=True):
+        """Speculative queueing of run_auto (on by default)."""
+        return self._c(lib().spmspv_set_speculation(self.h, 1 if on else 0))
+
+    def speculation_stats(self):
+        """(right guesses, wrong guesses) of run_auto's speculation so far."""
+        hits, misses = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        self._c(lib().spmspv_speculation_stats(self.h, ctypes.byref(hits), ctypes.byref(misses)))
+        return hits.value, misses.value
 
     def time_main_kernel(self, on=True):
         return self._c(lib().spmspv_time_main_kernel(self.h, 1 if on else 0))

@@ -1,0 +1,116 @@
+"""Speculative dispatch of run_auto: the slot guessed from the previous
+choice for the same x-size class is queued before the selector's choice
+reaches the host, every launch gated on the device-side choice. Results
+must not depend on whether the guess was right: each call is checked
+against the FP64 oracle (index sets exact, values within
+1e-5 * sum|a_ij x_j|) and the counts of right and wrong guesses against the
+sequence of slots forced through the fixed-slot selector mode."""
+import numpy as np
+import pytest
+
+import oracle
+import spmspv_amd as sa
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def check(m, cp, ri, cv, key, val, yk, yv, tag):
+    ek, ev, eb = oracle.spmspv_f64(m, cp, ri, cv, key, val)
+    assert yk.size == ek.size, f"{tag}: ynnz {yk.size} != {ek.size}"
+    np.testing.assert_array_equal(yk, ek, err_msg=tag)
+    err = np.abs(yv.astype(np.float64) - ev)
+    assert np.all(err <= TOL * eb + 1e-30), f"{tag}: values out of bound"
+
+
+@pytest.fixture(scope="module")
+def rmat12_csr5(gpu):
+    rp, col, val = sa.gen_rmat_csr(12, 16, seed=41)
+    n = 1 << 12
+    M = sa.Matrix(n, n, rp, col, val)
+    M.A.set_spmv_type(1)
+    return M, oracle.csr_to_csc(n, n, rp, col, val)
+
+
+def test_forced_slot_sequence(rmat12_csr5):
+    """Fixed-slot mode forces the choice; the guess is the previous choice
+    of the class, so the sequence gives a known mix of right and wrong
+    guesses, each of which must leave the result intact (sparse output and
+    the dense output of the same call)."""
+    M, (cp, ri, cv) = rmat12_csr5
+    A = M.A
+    n = M.n
+    key, val = sa.gen_sparse_vector(n, 300, 4242)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    # prepare the row formats once, outside the sequence (speculation only
+    # guesses slots whose launches need no set-up)
+    A.set_sparse_x(dk, dv, key.size)
+    A.run(7, y_key=yk, y_val=yv)
+    A.set_speculation(True)
+    h0, m0 = A.speculation_stats()
+    seq = [2, 2, 7, 7, 0, 5, 5, 2, 3, 3, 2]
+    try:
+        for i, slot in enumerate(seq):
+            A.selector_mode(2, slot)
+            A.set_sparse_x(dk, dv, key.size)
+            yd = sa.DeviceArray(n, np.float32) if i % 3 == 0 else None
+            ynnz, used = A.run_auto(y_dense=yd, y_key=yk, y_val=yv)
+            print(f"call {i}: forced {slot} used {used} ynnz {ynnz} stats {A.speculation_stats()}",
+                  flush=True)
+            assert used == slot
+            check(M.m, cp, ri, cv, key, val, yk.to_host(ynnz), yv.to_host(ynnz), f"call {i} slot {slot}")
+            if yd is not None:
+                dense = yd.to_host()
+                ek, ev, eb = oracle.spmspv_f64(M.m, cp, ri, cv, key, val)
+                ref = np.zeros(n)
+                ref[ek] = ev
+                bound = np.zeros(n)
+                bound[ek] = eb
+                assert np.all(np.abs(dense - ref) <= TOL * bound + 1e-30), f"call {i}: dense output"
+    finally:
+        A.selector_mode(1)
+    h1, m1 = A.speculation_stats()
+    # (previous choice -> forced slot): none->2 no guess, 2->2 right, 2->7
+    # wrong, 7->7 right, 7->0 wrong, 0->5 wrong, 5->5 right, 5->2 wrong,
+    # 2->3 wrong, 3->3 and 3->2 no guess (a sort slot is never guessed)
+    assert (h1 - h0, m1 - m0) == (3, 5)
+
+
+def test_speculation_on_off_same_results(rmat12_csr5):
+    """MLP selector over an alternating density sequence, with speculation
+    on and off: the same slots and the same outputs."""
+    import os
+    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "spmspv-adaptive_amd",
+                             "selector_weights.npy")).astype(np.float32)
+    M, (cp, ri, cv) = rmat12_csr5
+    A = M.A
+    n = M.n
+    A.selector_set_weights(w)
+    A.selector_mode(0)
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    xs = []
+    for i, d in enumerate([1e-3, 0.5, 1e-3, 0.05, 0.5, 1.0, 1e-3, 1.0, 0.05]):
+        k = max(1, int(round(d * n)))
+        key, val = sa.gen_sparse_vector(n, k, 900 + i)
+        xs.append((key, val, sa.DeviceArray.from_host(key, np.int32), sa.DeviceArray.from_host(val, np.float32)))
+    try:
+        runs = {}
+        for on in (False, True, True):
+            A.set_speculation(on)
+            got = []
+            for i, (key, val, dk, dv) in enumerate(xs):
+                A.set_sparse_x(dk, dv, key.size)
+                ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+                k_h, v_h = yk.to_host(ynnz), yv.to_host(ynnz)
+                check(M.m, cp, ri, cv, key, val, k_h, v_h, f"speculation {on} x {i} slot {used}")
+                got.append(used)
+            runs.setdefault(on, []).append(got)
+            print(f"speculation {on}: slots {got} stats {A.speculation_stats()}", flush=True)
+        assert runs[False][0] == runs[True][0] == runs[True][1]
+    finally:
+        A.selector_mode(1)
+        A.set_speculation(True)
