@@ -106,11 +106,13 @@ static size_t carve(spmspv_handle_t h, char* base) {
   int* rch = c.take<int>(row_tiles);
   long long* ndev = c.take<long long>(4);
   float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
+  long long* selp = c.take<long long>(3 * kSelMaxGrid);
   if (base) {
     h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
     h->sort_words = dscan_tiles;
     h->red_words = red_tiles;
     h->ctl = ctl;
+    h->sel_part = selp;
     h->mat = mat;
     h->mat_partials = mp;
     h->ptr = ptr;
@@ -1022,8 +1024,13 @@ int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]) {
 // at once) and the chosen slot is queued as usual. Only slots whose launches
 // need nothing from the host about x (no bin_len readback) are guessed: the
 // atomic column slots and the load-balanced row slots.
+static bool is_column_slot(int kid) {
+  return kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_NAIVE_SORT ||
+         kid == SPMSPV_K_COL_LB_ATOMIC || kid == SPMSPV_K_COL_LB_SORT || kid == SPMSPV_K_COL_BUCKET;
+}
+
 static bool speculable(spmspv_handle_t h, int kid) {
-  if (kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_LB_ATOMIC) return h->prep_valid;
+  if (kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_LB_ATOMIC) return true;
   if (kid == SPMSPV_K_ROW_LB || kid == SPMSPV_K_SPMV_LB)
     return h->csr_row && (h->spmv_type == 1 ? h->csr5_ready : h->row_ready);
   return false;
@@ -1041,10 +1048,25 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
                    h->sig_dev + 1};
   __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
-  if ((rc = run_prep(h, true, &sel))) return rc;
   const int cls = size_class(h->xnnz);
   const int guess = h->spec_last[cls] - 1;
   const bool spec = h->speculate && guess >= 0 && speculable(h, guess);
+  if (spec && !is_column_slot(guess)) {
+    // a row slot needs none of the column scan: features only, with x's
+    // support encoding (which the row slots read) scattered on the way
+    float* enc = nullptr;
+    if (!h->enc_valid) {
+      launch_fill_f32(h->stream, h->x_enc, h->n, 0.f);
+      enc = h->x_enc;
+    }
+    launch_select_feat(h->stream, h->csc_colptr, h->x_key, h->x_val, h->n, h->xnnz, h->sel_part, h->ctl,
+                       sel, enc);
+    h->launches += 2;
+    if ((rc = check_launch(h, "selector"))) return rc;
+    h->enc_valid = true;
+  } else if ((rc = run_prep(h, true, &sel))) {
+    return rc;
+  }
   if (spec) {
     struct Ungate {
       spmspv_handle_t h;

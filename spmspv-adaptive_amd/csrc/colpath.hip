@@ -190,6 +190,143 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     *sel.stash = sel_word((long long)pre + btot, sel.mat ? ctl->kernel_choice : -1);
 }
 
+// Features-only selector launch: bin_len and the largest and smallest
+// column length over x's support (a plain reduction: no scan, no tile
+// chain), then the selector in the last workgroup. For calls whose slot is
+// a row slot, which needs none of the column scan. The last workgroup waits
+// for the partials of the others: the grid is at most kSelMaxGrid
+// workgroups, all of them resident, and it is dispatched last.
+template <int NT, int KPT>
+__global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colptr,
+                                                    const int* __restrict__ xkey,
+                                                    const float* __restrict__ xval, int ncols, int xnnz,
+                                                    long long* part, Ctl* ctl, SelectorArgs sel,
+                                                    float* __restrict__ enc) {
+  __shared__ __align__(16) float s_w[kNWeights];
+  __shared__ MatFeat s_mf;
+  __shared__ long long s_l[NT / kWave];
+  __shared__ int s_i[NT / kWave];
+  const bool last = blockIdx.x == gridDim.x - 1;
+  const bool selects = last && sel.mat != nullptr;
+  constexpr int kWPT = (kNWeights + NT - 1) / NT;
+  constexpr int kMF = sizeof(MatFeat) / sizeof(double);
+  float wr[kWPT];
+  double mfr = 0.0;
+  if (selects) {
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        wr[i] = idx < kNWeights ? sel.weights[idx] : 0.f;
+      }
+    }
+    if (threadIdx.x < kMF) mfr = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
+  }
+  const int w = threadIdx.x / kWave;
+  long long sum = 0;
+  int mx = 0, mn = INT_MAX;
+  for (long long t0 = (long long)blockIdx.x * NT * KPT; t0 < xnnz; t0 += (long long)gridDim.x * NT * KPT) {
+    const long long b0 = t0 + (long long)w * kWave * KPT + lane();
+    int c[KPT], beg[KPT], end[KPT], ok[KPT];
+    float xv[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const long long i = b0 + (long long)j * kWave;
+      const long long ic = i < xnnz ? i : (long long)xnnz - 1;
+      c[j] = xkey[ic];
+      xv[j] = enc ? xval[ic] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      ok[j] = (b0 + (long long)j * kWave < xnnz && (unsigned)c[j] < (unsigned)ncols) ? 1 : 0;
+      const int ci = c[j] & -ok[j];
+      beg[j] = colptr[ci];
+      end[j] = colptr[ci + ok[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      if (ok[j]) {
+        const int len = end[j] - beg[j];
+        sum += len;
+        mx = max(mx, len);
+        mn = min(mn, len);
+      }
+    }
+    if (enc) {
+      // support encoding (as launch_scatter with encode): -0.0 marks a zero
+      // value inside the support
+#pragma unroll
+      for (int j = 0; j < KPT; ++j)
+        if (ok[j]) enc[c[j]] = xv[j] == 0.f ? -0.f : xv[j];
+    }
+  }
+  sum = block_sum<NT, long long>(sum, s_l);
+  mx = block_max<NT, int>(mx, s_i);
+  mn = block_min<NT, int>(mn, s_i);
+  if (!last) {
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&part[blockIdx.x], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part[kSelMaxGrid + blockIdx.x], (long long)mx, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part[2 * kSelMaxGrid + blockIdx.x], (long long)mn, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // partials in place before the count
+      __hip_atomic_fetch_add(&ctl->done[kSlotSelect], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    while (__hip_atomic_load(&ctl->done[kSlotSelect], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+           gridDim.x - 1)
+      __builtin_amdgcn_s_sleep(2);
+    __hip_atomic_store(&ctl->done[kSlotSelect], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  long long osum = 0;
+  int omx = 0, omn = INT_MAX;
+  for (unsigned u = threadIdx.x; u + 1 < gridDim.x; u += NT) {
+    osum += __hip_atomic_load(&part[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    omx = max(omx, (int)__hip_atomic_load(&part[kSelMaxGrid + u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT));
+    omn = min(omn, (int)__hip_atomic_load(&part[2 * kSelMaxGrid + u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT));
+  }
+  sum += block_sum<NT, long long>(osum, s_l);
+  mx = max(mx, block_max<NT, int>(omx, s_i));
+  mn = min(mn, block_min<NT, int>(omn, s_i));
+  if (threadIdx.x == 0) {
+    ctl->bin_len = sum;
+    ctl->max_len = mx;
+    ctl->min_len = mn;
+    ctl->xnnz = xnnz;
+  }
+  if (!selects) return;
+  if (sel.weights) {
+#pragma unroll
+    for (int i = 0; i < kWPT; ++i) {
+      const int idx = threadIdx.x + i * NT;
+      if (idx < kNWeights) s_w[wlds_index(idx)] = wr[i];
+    }
+  }
+  if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
+  __syncthreads();
+  SelectorArgs sl = sel;
+  if (sel.weights) sl.weights = s_w;
+  sl.mat = &s_mf;
+  select_in_wave(ctl, sl, (double)xnnz, (double)sum, (double)mx);
+  if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(sum, ctl->kernel_choice);
+}
+
+void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
+                        int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
+                        float* enc) {
+  constexpr int kPer = 256 * 16;
+  const int grid = (int)std::max<long long>(
+      1, std::min<long long>(kSelMaxGrid, ((long long)xnnz + kPer - 1) / kPer));
+  hipLaunchKernelGGL((k_select_feat<256, 16>), dim3(grid), dim3(256), 0, s, colptr, xkey, xval, ncols,
+                     xnnz, part, ctl, sel, enc);
+}
+
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
                  const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
                  int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep) {
@@ -293,13 +430,29 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
     const long long b0 = d0 - a0;
     const long long b1 = d1 - a1;
     const int sa = a0 > 0 ? a0 - 1 : 0;
-    const int nseg = a1 - sa + 1;
-    for (int k = threadIdx.x; k < nseg; k += NT) {
-      const int i = sa + k;
-      s_ptr[k] = ptr[i];
-      if (i < xnnz) {
-        s_beg[k] = xbeg[i];
-        s_xv[k] = xval[i];
+    const int nseg = a1 - sa + 1;  // <= T + 2
+    {
+      // every load of the segment table issued at once (clamped indices)
+      constexpr int SEGPT = (T + 2 + NT - 1) / NT;
+      int pv[SEGPT], bv[SEGPT];
+      float xv[SEGPT];
+#pragma unroll
+      for (int u = 0; u < SEGPT; ++u) {
+        const int k = threadIdx.x + u * NT;
+        const int i = sa + (k < nseg ? k : 0);
+        const int ic = i < xnnz ? i : (xnnz > 0 ? xnnz - 1 : 0);
+        pv[u] = ptr[i];
+        bv[u] = xbeg[ic];
+        xv[u] = xval[ic];
+      }
+#pragma unroll
+      for (int u = 0; u < SEGPT; ++u) {
+        const int k = threadIdx.x + u * NT;
+        if (k < nseg) {
+          s_ptr[k] = pv[u];
+          s_beg[k] = bv[u];
+          s_xv[k] = xv[u];
+        }
       }
     }
     __syncthreads();
@@ -326,19 +479,40 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
       }
     }
     __syncthreads();
-    const int np = (int)(b1 - b0);
-    for (int q = threadIdx.x; q < np; q += NT) {
-      const int sg = s_seg[q];
-      const long long p = b0 + q;
-      const int idx = s_beg[sg] + (int)(p - s_ptr[sg]);
-      const int r = crow[idx];
-      const float v = cval[idx] * s_xv[sg];
-      if (MODE == 0) {
-        atomic_add_f32(acc + r, v);
-        marks[r] = 1;
-      } else {
-        bkey[p] = r;
-        bval[p] = v;
+    const int np = (int)(b1 - b0);  // <= T
+    if (np > 0) {
+      // the VT products of a thread: indices from LDS, then all their
+      // matrix loads issued together
+      int idx[VT];
+      float xs[VT];
+#pragma unroll
+      for (int k = 0; k < VT; ++k) {
+        const int q0 = threadIdx.x + k * NT;
+        const int q = q0 < np ? q0 : 0;
+        const int sg = s_seg[q];
+        idx[k] = s_beg[sg] + (int)(b0 + q - s_ptr[sg]);
+        xs[k] = s_xv[sg];
+      }
+      int r[VT];
+      float cv[VT];
+#pragma unroll
+      for (int k = 0; k < VT; ++k) {
+        r[k] = crow[idx[k]];
+        cv[k] = cval[idx[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < VT; ++k) {
+        const int q = threadIdx.x + k * NT;
+        if (q < np) {
+          const float v = cv[k] * xs[k];
+          if (MODE == 0) {
+            atomic_add_f32(acc + r[k], v);
+            marks[r[k]] = 1;
+          } else {
+            bkey[b0 + q] = r[k];
+            bval[b0 + q] = v;
+          }
+        }
       }
     }
     __syncthreads();

@@ -50,6 +50,7 @@ struct spmspv_handle_s {
   void* arena = nullptr;
   size_t arena_bytes = 0;
   spmspv::Ctl* ctl = nullptr;
+  long long* sel_part = nullptr;  // partials of the features-only selector launch
   spmspv::MatFeat* mat = nullptr;
   double* mat_partials = nullptr;
   int* ptr = nullptr;        // n+1 (x scan); doubles as the degree histogram

@@ -26,7 +26,8 @@ constexpr int kPersistGrid = 1024;        // workgroups of a persistent launch
 
 // Ticket slots in Ctl (one per persistent kernel kind).
 enum Slot { kSlotPrep = 0, kSlotCompact, kSlotHist, kSlotSort, kSlotReduce, kSlotMat, kSlotBfs,
-            kSlotBatch, kSlotBucket };
+            kSlotBatch, kSlotBucket, kSlotSelect };
+constexpr int kSelMaxGrid = 512;  // workgroups of the features-only selector launch
 
 struct MatFeat {  // device-resident matrix features (computeMatFeture_serial)
   double max_row, min_row, avg_row, range, std_row, equality, gini, std_trunc;
@@ -68,6 +69,13 @@ __host__ __device__ inline int sel_word_choice(long long w) {
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
                  const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
                  int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep);
+// Features-only selector: ctl->{bin_len, max_len, min_len, xnnz} of x and,
+// with sel.mat set, the selector's choice (and the selector word in
+// sel.stash). part: 3 * kSelMaxGrid partials. No column scan. With enc set
+// (already all +0.0), x's support encoding is scattered into it as well.
+void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
+                        int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
+                        float* enc);
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
                       const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,

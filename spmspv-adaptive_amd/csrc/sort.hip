@@ -433,9 +433,25 @@ __global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys
   {
     const long long base = (long long)t * TILE;
     const int cnt = (int)min((long long)TILE, n - base);
-    for (int i = threadIdx.x; i < cnt; i += NT) {
-      s_k[i] = keys[base + i];
-      s_v[i] = vals[base + i];
+    {
+      // all of a thread's loads at once (clamped), then into LDS
+      unsigned kk[IPT];
+      float vv[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int i = threadIdx.x + j * NT;
+        const long long gi = base + (i < cnt ? i : 0);
+        kk[j] = keys[gi];
+        vv[j] = vals[gi];
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int i = threadIdx.x + j * NT;
+        if (i < cnt) {
+          s_k[i] = kk[j];
+          s_v[i] = vv[j];
+        }
+      }
     }
     const unsigned prev = base > 0 ? keys[base - 1] : 0u;
     __syncthreads();
