@@ -346,18 +346,24 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       float* bv = h->sb.valA;
       if (lb)
         launch_lb_gather(s, 1, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
-                         h->xnnz, nullptr, nullptr, nullptr, bk, bv, lb_tiles_hint(h));
+                         h->xnnz, nullptr, nullptr, nullptr, bk, bv, lb_tiles_hint(h), h->gate);
       else
         launch_naive_col(s, 1, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, h->ptr,
-                         h->n, h->xnnz, nullptr, h->team_col, nullptr, nullptr, bk, bv);
+                         h->n, h->xnnz, nullptr, h->team_col, nullptr, nullptr, bk, bv, h->gate);
       int* ok = y_key ? y_key : h->yk_tmp;
       float* ov = y_val ? y_val : h->yv_tmp;
-      if ((rc = ensure_binlen_host(h))) return rc;
-      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, h->binlen_host, h->nnz, next_ep(h, 5));
+      // queued before the selector's word arrives (speculation): every grid
+      // is sized from the device-side count instead of the host's bin_len
+      long long hint = -1;
+      if (!h->gate.word) {
+        if ((rc = ensure_binlen_host(h))) return rc;
+        hint = h->binlen_host;
+      }
+      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, hint, h->nnz, next_ep(h, 5), h->gate);
       h->launches += 6;
       if (y_dense) {
-        launch_fill_f32(s, y_dense, m, 0.f);
-        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
+        launch_fill_f32(s, y_dense, m, 0.f, h->gate);
+        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense, false, h->gate);
       }
       return check_launch(h, "column sort kernel");
     }
@@ -1022,15 +1028,18 @@ int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]) {
 // selector, every launch carrying a gate on the device-side choice. A right
 // guess costs nothing more; a wrong one costs the gated launches (each exits
 // at once) and the chosen slot is queued as usual. Only slots whose launches
-// need nothing from the host about x (no bin_len readback) are guessed: the
-// atomic column slots and the load-balanced row slots.
+// need nothing from the host about x are guessed: the atomic and sort column
+// slots (the sort then sizes its grids from the device-side bin_len) and the
+// load-balanced row slots.
 static bool is_column_slot(int kid) {
   return kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_NAIVE_SORT ||
          kid == SPMSPV_K_COL_LB_ATOMIC || kid == SPMSPV_K_COL_LB_SORT || kid == SPMSPV_K_COL_BUCKET;
 }
 
 static bool speculable(spmspv_handle_t h, int kid) {
-  if (kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_LB_ATOMIC) return true;
+  if (kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_LB_ATOMIC ||
+      kid == SPMSPV_K_COL_NAIVE_SORT || kid == SPMSPV_K_COL_LB_SORT)
+    return true;
   if (kid == SPMSPV_K_ROW_LB || kid == SPMSPV_K_SPMV_LB)
     return h->csr_row && (h->spmv_type == 1 ? h->csr5_ready : h->row_ready);
   return false;

@@ -66,7 +66,12 @@ int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int
       return rc;
     SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
                      h->batch_stash + v};
-    if ((rc = run_prep(h, false, &sel))) return rc;
+    // features only: the chosen pipeline runs the column scan itself when
+    // its slot needs one (the scan buffers hold one vector at a time)
+    launch_select_feat(h->stream, h->csc_colptr, h->x_key, h->x_val, h->n, h->xnnz, h->sel_part, h->ctl,
+                       sel, nullptr);
+    h->launches++;
+    HIP_TRY(h, hipGetLastError());
   }
   std::vector<long long> st((size_t)nvec);
   HIP_TRY(h, hipMemcpyAsync(st.data(), h->batch_stash, sizeof(long long) * nvec,

@@ -694,7 +694,8 @@ void launch_fill_i32(hipStream_t s, int* p, long long n, int v) {
 }
 
 __global__ void k_scatter(const int* __restrict__ key, const float* __restrict__ val, int nnz_arg,
-                          const int* nnz_dev, int len, float* __restrict__ dense, int encode) {
+                          const int* nnz_dev, int len, float* __restrict__ dense, int encode, Gate g) {
+  if (gate_shut(g)) return;
   const int nnz = nnz_dev ? *nnz_dev : nnz_arg;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
     const int k = key[i];
@@ -703,11 +704,11 @@ __global__ void k_scatter(const int* __restrict__ key, const float* __restrict__
   }
 }
 void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, const int* nnz_dev,
-                    int len, float* dense, bool encode) {
+                    int len, float* dense, bool encode, Gate g) {
   int grid = nnz_dev ? 2048 : std::max(1, std::min(8192, (nnz + 255) / 256));
   if (!nnz_dev && nnz <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, key, val, nnz, nnz_dev, len, dense,
-                     encode ? 1 : 0);
+                     encode ? 1 : 0, g);
 }
 
 __global__ void k_encode_x(const float* __restrict__ x, const unsigned* __restrict__ bits, int n,

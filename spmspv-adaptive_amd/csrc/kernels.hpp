@@ -100,7 +100,7 @@ void launch_fill_i32(hipStream_t s, int* p, long long n, int v);
 // dense[key[i]] = val[i]; with `encode`, a zero value is stored as -0.0
 // (the support encoding: +0.0 outside the support, nonzero bits inside).
 void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, const int* nnz_dev,
-                    int len, float* dense, bool encode = false);
+                    int len, float* dense, bool encode = false, Gate g = {});
 // enc[c] = bit c of bits ? (x[c] == 0 ? -0.0 : x[c]) : +0.0
 void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc);
 void launch_set_bits(hipStream_t s, const int* key, int nnz, const int* nnz_dev, int len,
@@ -125,11 +125,11 @@ struct SortBufs {
 // n_hint: item count when the host knows it (exact grids), else -1 and the
 // grids cover n_cap. ep: first of the 5 epochs the sort + reduce use.
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
-                        long long binlen_hint, long long n_cap, Ep ep);
+                        long long binlen_hint, long long n_cap, Ep ep, Gate g = {});
 // Sorts (keyA, valA)[*n_dev] by key < range; returns 0 if the result is in
 // A, 1 if in B.
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, long long n_cap, bool skip_small, Ep ep);
+                      long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g = {});
 
 // Bucket slot: partition the gathered bins by the top 8 row bits, then one
 // workgroup per (bucket, 8192-row window) accumulates in LDS and writes its

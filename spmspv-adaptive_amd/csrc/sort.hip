@@ -202,7 +202,9 @@ __device__ __forceinline__ void load_tile_keys(const unsigned* __restrict__ kin,
 template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_digit_count(const unsigned* __restrict__ kin,
                                                     const long long* __restrict__ n_dev, int shift,
-                                                    unsigned* __restrict__ cnt, int small_cap) {
+                                                    unsigned* __restrict__ cnt, int small_cap,
+                                                    Gate g) {
+  if (gate_shut(g)) return;
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
   static_assert(NT == 256, "one thread per digit");
@@ -210,20 +212,22 @@ __global__ __launch_bounds__(NT) void k_digit_count(const unsigned* __restrict__
   const long long n = *n_dev;
   if (n <= small_cap) return;
   const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
-  const unsigned t = blockIdx.x;
-  if (t >= ntiles) return;
   const int w = threadIdx.x / kWave;
-  unsigned key[IPT];
-  bool valid[IPT];
-  int rank[IPT];
-  unsigned dig[IPT];
-  load_tile_keys<NT, IPT>(kin, n, (long long)t * TILE, key, valid);
-  wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
-  __syncthreads();
-  unsigned tot = 0;
+  // tiles are independent: any grid covers them
+  for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    unsigned key[IPT];
+    bool valid[IPT];
+    int rank[IPT];
+    unsigned dig[IPT];
+    load_tile_keys<NT, IPT>(kin, n, (long long)t * TILE, key, valid);
+    wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
+    __syncthreads();
+    unsigned tot = 0;
 #pragma unroll
-  for (int q = 0; q < NW; ++q) tot += s_cnt[q * 256 + threadIdx.x];
-  cnt[(size_t)threadIdx.x * ntiles + t] = tot;
+    for (int q = 0; q < NW; ++q) tot += s_cnt[q * 256 + threadIdx.x];
+    cnt[(size_t)threadIdx.x * ntiles + t] = tot;
+    __syncthreads();
+  }
 }
 
 // In-place exclusive scan of cnt[0 .. 256 * ntiles) (ntiles from n and the
@@ -232,7 +236,8 @@ template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_digit_scan(unsigned* __restrict__ cnt,
                                                    const long long* __restrict__ n_dev, int sort_tile,
                                                    int small_cap, unsigned long long* state, Ctl* ctl,
-                                                   Ep epa) {
+                                                   Ep epa, Gate g) {
+  if (gate_shut(g)) return;
   __shared__ unsigned long long s_pre;
   __shared__ unsigned s_t[NT / kWave];
   const long long n = *n_dev;
@@ -271,7 +276,8 @@ __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned* __restrict_
                                                      float* __restrict__ vout,
                                                      const long long* __restrict__ n_dev, int shift,
                                                      const unsigned* __restrict__ pos0,
-                                                     int small_cap) {
+                                                     int small_cap, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
   static_assert(NT == 256, "one thread per digit");
@@ -284,47 +290,49 @@ __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned* __restrict_
   const long long n = *n_dev;
   if (n <= small_cap) return;
   const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
-  const unsigned t = blockIdx.x;
-  if (t >= ntiles) return;
   const int w = threadIdx.x / kWave;
   const int d = threadIdx.x;
-  const long long base = (long long)t * TILE;
-  // this tile's global start of digit d, from the scan (issued with the keys)
-  const unsigned gb = pos0[(size_t)d * ntiles + t];
-  unsigned key[IPT];
-  float val[IPT];
-  bool valid[IPT];
-  int rank[IPT];
-  unsigned dig[IPT];
-  load_tile_keys<NT, IPT>(kin, n, base, key, valid);
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const long long gi = base + (long long)w * kWave * IPT + j * kWave + lane();
-    val[j] = vin[valid[j] ? gi : n - 1];
-  }
-  wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
-  __syncthreads();
-  const unsigned tot = digit_wave_offsets<NW>(s_cnt);
-  unsigned dtot;
-  s_dstart[d] = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
-  s_gbase[d] = gb;
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    if (valid[j]) {
-      const unsigned p = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
-      s_k[p] = key[j];
-      s_v[p] = val[j];
+  // tiles are independent: any grid covers them
+  for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long base = (long long)t * TILE;
+    // this tile's global start of digit d, from the scan (issued with the keys)
+    const unsigned gb = pos0[(size_t)d * ntiles + t];
+    unsigned key[IPT];
+    float val[IPT];
+    bool valid[IPT];
+    int rank[IPT];
+    unsigned dig[IPT];
+    load_tile_keys<NT, IPT>(kin, n, base, key, valid);
+  #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const long long gi = base + (long long)w * kWave * IPT + j * kWave + lane();
+      val[j] = vin[valid[j] ? gi : n - 1];
     }
-  }
-  __syncthreads();
-  const int cnt = (int)min((long long)TILE, n - base);
-  for (int i = threadIdx.x; i < cnt; i += NT) {
-    const unsigned k = s_k[i];
-    const unsigned dd = (k >> shift) & 255u;
-    const unsigned g = s_gbase[dd] + (i - s_dstart[dd]);
-    kout[g] = k;
-    vout[g] = s_v[i];
+    wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
+    __syncthreads();
+    const unsigned tot = digit_wave_offsets<NW>(s_cnt);
+    unsigned dtot;
+    s_dstart[d] = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
+    s_gbase[d] = gb;
+    __syncthreads();
+  #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (valid[j]) {
+        const unsigned p = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
+        s_k[p] = key[j];
+        s_v[p] = val[j];
+      }
+    }
+    __syncthreads();
+    const int cnt = (int)min((long long)TILE, n - base);
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      const unsigned k = s_k[i];
+      const unsigned dd = (k >> shift) & 255u;
+      const unsigned g = s_gbase[dd] + (i - s_dstart[dd]);
+      kout[g] = k;
+      vout[g] = s_v[i];
+    }
+    __syncthreads();
   }
 }
 
@@ -336,7 +344,8 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
                                                    const float* __restrict__ vin,
                                                    const long long* __restrict__ n_dev, int passes,
                                                    int mode, Ctl* ctl, int* ykey, float* yval,
-                                                   unsigned* kout, float* vout) {
+                                                   unsigned* kout, float* vout, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
   __shared__ unsigned s_k[TILE];
@@ -415,7 +424,8 @@ __global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys
                                                const long long* __restrict__ n_dev, Ctl* ctl,
                                                int* ykey, float* yval, float* carry_val,
                                                int* carry_idx, unsigned long long* state, Ep epa,
-                                               int small_cap) {
+                                               int small_cap, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
   __shared__ unsigned s_k[TILE];
@@ -469,7 +479,8 @@ __global__ __launch_bounds__(NT) void k_reduce(const unsigned* __restrict__ keys
 
 __global__ void k_carry_fixup(const float* __restrict__ carry_val, const int* __restrict__ carry_idx,
                               const long long* __restrict__ n_dev, int tile, int small_cap,
-                              float* yval) {
+                              float* yval, Gate g) {
+  if (gate_shut(g)) return;
   const long long n = *n_dev;
   if (n <= small_cap) return;
   const long long ntiles = (n + tile - 1) / tile;
@@ -500,22 +511,23 @@ static int tiles_grid(long long n_hint, long long n_cap, int tile) {
 // n <= small_cap (the single-workgroup path handled them).
 static void launch_digit_pass(hipStream_t s, const SortBufs& b, const unsigned* kin, const float* vin,
                               unsigned* kout, float* vout, int shift, const long long* n_dev, Ctl* ctl,
-                              long long n_hint, long long n_cap, int small_cap, Ep e) {
-  const int pgrid = tiles_grid(n_hint, n_cap, kSortTile);
-  const int sgrid = (int)std::max<long long>(1, (256LL * pgrid + kDScanTile - 1) / kDScanTile);
+                              long long n_hint, long long n_cap, int small_cap, Ep e, Gate g) {
+  const int tiles = tiles_grid(n_hint, n_cap, kSortTile);
+  const int pgrid = std::min(tiles, 2048);  // count / scatter loop over their tiles
+  const int sgrid = (int)std::max<long long>(1, (256LL * tiles + kDScanTile - 1) / kDScanTile);
   hipLaunchKernelGGL((k_digit_count<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, n_dev,
-                     shift, b.dcount, small_cap);
+                     shift, b.dcount, small_cap, g);
   hipLaunchKernelGGL((k_digit_scan<kDScanNT, kDScanIPT>), dim3(sgrid), dim3(kDScanNT), 0, s, b.dcount,
-                     n_dev, kSortTile, small_cap, b.dstate, ctl, e);
+                     n_dev, kSortTile, small_cap, b.dstate, ctl, e, g);
   hipLaunchKernelGGL((k_sort_scatter<kSortNT, kSortIPT>), dim3(pgrid), dim3(kSortNT), 0, s, kin, vin, kout,
-                     vout, n_dev, shift, b.dcount, small_cap);
+                     vout, n_dev, shift, b.dcount, small_cap, g);
 }
 
 // Sorts (keyA, valA)[*n_dev] by key < range; returns 0 when the result is
 // in A, 1 when in B. ep.dev >= 0 (captured graphs) uses device epochs
 // ep.dev + pass; otherwise ep.v + pass.
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, long long n_cap, bool skip_small, Ep ep) {
+                      long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g) {
   const int passes = key_passes(range);
   const int final_b = passes & 1;
   const bool small_known = n_hint >= 0 && n_hint <= kSmallSortCap;
@@ -523,7 +535,7 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
   if (!big_known && !skip_small) {
     hipLaunchKernelGGL((k_sort_small<kSortNT, kSortIPT>), dim3(1), dim3(kSortNT), 0, s, b.keyA,
                        b.valA, n_dev, passes, 1, ctl, nullptr, nullptr, final_b ? b.keyB : b.keyA,
-                       final_b ? b.valB : b.valA);
+                       final_b ? b.valB : b.valA, g);
   }
   if (small_known) return final_b;
   for (int p = 0; p < passes; ++p) {
@@ -532,7 +544,7 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
     unsigned* kout = (p & 1) ? b.keyA : b.keyB;
     float* vout = (p & 1) ? b.valA : b.valB;
     const Ep e{ep.v + (unsigned)p, ep.dev < 0 ? -1 : ep.dev + p};
-    launch_digit_pass(s, b, kin, vin, kout, vout, 8 * p, n_dev, ctl, n_hint, n_cap, kSmallSortCap, e);
+    launch_digit_pass(s, b, kin, vin, kout, vout, 8 * p, n_dev, ctl, n_hint, n_cap, kSmallSortCap, e, g);
   }
   return final_b;
 }
@@ -542,30 +554,30 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
 // ceil(n / kSortTile)) is the start of digit d in the output.
 void launch_partition_pass(hipStream_t s, const SortBufs& b, int shift, const long long* n_dev, Ctl* ctl,
                            long long n_hint, long long n_cap, Ep ep) {
-  launch_digit_pass(s, b, b.keyA, b.valA, b.keyB, b.valB, shift, n_dev, ctl, n_hint, n_cap, -1, ep);
+  launch_digit_pass(s, b, b.keyA, b.valA, b.keyB, b.valB, shift, n_dev, ctl, n_hint, n_cap, -1, ep, Gate{});
 }
 
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
-                        long long binlen_hint, long long n_cap, Ep ep) {
+                        long long binlen_hint, long long n_cap, Ep ep, Gate g) {
   const long long* n_dev = &ctl->bin_len;
   const int passes = key_passes(m);
   const bool small_known = binlen_hint >= 0 && binlen_hint <= kSmallSortCap;
   const bool big_known = binlen_hint > kSmallSortCap;
   if (!big_known) {
     hipLaunchKernelGGL((k_sort_small<kSortNT, kSortIPT>), dim3(1), dim3(kSortNT), 0, s, b.keyA,
-                       b.valA, n_dev, passes, 0, ctl, ykey, yval, nullptr, nullptr);
+                       b.valA, n_dev, passes, 0, ctl, ykey, yval, nullptr, nullptr, g);
   }
   if (small_known) return;
-  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, n_cap, true, ep);
+  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, n_cap, true, ep, g);
   const unsigned* kf = fb ? b.keyB : b.keyA;
   const float* vf = fb ? b.valB : b.valA;
   const Ep er{ep.v + 4u, ep.dev < 0 ? -1 : ep.dev + 4};
   hipLaunchKernelGGL((k_reduce<kRedNT, kRedIPT>), dim3(tiles_grid(binlen_hint, n_cap, kRedTile)),
                      dim3(kRedNT), 0, s, kf, vf, n_dev, ctl, ykey, yval, b.carry_val, b.carry_idx,
-                     b.red_state, er, kSmallSortCap);
+                     b.red_state, er, kSmallSortCap, g);
   const int fgrid = std::min(2048, std::max(1, tiles_grid(binlen_hint, n_cap, kRedTile) / 256 + 1));
   hipLaunchKernelGGL(k_carry_fixup, dim3(fgrid), dim3(256), 0, s, b.carry_val, b.carry_idx, n_dev,
-                     kRedTile, kSmallSortCap, yval);
+                     kRedTile, kSmallSortCap, yval, g);
 }
 
 }  // namespace spmspv

@@ -51,7 +51,7 @@ def test_forced_slot_sequence(rmat12_csr5):
     A.run(7, y_key=yk, y_val=yv)
     A.set_speculation(True)
     h0, m0 = A.speculation_stats()
-    seq = [2, 2, 7, 7, 0, 5, 5, 2, 3, 3, 2]
+    seq = [2, 2, 7, 7, 0, 5, 5, 2, 3, 3, 1, 1, 2]
     try:
         for i, slot in enumerate(seq):
             A.selector_mode(2, slot)
@@ -75,8 +75,8 @@ def test_forced_slot_sequence(rmat12_csr5):
     h1, m1 = A.speculation_stats()
     # (previous choice -> forced slot): none->2 no guess, 2->2 right, 2->7
     # wrong, 7->7 right, 7->0 wrong, 0->5 wrong, 5->5 right, 5->2 wrong,
-    # 2->3 wrong, 3->3 and 3->2 no guess (a sort slot is never guessed)
-    assert (h1 - h0, m1 - m0) == (3, 5)
+    # 2->3 wrong, 3->3 right, 3->1 wrong, 1->1 right, 1->2 wrong
+    assert (h1 - h0, m1 - m0) == (5, 7)
 
 
 def test_speculation_on_off_same_results(rmat12_csr5):
