@@ -1031,9 +1031,10 @@ int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]) {
 // need nothing from the host about x are guessed: the atomic and sort column
 // slots (the sort then sizes its grids from the device-side bin_len) and the
 // load-balanced row slots.
-static bool is_column_slot(int kid) {
-  return kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_NAIVE_SORT ||
-         kid == SPMSPV_K_COL_LB_ATOMIC || kid == SPMSPV_K_COL_LB_SORT || kid == SPMSPV_K_COL_BUCKET;
+// slots whose launches read the column scan of x (ptr / xbeg / split)
+static bool needs_scan(int kid) {
+  return kid == SPMSPV_K_COL_NAIVE_SORT || kid == SPMSPV_K_COL_LB_ATOMIC ||
+         kid == SPMSPV_K_COL_LB_SORT || kid == SPMSPV_K_COL_BUCKET;
 }
 
 static bool speculable(spmspv_handle_t h, int kid) {
@@ -1060,19 +1061,20 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   const int cls = size_class(h->xnnz);
   const int guess = h->spec_last[cls] - 1;
   const bool spec = h->speculate && guess >= 0 && speculable(h, guess);
-  if (spec && !is_column_slot(guess)) {
-    // a row slot needs none of the column scan: features only, with x's
-    // support encoding (which the row slots read) scattered on the way
+  if (spec && !needs_scan(guess)) {
+    // the guessed slot needs none of the column scan: features only, with
+    // x's support encoding (which the row slots read) scattered on the way
+    const bool row = guess >= SPMSPV_K_ROW_NAIVE && guess <= SPMSPV_K_SPMV_LB;
     float* enc = nullptr;
-    if (!h->enc_valid) {
+    if (row && !h->enc_valid) {
       launch_fill_f32(h->stream, h->x_enc, h->n, 0.f);
       enc = h->x_enc;
     }
     launch_select_feat(h->stream, h->csc_colptr, h->x_key, h->x_val, h->n, h->xnnz, h->sel_part, h->ctl,
                        sel, enc);
-    h->launches += 2;
+    h->launches += enc ? 2 : 1;
     if ((rc = check_launch(h, "selector"))) return rc;
-    h->enc_valid = true;
+    if (enc) h->enc_valid = true;
   } else if ((rc = run_prep(h, true, &sel))) {
     return rc;
   }
