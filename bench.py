@@ -33,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "spmspv-adaptive_amd"))
 HBM_PEAK_GBS = 8000.0
 DENSITIES = [1e-4, 1e-3, 1e-2, 0.1, 0.5, 1.0]
 WEIGHTS = os.path.join(ROOT, "spmspv-adaptive_amd", "selector_weights.npy")
+BFS_WEIGHTS = os.path.join(ROOT, "spmspv-adaptive_amd", "bfs_selector_weights.npy")
 AUTO_SLOT = 99  # --only-slot value that times the selector-dispatched call
 
 
@@ -95,6 +96,8 @@ def run_bfs(sa, args, A, rp, col, n):
             B.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
         else:
             B.selector_mode(1)
+    if os.path.exists(BFS_WEIGHTS):  # the BFS levels' own selector
+        B.bfs_selector_set_weights(np.load(BFS_WEIGHTS).astype(np.float32))
     deg = np.diff(brp)
     cand = np.nonzero(deg > 0)[0]
     rng = np.random.default_rng(0x5EED0003)

@@ -262,6 +262,18 @@ int spmspv_mlp_forward(const float* params, const float* X, int B, float* logits
  * seconds, reached}. */
 int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
                double stats[4]);
+/* No counterpart in the reference: spmspv_bfs that also records, for each of
+ * the first max_levels levels, the level's device time in ms (level_ms,
+ * host) and the 16 selector features of its frontier (level_feat, host,
+ * 16 floats per level) -- the labels of the BFS selector's training set.
+ * Uses the handle's timing events. */
+int spmspv_bfs_trace(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
+                     double stats[4], int max_levels, float* level_ms, float* level_feat);
+/* No counterpart in the reference: selector weights for the BFS levels
+ * (trained on BFS level timings, whose boolean kernels rank differently from
+ * the SpMSpV slots); without them the BFS driver uses the SpMSpV selector's
+ * weights. */
+int spmspv_bfs_selector_set_weights(spmspv_handle_t h, const float* w, int count);
 
 /* ---- batched multi-vector SpMSpV (no reference counterpart) ------------- */
 /* nvec sparse vectors in CSR-of-vectors form: d_vptr[nvec+1] (device),

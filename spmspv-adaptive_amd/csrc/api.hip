@@ -106,6 +106,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
   int* rch = c.take<int>(row_tiles);
   long long* ndev = c.take<long long>(4);
   float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
+  float* wb = c.take<float>(SPMSPV_MLP_NWEIGHTS);
   long long* selp = c.take<long long>(3 * kSelMaxGrid);
   if (base) {
     h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
@@ -137,6 +138,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
     h->row_carry_hit = rch;
     h->n_dev = ndev;
     h->weights_dev = w;
+    h->bfs_weights_dev = wb;
   }
   return c.off;
 }
@@ -622,6 +624,7 @@ int spmspv_dealloc_pre_buffer(spmspv_handle_t h) {
   h->mat_ready = false;
   h->prep_valid = false;
   h->have_weights = false;
+  h->have_bfs_weights = false;
   return SPMSPV_SUCCESS;
 }
 
@@ -980,6 +983,16 @@ int spmspv_selector_set_weights(spmspv_handle_t h, const float* w, int count) {
   HIP_TRY(h, hipMemcpy(h->weights_dev, w, sizeof(float) * count, hipMemcpyHostToDevice));
   h->have_weights = true;
   h->sel_mode = 0;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_bfs_selector_set_weights(spmspv_handle_t h, const float* w, int count) {
+  if (!h || !w || count != SPMSPV_MLP_NWEIGHTS)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "bfs selector weights: expected SPMSPV_MLP_NWEIGHTS floats");
+  int rc = ensure_arena(h);
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpy(h->bfs_weights_dev, w, sizeof(float) * count, hipMemcpyHostToDevice));
+  h->have_bfs_weights = true;
   return SPMSPV_SUCCESS;
 }
 

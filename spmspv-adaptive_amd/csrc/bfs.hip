@@ -42,12 +42,14 @@ struct BfsState {
   hipStream_t own_stream = nullptr;
   int* levels_bound = nullptr;              // levels array the graphs were captured with
   int bound_mode = -9, bound_fixed = -9;    // selector settings captured in the graphs
+  const float* bound_weights = nullptr;
   hipGraphExec_t graph[8][2] = {};
 };
 
 namespace {
 
 constexpr int kBfsGrid = 2048;
+constexpr int kFeatPin = 32;  // h_pin words 32..47: the next level's selector features
 
 __device__ __forceinline__ int wave_append(int* counter, bool pred) {
   const uint64_t b = __ballot(pred);
@@ -428,6 +430,7 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
               h->tile_mm, h->ctl, &sel, Ep{0, 0});
   (void)hipMemcpyAsync(h->h_pin + 16, b->cnt + nxt, sizeof(int), hipMemcpyDeviceToHost, s);
   (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
+  (void)hipMemcpyAsync(h->h_pin + kFeatPin, h->ctl->feat, sizeof(float) * 16, hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace
@@ -446,8 +449,11 @@ int spmspv_bfs_release(spmspv_handle_t h) {
   return SPMSPV_SUCCESS;
 }
 
-int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
-               double stats[4]) {
+// The level loop; with max_levels > 0 it also records, per level (the
+// frontier of level L is the one reached at level L-1), the device time of
+// the level and the selector features of its frontier.
+static int bfs_run(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
+                   double stats[4], int max_levels, float* level_ms, float* level_feat) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   if (src < 0 || src >= h->n || !d_levels || policy < -1 || policy >= SPMSPV_NUM_SLOTS)
     return set_error(h, SPMSPV_INVALID_ARGUMENT, "bfs: bad source, levels or policy");
@@ -460,7 +466,11 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
   BfsState* b = h->bfs;
   hipStream_t s = h->stream ? h->stream : b->own_stream;
   const int smode = policy >= 0 ? 2 : h->sel_mode, sfixed = policy >= 0 ? policy : h->sel_fixed;
-  if (use_graphs && (b->levels_bound != d_levels || b->bound_mode != smode || b->bound_fixed != sfixed)) {
+  // the BFS levels' own weights when set (trained on BFS levels), else the
+  // SpMSpV selector's
+  const float* wsel = h->have_bfs_weights ? h->bfs_weights_dev : (h->have_weights ? h->weights_dev : nullptr);
+  if (use_graphs && (b->levels_bound != d_levels || b->bound_mode != smode || b->bound_fixed != sfixed ||
+                     b->bound_weights != wsel)) {
     for (auto& g : b->graph)
       for (auto& e : g)
         if (e) {
@@ -470,12 +480,13 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
     b->levels_bound = d_levels;
     b->bound_mode = smode;
     b->bound_fixed = sfixed;
+    b->bound_weights = wsel;
   }
   // level L's kernels are enqueued on s, which the handle's own launches
   // (prep, sort) must share
   hipStream_t saved = h->stream;
   h->stream = s;
-  SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, smode, sfixed};
+  SelectorArgs sel{wsel, h->mat, smode, sfixed};
   const auto t0 = std::chrono::steady_clock::now();
   hipLaunchKernelGGL(k_bfs_init, dim3(std::min(4096, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,
                      h->n, b->visited, b->fbits[0], b->fbits[1], b->front[0], b->cnt, b->reached, src);
@@ -483,13 +494,20 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
               h->scan_state, h->tile_mm, h->ctl, &sel, next_ep(h, 1));
   (void)hipMemcpyAsync(h->h_pin + 16, b->cnt, sizeof(int), hipMemcpyDeviceToHost, s);
   (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
+  (void)hipMemcpyAsync(h->h_pin + kFeatPin, h->ctl->feat, sizeof(float) * 16, hipMemcpyDeviceToHost, s);
   hipError_t e = hipStreamSynchronize(s);
+  const bool trace = max_levels > 0 && level_ms && level_feat;
   int cur = 0, nlev = 1;
   int slot_hist[8] = {0};
   while (e == hipSuccess && h->h_pin[16] > 0) {
     int slot = policy >= 0 ? policy : h->h_pin[17];
     if (slot < 0 || slot >= SPMSPV_NUM_SLOTS) slot = SPMSPV_K_COL_LB_ATOMIC;
     slot_hist[slot]++;
+    const int lev = nlev - 1;  // index of the level about to run
+    if (trace && lev < max_levels) {
+      std::memcpy(level_feat + 16 * lev, h->h_pin + kFeatPin, sizeof(float) * 16);
+      (void)hipEventRecord(h->ev0, s);
+    }
     if (use_graphs) {
       hipGraphExec_t& ge = b->graph[slot][cur];
       if (!ge) {
@@ -511,7 +529,13 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
       enqueue_level(h, s, slot, cur, d_levels, sel);
       e = hipGetLastError();
     }
+    if (trace && lev < max_levels && e == hipSuccess) e = hipEventRecord(h->ev1, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (trace && lev < max_levels && e == hipSuccess) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+      level_ms[lev] = ms;
+    }
     cur ^= 1;
     ++nlev;
     if (nlev > h->n + 1) break;  // cannot happen on a correct run; bounds the loop
@@ -541,6 +565,18 @@ int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_gr
     stats[3] = (double)red[0];
   }
   return SPMSPV_SUCCESS;
+}
+
+int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
+               double stats[4]) {
+  return bfs_run(h, src, d_levels, policy, use_graphs, stats, 0, nullptr, nullptr);
+}
+
+int spmspv_bfs_trace(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
+                     double stats[4], int max_levels, float* level_ms, float* level_feat) {
+  if (max_levels < 1 || !level_ms || !level_feat)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT, "bfs trace: need max_levels >= 1 and both arrays");
+  return bfs_run(h, src, d_levels, policy, use_graphs, stats, max_levels, level_ms, level_feat);
 }
 
 }  // extern "C"

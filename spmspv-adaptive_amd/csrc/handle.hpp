@@ -77,6 +77,7 @@ struct spmspv_handle_s {
   int* row_carry_hit = nullptr;
   long long* n_dev = nullptr;
   float* weights_dev = nullptr;
+  float* bfs_weights_dev = nullptr;  // the BFS levels' own selector weights
   float* flush_buf = nullptr;
   size_t flush_bytes = 0;
   bool mat_ready = false;
@@ -84,6 +85,7 @@ struct spmspv_handle_s {
 
   // selector
   bool have_weights = false;
+  bool have_bfs_weights = false;
   int sel_mode = 1;
   int sel_fixed = 0;
   int team_col = 16;

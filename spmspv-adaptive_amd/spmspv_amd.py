@@ -108,6 +108,8 @@ _SIGS = {
     "spmspv_autotune2": [_vp, _i, _i, _vp],
     "spmspv_time_main_kernel": [_vp, _i],
     "spmspv_set_speculation": [_vp, _i],
+    "spmspv_bfs_trace": [_vp, _i, _vp, _i, _i, _vp, _i, _vp, _vp],
+    "spmspv_bfs_selector_set_weights": [_vp, _vp, _i],
     "spmspv_speculation_stats": [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "spmspv_main_kernel_ms": [_vp, _vp],
     "spmspv_autotune": [_vp, _i, _vp],
@@ -406,6 +408,21 @@ class SpmspvHandle:
         st = np.zeros(4, np.float64)
         self._c(lib().spmspv_bfs(self.h, src, levels.ptr, policy, use_graphs, _np(st)))
         return st
+
+    def bfs_trace(self, src, levels, policy=-1, use_graphs=0, max_levels=128):
+        """bfs that also returns, per level, its device time (ms) and the
+        selector features of its frontier: (stats, level_ms, level_feat)."""
+        st = np.zeros(4, np.float64)
+        ms = np.zeros(max_levels, np.float32)
+        ft = np.zeros((max_levels, MLP_F), np.float32)
+        self._c(lib().spmspv_bfs_trace(self.h, src, levels.ptr, policy, use_graphs, _np(st), max_levels,
+                                       _np(ms), _np(ft)))
+        nl = min(int(st[0]), max_levels)
+        return st, ms[:nl], ft[:nl]
+
+    def bfs_selector_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        return self._c(lib().spmspv_bfs_selector_set_weights(self.h, _np(w), w.size))
 
     def run_batched(self, vptr, vkey, vval, yoff, ykey, yval, ynnz):
         """Batched SpMSpV over the vectors vptr/vkey/vval (device arrays);

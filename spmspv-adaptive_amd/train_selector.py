@@ -16,6 +16,10 @@ The reference labels each (matrix, x) pair with the fastest of its kernels
           4 x densities per matrix, log-uniform in [1e-5, 1]; matrix seeds
           0..1999 train, 2000..2399 held out. Sizes are capped at 2^25
           nonzeros per matrix so that a shard fits one GPU session.
+  bfs-corpus  BFS level labels: symmetric R-MAT / uniform graphs, a few
+          sources each, every level timed under the 8 fixed slots
+          (spmspv_bfs_trace), features of each level's frontier; trained
+          with `train` into the BFS levels' own weights.
   train   trains the MLP 16->64->64->8 on the GPU with the library's own
           kernels (spmspv_mlp_train) on soft targets
           p_k ~ exp(-(t_k / t_best - 1) / tau), then reports held-out
@@ -144,6 +148,107 @@ def cmd_corpus(args):
     print(f"corpus {len(feats)} samples -> {args.out} in {time.time() - t0:.1f}s", flush=True)
 
 
+def bfs_graph_spec(seed):
+    rng = np.random.default_rng(seed + 2_000_003)
+    if seed % 2 == 0:
+        scale = int(rng.integers(14, 22))
+        a = float(rng.uniform(0.45, 0.65))
+        ef = int(np.exp(rng.uniform(np.log(4), np.log(32))))
+        ef = max(1, min(ef, NNZ_CAP // (2 << scale)))
+        return dict(kind="rmat", scale=scale, a=a, ef=ef)
+    n = int(2 ** rng.uniform(14, 21))
+    mean = float(np.exp(rng.uniform(np.log(2), np.log(16))))
+    return dict(kind="uniform", n=n, mean=min(mean, NNZ_CAP / (2 * n)))
+
+
+def build_bfs_graph(spec, seed):
+    """Symmetric pattern (BFS follows edges both ways)."""
+    if spec["kind"] == "rmat":
+        a = spec["a"]
+        b = c = (0.95 - a) / 2
+        rp, col, val = sa.gen_rmat_csr(spec["scale"], spec["ef"], a, b, c, seed)
+        return 1 << spec["scale"], rp, col, val
+    import scipy.sparse as sp
+    n = spec["n"]
+    rp, col, val = sa.gen_random_csr(n, n, spec["mean"], seed)
+    A = sp.csr_matrix((np.ones(col.size, np.float32), col, rp), shape=(n, n))
+    A = (A + A.T).tocsr()
+    A.setdiag(0)
+    A.eliminate_zeros()
+    A.sort_indices()
+    return n, A.indptr.astype(np.int32), A.indices.astype(np.int32), np.ones(A.nnz, np.float32)
+
+
+def rule_from_features(f):
+    """The rule fallback (tree_select in csrc/selector.hpp) on recorded
+    features: the inputs it reads are recovered from their log2 forms."""
+    n = 2.0 ** f[1]
+    nnz = 2.0 ** f[2] - 1.0
+    avg = 2.0 ** f[4] - 1.0
+    std = f[5] * (avg + 1e-9)
+    xnnz = 2.0 ** f[8] - 1.0
+    bl = 2.0 ** f[9] - 1.0
+    mx = 2.0 ** f[10] - 1.0
+    density = xnnz / max(n, 1.0)
+    skewed = std > 2.0 * avg + 1.0 or mx > 8.0 * (avg + 1.0)
+    if density >= 0.3:
+        return 7 if skewed else 6
+    if density >= 0.05:
+        return 5 if skewed else 4
+    if bl <= 3072:
+        return 1
+    return 2 if skewed else 0
+
+
+def cmd_bfs_corpus(args):
+    """BFS level labels: every level of a BFS from a source runs the same
+    frontier whatever slot runs it (levels are unique), so the 8 fixed-slot
+    runs give the 8 times of each level; its features are those the driver's
+    selector computes for that frontier."""
+    feats, times, rule, meta = [], [], [], []
+    t0 = time.time()
+
+    def save():
+        np.savez(args.out, feats=np.array(feats, np.float32).reshape(-1, sa.MLP_F),
+                 times=np.array(times, np.float64).reshape(-1, sa.NUM_SLOTS),
+                 rule=np.array(rule, np.int32), meta=np.array(meta, np.int64).reshape(-1, 4))
+
+    for seed in range(args.start, args.start + args.count):
+        if args.budget_s and time.time() - t0 > args.budget_s:
+            print(f"bfs corpus stopped at seed {seed} (time budget)", flush=True)
+            break
+        spec = bfs_graph_spec(seed)
+        n, rp, col, val = build_bfs_graph(spec, seed)
+        if rp[-1] == 0:
+            continue
+        M = sa.Matrix(n, n, rp, col, val)
+        A = M.A
+        lev = sa.DeviceArray(n, np.int32)
+        deg = np.diff(rp)
+        cand = np.nonzero(deg > 0)[0]
+        rng = np.random.default_rng(seed + 11)
+        for src in rng.choice(cand, size=min(args.sources, cand.size), replace=False):
+            per = []
+            f0 = None
+            for pol in range(sa.NUM_SLOTS):
+                A.bfs_trace(int(src), lev, policy=pol, use_graphs=1)  # graphs captured, caches warm
+                st, ms, ft = A.bfs_trace(int(src), lev, policy=pol, use_graphs=1)
+                per.append(ms)
+                if f0 is None:
+                    f0 = ft
+            nl = min(len(p) for p in per)
+            for L in range(nl):
+                feats.append(f0[L])
+                times.append([float(per[p][L]) for p in range(sa.NUM_SLOTS)])
+                rule.append(rule_from_features(f0[L]))
+                meta.append([seed, n, int(rp[-1]), int(src)])
+        del M, A
+        save()
+        print(f"bfs corpus seed {seed} {spec} {len(feats)} levels {time.time() - t0:.1f}s", flush=True)
+    save()
+    print(f"bfs corpus {len(feats)} levels -> {args.out} in {time.time() - t0:.1f}s", flush=True)
+
+
 def soft_targets(times, tau):
     rel = times / times.min(axis=1, keepdims=True) - 1.0
     z = np.exp(-rel / tau)
@@ -226,6 +331,13 @@ def main():
                    help="time the slots with the default widths instead of the autotuned ones")
     c.add_argument("--budget-s", type=float, default=0.0,
                    help="stop starting new matrices after this many seconds (results are kept)")
+    bc = sub.add_parser("bfs-corpus")
+    bc.add_argument("--start", type=int, default=0)
+    bc.add_argument("--count", type=int, default=50)
+    bc.add_argument("--sources", type=int, default=4)
+    bc.add_argument("--out", required=True)
+    bc.add_argument("--budget-s", type=float, default=0.0,
+                    help="stop starting new graphs after this many seconds (results are kept)")
     t = sub.add_parser("train")
     t.add_argument("--corpus", nargs="+", required=True)
     t.add_argument("--holdout-from", type=int, default=2000)
@@ -237,7 +349,7 @@ def main():
     t.add_argument("--out", default=os.path.join(HERE, "selector_weights.npy"))
     t.add_argument("--report", default=None)
     args = ap.parse_args()
-    {"corpus": cmd_corpus, "train": cmd_train}[args.cmd](args)
+    {"corpus": cmd_corpus, "bfs-corpus": cmd_bfs_corpus, "train": cmd_train}[args.cmd](args)
 
 
 if __name__ == "__main__":

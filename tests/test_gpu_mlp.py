@@ -81,15 +81,19 @@ def test_mlp_training_fits_a_rule(gpu):
     assert acc > 0.95, acc
 
 
-def test_trained_selector_gain_on_heldout(gpu):
-    """The committed network, scored on the committed held-out corpus
-    (matrix seeds 2000-2399, every (matrix, x) timed on all eight slots):
-    its geomean regret against the fastest slot is at most 5% and below
+@pytest.mark.parametrize("which", ["spmspv", "bfs"])
+def test_trained_selector_gain_on_heldout(gpu, which):
+    """The committed networks, scored on their committed held-out corpora
+    (SpMSpV: matrix seeds 2000-2399, every (matrix, x) timed on all eight
+    slots; BFS: graph seeds 300-359, every BFS level timed under all eight
+    slots): geomean regret against the fastest slot at most 5% and below
     both the rule fallback's and the best single fixed slot's."""
     import os
     here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "spmspv-adaptive_amd")
-    w = np.load(os.path.join(here, "selector_weights.npy")).astype(np.float32)
-    d = np.load(os.path.join(here, "selector_corpus", "heldout_seeds_2000_2399.npz"), allow_pickle=False)
+    wname, cname = (("selector_weights.npy", "heldout_seeds_2000_2399.npz") if which == "spmspv" else
+                    ("bfs_selector_weights.npy", "bfs_heldout_seeds_300_359.npz"))
+    w = np.load(os.path.join(here, wname)).astype(np.float32)
+    d = np.load(os.path.join(here, "selector_corpus", cname), allow_pickle=False)
     X, T, rule = d["feats"], d["times"], d["rule"]
     mean, istd = w[sa.MLP_NPARAM:sa.MLP_NPARAM + F], w[sa.MLP_NPARAM + F:]
     Xn = np.ascontiguousarray((X - mean) * istd, np.float32)
@@ -104,6 +108,26 @@ def test_trained_selector_gain_on_heldout(gpu):
     mlp = regret(pred)
     rules = regret(rule)
     fixed = min(regret(np.full(len(pred), s)) for s in range(C))
-    print(f"held-out regret: mlp {mlp:.4f} rules {rules:.4f} best fixed {fixed:.4f}")
+    print(f"{which} held-out regret: mlp {mlp:.4f} rules {rules:.4f} best fixed {fixed:.4f}")
     assert mlp <= 0.05
     assert mlp < rules and mlp < fixed
+
+
+def test_rule_restatement_matches_device_rule(gpu):
+    """train_selector.rule_from_features (the rule fallback recomputed from
+    recorded features, used to score the BFS corpus) picks the slot the
+    device's rule fallback picks, on an R-MAT s12 matrix over a density
+    sweep."""
+    import train_selector as ts
+    rp, col, val = sa.gen_rmat_csr(12, 16, seed=91)
+    M = sa.Matrix(1 << 12, 1 << 12, rp, col, val)
+    M.A.selector_mode(1)
+    for k in (1, 7, 60, 400, 1500, 4096):
+        key, v = sa.gen_sparse_vector(M.n, k, 300 + k)
+        dk = sa.DeviceArray.from_host(key, np.int32)
+        dv = sa.DeviceArray.from_host(v, np.float32)
+        M.A.set_sparse_x(dk, dv, k)
+        dev = M.A.select()
+        f = M.A.features().astype(np.float64)
+        assert ts.rule_from_features(f) == dev, f"xnnz {k}: python rule {ts.rule_from_features(f)} device {dev}"
+
