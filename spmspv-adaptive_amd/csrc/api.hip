@@ -378,8 +378,9 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       int* ok = y_key ? y_key : h->yk_tmp;
       float* ov = y_val ? y_val : h->yv_tmp;
       if ((rc = ensure_binlen_host(h))) return rc;
-      launch_bucket(s, h->sb, m, h->ctl, h->scan_state, ok, ov, h->binlen_host, h->nnz, next_ep(h, 2));
-      h->launches += 4;
+      launch_bucket(s, h->sb, m, h->ctl, h->acc, h->marks, h->binlen_host, h->nnz, next_ep(h, 1));
+      launch_compact(s, nullptr, h->marks, m, h->acc, 1, ok, ov, h->scan_state, h->ctl, next_ep(h, 1));
+      h->launches += 6;
       if (y_dense) {
         launch_fill_f32(s, y_dense, m, 0.f);
         launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);

@@ -114,3 +114,32 @@ def test_speculation_on_off_same_results(rmat12_csr5):
     finally:
         A.selector_mode(1)
         A.set_speculation(True)
+
+
+@pytest.mark.parametrize("slot", [0, 2, 3, 7])
+def test_edge_sizes_under_speculation(rmat12_csr5, slot):
+    """Empty x, a single entry and full x, each called twice in a row with
+    the slot forced (the second call takes the speculative path: the
+    features-only selector for slots 0 and 7, the column scan for 2 and 3),
+    against the FP64 oracle."""
+    M, (cp, ri, cv) = rmat12_csr5
+    A = M.A
+    n = M.n
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    A.set_speculation(True)
+    try:
+        A.selector_mode(2, slot)
+        for k in (0, 1, n):
+            key, val = sa.gen_sparse_vector(n, k, 5000 + k) if k else (np.zeros(0, np.int32),
+                                                                         np.zeros(0, np.float32))
+            dk = sa.DeviceArray.from_host(key if k else np.zeros(1, np.int32), np.int32)
+            dv = sa.DeviceArray.from_host(val if k else np.zeros(1, np.float32), np.float32)
+            for rep in range(2):
+                A.set_sparse_x(dk, dv, k)
+                ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+                assert used == slot
+                check(M.m, cp, ri, cv, key, val, yk.to_host(ynnz), yv.to_host(ynnz),
+                      f"slot {slot} xnnz {k} call {rep}")
+    finally:
+        A.selector_mode(1)
