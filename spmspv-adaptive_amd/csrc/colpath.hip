@@ -132,7 +132,9 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   }
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
-  if (threadIdx.x == 0) {
+  // a single tile (small x) needs neither the look-back nor the tile max/min
+  const bool single = ntiles == 1;
+  if (!single && threadIdx.x == 0) {
     // device-coherent stores, complete before this tile's aggregate is
     // published; the last tile reads them with device-coherent loads once
     // the look-back has seen every earlier tile (no cache write-back fence)
@@ -142,7 +144,8 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   }
   long long btot;
   const long long bex = block_excl_sum<NT, long long>(tsum, s_l, &btot);
-  const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
+  const unsigned long long pre =
+      single ? 0ull : tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
   long long run = (long long)pre + bex;
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
@@ -161,14 +164,18 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
   }
   if (t != ntiles - 1) return;
-  // last tile: every earlier tile has published, so its tile_mm is in place
-  int gmx = 0, gmn = INT_MAX;
-  for (unsigned u = threadIdx.x; u < ntiles; u += NT) {
-    gmx = max(gmx, __hip_atomic_load(&tile_mm[2 * u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    gmn = min(gmn, __hip_atomic_load(&tile_mm[2 * u + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  int gmx = mx, gmn = mn;
+  if (!single) {
+    // last tile: every earlier tile has published, so its tile_mm is in place
+    gmx = 0;
+    gmn = INT_MAX;
+    for (unsigned u = threadIdx.x; u < ntiles; u += NT) {
+      gmx = max(gmx, __hip_atomic_load(&tile_mm[2 * u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      gmn = min(gmn, __hip_atomic_load(&tile_mm[2 * u + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    gmx = block_max<NT, int>(gmx, s_i);
+    gmn = block_min<NT, int>(gmn, s_i);
   }
-  gmx = block_max<NT, int>(gmx, s_i);
-  gmn = block_min<NT, int>(gmn, s_i);
   if (threadIdx.x == 0) {
     const long long bl = (long long)pre + btot;
     ctl->bin_len = bl;

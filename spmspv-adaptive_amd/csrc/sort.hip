@@ -213,14 +213,18 @@ __global__ __launch_bounds__(NT) void k_digit_count(const unsigned* __restrict__
   if (n <= small_cap) return;
   const unsigned ntiles = (unsigned)((n + TILE - 1) / TILE);
   const int w = threadIdx.x / kWave;
-  // tiles are independent: any grid covers them
+  // tiles are independent: any grid covers them. Only counts are needed
+  // here (no ranks): plain LDS atomics into a private histogram per wave
   for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
     unsigned key[IPT];
     bool valid[IPT];
-    int rank[IPT];
-    unsigned dig[IPT];
     load_tile_keys<NT, IPT>(kin, n, (long long)t * TILE, key, valid);
-    wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
+#pragma unroll
+    for (int q = 0; q < NW; ++q) s_cnt[q * 256 + threadIdx.x] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (valid[j]) atomicAdd(&s_cnt[w * 256 + ((key[j] >> shift) & 255u)], 1u);
     __syncthreads();
     unsigned tot = 0;
 #pragma unroll
