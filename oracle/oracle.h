@@ -9,7 +9,7 @@
  * cites the reference file:line it follows. Paths are relative to the
  * reference root; `la/` abbreviates `hice-spmspv/hice/la/`.
  *
- * Conventions: indices int32, values FP32 on input. The judged oracle
+ * Conventions: indices int32, values FP32 on input. The parity oracle
  * accumulates in FP64 and reports, per output row, the bound
  * sum_j |a_ij * x_j| so that FP32 results can be checked against
  * |y - y64| <= 1e-5 * bound.
@@ -63,7 +63,7 @@ void oracle_spmv_f64(int m, const int* row_ptr, const int* col,
                      const float* val, const float* x_dense,
                      double* y, double* bound);
 
-/* Level-synchronous BFS (the reference has none). Textbook
+/* Level-synchronous BFS (the reference has none: it replays recorded frontiers). Textbook
  * queue BFS over the CSC pattern: the neighbours of u are the rows of column
  * u. levels[n] gets -1 for unreached vertices. Returns the number of levels
  * (max level + 1). queue is scratch of size n. */

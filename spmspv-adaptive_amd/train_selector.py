@@ -203,7 +203,7 @@ def rule_from_features(f):
 def cmd_bfs_corpus(args):
     """BFS level labels: every level of a BFS from a source runs the same
     frontier whatever slot runs it (levels are unique), so the 8 fixed-slot
-    runs give the 8 times of each level; its features are those the driver's
+    runs give the 8 times of each level; its features are those the BFS loop's
     selector computes for that frontier."""
     feats, times, rule, meta = [], [], [], []
     t0 = time.time()
