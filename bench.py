@@ -261,6 +261,7 @@ def main():
         sys.exit(2)
 
     import spmspv_amd as sa
+    ROW_TILED = (sa.K_ROW_LB, sa.K_SPMV_LB)  # the slots whose tile kernel is timed alone
 
     if args.batched:
         print(json.dumps({"batched": run_batched_bench(sa, args)}), flush=True)
@@ -328,19 +329,24 @@ def main():
             timed_call(i)
     step_ms = []
     per_density = [[] for _ in DENSITIES]
+    kern_ms = [[] for _ in DENSITIES]  # the row slots' tile kernel, HIP events on its stream
     chosen = [None] * len(DENSITIES)
     ynnzs = [0] * len(DENSITIES)
+    A.time_main_kernel(True)
     sa.synchronize()
     for _ in range(args.steps):
         tot = 0.0
         for i in range(len(DENSITIES)):
             ms, ynnz, used = timed_call(i)
             per_density[i].append(ms)
+            if used in ROW_TILED:
+                kern_ms[i].append(A.main_kernel_ms())
             chosen[i] = used
             ynnzs[i] = ynnz
             tot += ms
         step_ms.append(tot)
     sa.synchronize()
+    A.time_main_kernel(False)
     step_bytes = sum(alg_bytes(xs[i][0], meta[i], ynnzs[i]) for i in range(len(DENSITIES)))
     ms_step = float(np.median(step_ms))
     value = step_bytes / (ms_step * 1e-3) / 1e9
@@ -370,35 +376,44 @@ def main():
         regret = sel_ms / best_per_density - 1.0
 
     # ---- roofline of the dominant kernel: the slot the selector spends the
-    # most time in, timed alone (cold) with events on its stream
-    dom_i = int(np.argmax([np.median(v) for v in per_density]))
-    dom_slot = chosen[dom_i]
-    reps = 10
-    dom_bytes = alg_bytes(xs[dom_i][0], meta[dom_i], ynnzs[dom_i])
-    kernel_ms = None
-    if dom_slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
-        # the tile kernel alone, events recorded on its own stream around it
-        A.time_main_kernel(True)
-        kms = []
-        for _ in range(reps):
-            timed_call(dom_i, dom_slot)
-            kms.append(A.main_kernel_ms())
-        A.time_main_kernel(False)
-        kernel_ms = float(np.median(kms))
-    dts = [timed_call(dom_i, dom_slot)[0] for _ in range(reps)]
-    call_ms = float(np.median(dts))
-    dom_ms = kernel_ms if kernel_ms is not None else call_ms
+    # most time in over the step, over all of its launches in the timed
+    # steps: algorithmic bytes per launch / average launch duration. For the
+    # row slots the duration is the tile kernel's own (HIP events on the
+    # stream it is launched on, recorded during the timed steps); for the
+    # column slots, whose calls are chains of short kernels, the whole call.
+    med = [float(np.median(v)) for v in per_density]
+    slot_ms = {}
+    for i in range(len(DENSITIES)):
+        slot_ms[chosen[i]] = slot_ms.get(chosen[i], 0.0) + med[i]
+    dom_slot = max(slot_ms, key=slot_ms.get)
+    dom_is = [i for i in range(len(DENSITIES)) if chosen[i] == dom_slot]
+    launches = [(alg_bytes(xs[i][0], meta[i], ynnzs[i]), km) for i in dom_is for km in kern_ms[i]]
+    timed_kernel = dom_slot in ROW_TILED and len(launches) > 0
+    if not timed_kernel:
+        launches = [(alg_bytes(xs[i][0], meta[i], ynnzs[i]), cm) for i in dom_is for cm in per_density[i]]
+    dom_bytes = float(np.mean([b for b, _ in launches]))
+    dom_ms = float(np.mean([t for _, t in launches]))
+    call_ms = float(np.mean([cm for i in dom_is for cm in per_density[i]]))
     dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
     fmt = ""
-    if dom_slot in (sa.K_ROW_LB, sa.K_SPMV_LB) and tuned and tuned["spmv_type"] == 1:
-        fmt = "-csr5"
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", f"{sa.KERNEL_NAMES[dom_slot]}{fmt}_rmat{args.scale}_d{DENSITIES[dom_i]}.json")
-    if os.path.exists(tpath):
+    if dom_slot in ROW_TILED and tuned and tuned["spmv_type"] == 1:
+        fmt = f"-csr5-s{tuned['sigma']}"
+    # measured HBM bytes per launch: the mean of the committed FETCH_SIZE +
+    # WRITE_SIZE profiles of this kernel at each density it ran at (null
+    # unless every one of them is there)
+    traffic, traffic_src = None, []
+    for i in dom_is:
+        tpath = os.path.join(ROOT, "profiles",
+                             f"{sa.KERNEL_NAMES[dom_slot]}{fmt}_rmat{args.scale}_d{DENSITIES[i]}.json")
+        if not os.path.exists(tpath):
+            traffic_src = None
+            break
         with open(tpath) as f:
             tj = json.load(f)
-        traffic = tj["read_bytes"] + tj["write_bytes"]
-        traffic_src = os.path.relpath(tpath, ROOT)
+        traffic_src.append((tj["read_bytes"] + tj["write_bytes"], os.path.relpath(tpath, ROOT)))
+    if traffic_src:
+        traffic = float(np.mean([t for t, _ in traffic_src]))
+        traffic_src = [p for _, p in traffic_src]
 
     bfs = run_bfs(sa, args, A, rp, col, n) if args.bfs_sources > 0 else None
 
@@ -442,9 +457,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": sa.KERNEL_NAMES[dom_slot] + fmt, "density": DENSITIES[dom_i],
+                     "kernel": sa.KERNEL_NAMES[dom_slot] + fmt,
+                     "densities": [DENSITIES[i] for i in dom_is], "launches": len(launches),
                      "launch_ms": round(dom_ms, 5), "call_ms": round(call_ms, 5),
-                     "timed": "tile kernel alone" if kernel_ms is not None else "whole call",
+                     "timed": "tile kernel alone, during the timed steps" if timed_kernel else "whole call",
                      "alg_bytes": dom_bytes},
         "cpu_baseline": cpu,
         "warm_value": round(warm_value, 3),

@@ -303,7 +303,8 @@ int spmspv_timer_stop(spmspv_handle_t h, float* ms);
 /* No counterpart in the reference (measurement): with on != 0, the next
  * K5/K7 calls record events around their tile kernel alone (not the
  * fix-up, compaction or x conversion); spmspv_main_kernel_ms returns the
- * last such kernel's duration. */
+ * duration of the last call's tile kernel, or SPMSPV_INVALID_ARGUMENT when
+ * the last call ran a slot without one. */
 int spmspv_time_main_kernel(spmspv_handle_t h, int on);
 int spmspv_main_kernel_ms(spmspv_handle_t h, float* ms);
 /* Writes `bytes` through a scratch buffer to evict the 256 MB MALL. */

@@ -104,7 +104,7 @@ int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int
   h->sparse_valid = false;
   h->dense_valid = false;
   h->bits_valid = false;
-  h->enc_valid = false;
+  h->enc_valid = h->enc5_valid = false;
   h->prep_valid = false;
   h->binlen_host = -1;
   h->vec_type = -1;

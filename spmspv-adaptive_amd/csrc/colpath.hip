@@ -208,7 +208,8 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
                                                     const int* __restrict__ xkey,
                                                     const float* __restrict__ xval, int ncols, int xnnz,
                                                     long long* part, Ctl* ctl, SelectorArgs sel,
-                                                    float* __restrict__ enc) {
+                                                    float* __restrict__ enc,
+                                                    const int* __restrict__ enc_perm) {
   __shared__ __align__(16) float s_w[kNWeights];
   __shared__ MatFeat s_mf;
   __shared__ long long s_l[NT / kWave];
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
       // value inside the support
 #pragma unroll
       for (int j = 0; j < KPT; ++j)
-        if (ok[j]) enc[c[j]] = xv[j] == 0.f ? -0.f : xv[j];
+        if (ok[j]) enc[enc_perm ? enc_perm[c[j]] : c[j]] = xv[j] == 0.f ? -0.f : xv[j];
     }
   }
   sum = block_sum<NT, long long>(sum, s_l);
@@ -326,12 +327,12 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
 
 void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
                         int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
-                        float* enc) {
+                        float* enc, const int* enc_perm) {
   constexpr int kPer = 256 * 16;
   const int grid = (int)std::max<long long>(
       1, std::min<long long>(kSelMaxGrid, ((long long)xnnz + kPer - 1) / kPer));
   hipLaunchKernelGGL((k_select_feat<256, 16>), dim3(grid), dim3(256), 0, s, colptr, xkey, xval, ncols,
-                     xnnz, part, ctl, sel, enc);
+                     xnnz, part, ctl, sel, enc, enc_perm);
 }
 
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
@@ -701,36 +702,38 @@ void launch_fill_i32(hipStream_t s, int* p, long long n, int v) {
 }
 
 __global__ void k_scatter(const int* __restrict__ key, const float* __restrict__ val, int nnz_arg,
-                          const int* nnz_dev, int len, float* __restrict__ dense, int encode, Gate g) {
+                          const int* nnz_dev, int len, float* __restrict__ dense, int encode, Gate g,
+                          const int* __restrict__ perm) {
   if (gate_shut(g)) return;
   const int nnz = nnz_dev ? *nnz_dev : nnz_arg;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += gridDim.x * blockDim.x) {
     const int k = key[i];
     const float v = val[i];
-    if ((unsigned)k < (unsigned)len) dense[k] = (encode && v == 0.f) ? -0.f : v;
+    if ((unsigned)k < (unsigned)len) dense[perm ? perm[k] : k] = (encode && v == 0.f) ? -0.f : v;
   }
 }
 void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, const int* nnz_dev,
-                    int len, float* dense, bool encode, Gate g) {
+                    int len, float* dense, bool encode, Gate g, const int* perm) {
   int grid = nnz_dev ? 2048 : std::max(1, std::min(8192, (nnz + 255) / 256));
   if (!nnz_dev && nnz <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, key, val, nnz, nnz_dev, len, dense,
-                     encode ? 1 : 0, g);
+                     encode ? 1 : 0, g, perm);
 }
 
 __global__ void k_encode_x(const float* __restrict__ x, const unsigned* __restrict__ bits, int n,
-                           float* __restrict__ enc) {
+                           float* __restrict__ enc, const int* __restrict__ perm) {
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (long long)gridDim.x * blockDim.x) {
     const bool in = (bits[c >> 5] >> (c & 31)) & 1u;
     const float v = x[c];
-    enc[c] = in ? (v == 0.f ? -0.f : v) : 0.f;
+    enc[perm ? perm[c] : c] = in ? (v == 0.f ? -0.f : v) : 0.f;
   }
 }
-void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc) {
+void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc,
+                     const int* perm) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_encode_x, dim3(std::min(8192, (n + 255) / 256)), dim3(256), 0, s, x, bits, n,
-                     enc);
+                     enc, perm);
 }
 
 __global__ void k_set_bits(const int* __restrict__ key, int nnz_arg, const int* nnz_dev, int len,

@@ -126,7 +126,8 @@ __global__ void k_row_markers(const int* __restrict__ row_ptr, int m, int T, int
 }
 
 __global__ void k_csr5_transpose(const int* __restrict__ col, const float* __restrict__ val, int nnz,
-                                 int T, int S, long long total, int* __restrict__ ccol,
+                                 int T, int S, long long total, const int* __restrict__ cperm,
+                                 const int* __restrict__ item, int* __restrict__ ccol,
                                  float* __restrict__ cval) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
@@ -135,8 +136,11 @@ __global__ void k_csr5_transpose(const int* __restrict__ col, const float* __res
     const int i = o / kWave, l = o % kWave;
     const long long src = t * T + (long long)l * S + i;
     const bool ok = src < nnz;
-    ccol[q] = ok ? col[src] : 0;
-    cval[q] = ok ? val[src] : 0.f;
+    // with an item order, position src of the copy holds CSR item item[src]
+    const long long j = ok ? (item ? (long long)item[src] : src) : 0;
+    const int c = ok ? col[j] : 0;
+    ccol[q] = ok ? (cperm ? cperm[c] : c) : 0;
+    cval[q] = ok ? val[j] : 0.f;
   }
 }
 
@@ -310,6 +314,111 @@ __global__ void k_csr5_expand(const int* __restrict__ row_ptr, const int* __rest
 
 }  // namespace
 
+namespace {
+
+// ---- column order (most nonzeros first) -------------------------------------
+// Sort key of column c: nnz - count(c), so an ascending stable sort puts the
+// densest columns first and keeps equal counts in index order; the column
+// index rides along as the value's bits.
+__global__ void k_col_keys_from_ptr(const int* __restrict__ colptr, int n, long long nnz,
+                                    unsigned* __restrict__ keys, float* __restrict__ vals) {
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x) {
+    keys[c] = (unsigned)(nnz - (colptr[c + 1] - colptr[c]));
+    vals[c] = __int_as_float((int)c);
+  }
+}
+__global__ void k_col_count(const int* __restrict__ col, int nnz, int n, unsigned* cnt) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
+       j += (long long)gridDim.x * blockDim.x) {
+    const int c = col[j];
+    if ((unsigned)c < (unsigned)n) atomicAdd(cnt + c, 1u);
+  }
+}
+__global__ void k_col_keys_from_count(const unsigned* __restrict__ cnt, int n, long long nnz,
+                                      unsigned* __restrict__ keys, float* __restrict__ vals) {
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x) {
+    keys[c] = (unsigned)(nnz - (long long)cnt[c]);
+    vals[c] = __int_as_float((int)c);
+  }
+}
+__global__ void k_set_count(long long* p, long long v) { *p = v; }
+// sort 1 of the item order: key = new column of item j, value = j
+__global__ void k_item_keys_col(const int* __restrict__ col, int nnz, const int* __restrict__ cperm,
+                                unsigned* __restrict__ keys, float* __restrict__ vals) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
+       j += (long long)gridDim.x * blockDim.x) {
+    keys[j] = (unsigned)cperm[col[j]];
+    vals[j] = __int_as_float((int)j);
+  }
+}
+// sort 2: key = row of the item (binary search in row_ptr), value = item,
+// written into (keys, vals_out) from the order of sort 1 (in place allowed)
+__global__ void k_item_keys_row(const float* vals_in, int nnz, const int* __restrict__ row_ptr, int m,
+                                unsigned* keys, float* vals_out) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+       q += (long long)gridDim.x * blockDim.x) {
+    const float v = vals_in[q];
+    const int j = __float_as_int(v);
+    int lo = 0, hi = m;  // last row r with row_ptr[r] <= j
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (row_ptr[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    keys[q] = (unsigned)lo;
+    vals_out[q] = v;
+  }
+}
+__global__ void k_invert_order(const float* __restrict__ vals, int n, int* __restrict__ cperm) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    cperm[__float_as_int(vals[q])] = (int)q;
+}
+
+}  // namespace
+
+bool launch_column_order(hipStream_t s, const ColumnOrderArgs& a, const int* col, int nnz, int n,
+                         int* cperm) {
+  if (n <= 0 || n > a.sb_cap || nnz >= 0x7ffffffe) return false;
+  const int grid = std::max(1, std::min(8192, (n + 255) / 256));
+  if (a.csc_colptr) {
+    hipLaunchKernelGGL(k_col_keys_from_ptr, dim3(grid), dim3(256), 0, s, a.csc_colptr, n,
+                       (long long)nnz, a.sb.keyA, a.sb.valA);
+  } else {
+    launch_fill_i32(s, reinterpret_cast<int*>(a.sb.keyB), n, 0);
+    hipLaunchKernelGGL(k_col_count, dim3(std::max(1, std::min(8192, (nnz + 255) / 256))), dim3(256), 0, s,
+                       col, nnz, n, a.sb.keyB);
+    hipLaunchKernelGGL(k_col_keys_from_count, dim3(grid), dim3(256), 0, s, a.sb.keyB, n, (long long)nnz,
+                       a.sb.keyA, a.sb.valA);
+  }
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, a.n_dev, (long long)n);
+  const int fb = launch_radix_sort(s, a.sb, nnz + 1, a.n_dev, a.ctl, n, n, false, a.ep);
+  hipLaunchKernelGGL(k_invert_order, dim3(grid), dim3(256), 0, s, fb ? a.sb.valB : a.sb.valA, n, cperm);
+  return hipGetLastError() == hipSuccess;
+}
+
+// The items of every row ordered by their new column (two stable sorts: by
+// new column, then by row); returns the order (CSR item of every position)
+// in the sort buffers, valid until they are reused, or null.
+static const int* launch_item_order(hipStream_t s, const ColumnOrderArgs& a, const int* row_ptr,
+                                    const int* col, int nnz, int m, int n, const int* cperm) {
+  if (nnz <= 0 || nnz > a.sb_cap || m <= 0) return nullptr;
+  const int grid = std::max(1, std::min(8192, (nnz + 255) / 256));
+  hipLaunchKernelGGL(k_item_keys_col, dim3(grid), dim3(256), 0, s, col, nnz, cperm, a.sb.keyA, a.sb.valA);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, a.n_dev, (long long)nnz);
+  const Ep e1{a.ep.v + 4u, -1};
+  const int f1 = launch_radix_sort(s, a.sb, n, a.n_dev, a.ctl, nnz, nnz, false, e1);
+  // sort 2 reads its keys from wherever sort 1 left the items, into A
+  hipLaunchKernelGGL(k_item_keys_row, dim3(grid), dim3(256), 0, s, f1 ? a.sb.valB : a.sb.valA, nnz, row_ptr,
+                     m, a.sb.keyA, a.sb.valA);
+  const Ep e2{a.ep.v + 8u, -1};
+  const int f2 = launch_radix_sort(s, a.sb, m, a.n_dev, a.ctl, nnz, nnz, false, e2);
+  if (hipGetLastError() != hipSuccess) return nullptr;
+  return reinterpret_cast<const int*>(f2 ? a.sb.valB : a.sb.valA);
+}
+
 int csr5_sigma_of(int sigma, int m, long long nnz) {
   int s = sigma;
   if (s <= 0) {
@@ -327,7 +436,7 @@ int csr5_sigma_of(int sigma, int m, long long nnz) {
   return 32;
 }
 
-size_t csr5_bytes(int m, long long nnz, int S) {
+size_t csr5_bytes(int m, int n, long long nnz, int S) {
   const long long T = (long long)kWave * S;
   const long long ntiles = (nnz + T - 1) / T;
   auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
@@ -338,11 +447,13 @@ size_t csr5_bytes(int m, long long nnz, int S) {
   b += up(sizeof(int) * (size_t)m) * 4;                                  // rank, rows_ne, rows_empty, y_ne
   b += up(sizeof(int) * (size_t)(m / kScanPart + 2));                    // part sums
   b += up(sizeof(float) * (size_t)(ntiles + 1)) * 3;                     // carries
+  b += up(sizeof(int) * (size_t)std::max(n, 1));                         // column order
   return b;
 }
 
 hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const float* val, int m,
-                      int nnz, int S, void* mem, int* h_pin, Csr5* out) {
+                      int n, int nnz, int S, void* mem, int* h_pin, const ColumnOrderArgs* order,
+                      Csr5* out) {
   const int T = kWave * S;
   const int ntiles = (int)(((long long)nnz + T - 1) / T);
   auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
@@ -376,6 +487,14 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
   p += up(sizeof(float) * (size_t)(ntiles + 1));
   c.carry_hit = reinterpret_cast<int*>(p);
   p += up(sizeof(float) * (size_t)(ntiles + 1));
+  int* cperm = reinterpret_cast<int*>(p);
+  p += up(sizeof(int) * (size_t)std::max(n, 1));
+  c.n = n;
+  const int* item = nullptr;
+  if (order && launch_column_order(s, *order, col, nnz, n, cperm)) {
+    c.cperm = cperm;
+    item = launch_item_order(s, *order, row_ptr, col, nnz, m, n, cperm);
+  }
 
   const int parts = (m + kScanPart - 1) / kScanPart;
   if (parts > 0) {
@@ -397,7 +516,7 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
                        m, T, S, c.flags);
     const long long total = (long long)ntiles * T;
     hipLaunchKernelGGL(k_csr5_transpose, dim3((unsigned)std::min<long long>(8192, (total + 255) / 256)),
-                       dim3(256), 0, s, col, val, nnz, T, S, total, c.ccol, c.cval);
+                       dim3(256), 0, s, col, val, nnz, T, S, total, c.cperm, item, c.ccol, c.cval);
   }
   *out = c;
   return hipGetLastError();

@@ -75,7 +75,7 @@ void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, i
 // (already all +0.0), x's support encoding is scattered into it as well.
 void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
                         int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
-                        float* enc);
+                        float* enc, const int* enc_perm = nullptr);
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
                       const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,
@@ -99,10 +99,13 @@ void launch_signal_int(hipStream_t s, const int* src, long long* dst, Gate g = {
 void launch_fill_i32(hipStream_t s, int* p, long long n, int v);
 // dense[key[i]] = val[i]; with `encode`, a zero value is stored as -0.0
 // (the support encoding: +0.0 outside the support, nonzero bits inside).
+// With perm, entry k lands at dense[perm[k]].
 void launch_scatter(hipStream_t s, const int* key, const float* val, int nnz, const int* nnz_dev,
-                    int len, float* dense, bool encode = false, Gate g = {});
-// enc[c] = bit c of bits ? (x[c] == 0 ? -0.0 : x[c]) : +0.0
-void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc);
+                    int len, float* dense, bool encode = false, Gate g = {}, const int* perm = nullptr);
+// enc[c] = bit c of bits ? (x[c] == 0 ? -0.0 : x[c]) : +0.0 (at enc[perm[c]]
+// with perm)
+void launch_encode_x(hipStream_t s, const float* x, const unsigned* bits, int n, float* enc,
+                     const int* perm = nullptr);
 void launch_set_bits(hipStream_t s, const int* key, int nnz, const int* nnz_dev, int len,
                      unsigned* bits);
 void launch_clear_keys(hipStream_t s, const int* key, int nnz, float* dense, unsigned* bits);
@@ -142,6 +145,11 @@ void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, float* acc
 // --- CSR5 for wave64 (spmv_type 1) ----------------------------------------
 struct Csr5 {
   int S = 0;            // items per lane (sigma)
+  int n = 0;            // columns
+  // new index of every column (most nonzeros first: the gathers of the hot
+  // columns then share cache lines), or null when the columns keep their
+  // order; ccol holds the new indices and x must be given in that order
+  int* cperm = nullptr;
   const int* row_ptr = nullptr;  // the CSR it was built from
   int ntiles = 0;       // tiles of 64 * S items
   int n_ne = 0, n_empty = 0;
@@ -159,11 +167,30 @@ struct Csr5 {
 };
 // sigma (<= 0: from nonzeros per row) rounded up to an instantiated width
 int csr5_sigma_of(int sigma, int m, long long nnz);
-size_t csr5_bytes(int m, long long nnz, int S);
-// Builds the CSR5 copy of a CSR into `mem` (csr5_bytes); synchronises s.
+size_t csr5_bytes(int m, int n, long long nnz, int S);
+// What the column ordering of a CSR5 build may use: the sort buffers (n <=
+// their capacity), the control block, a device count word, fresh epochs,
+// and the CSC column pointer when there is one (else columns are counted).
+struct ColumnOrderArgs {
+  SortBufs sb;
+  long long sb_cap;
+  Ctl* ctl;
+  long long* n_dev;
+  Ep ep;  // 12 consecutive epochs
+  const int* csc_colptr;
+};
+// cperm[c] = rank of column c by nonzero count (descending, ties by index).
+// Returns false (no order) when the sort buffers cannot hold n keys.
+bool launch_column_order(hipStream_t s, const ColumnOrderArgs& a, const int* col, int nnz, int n,
+                         int* cperm);
+// Builds the CSR5 copy of a CSR into `mem` (csr5_bytes); with `order`, the
+// columns are renumbered by nonzero count and every row's items put in that
+// order (the caller's CSR is untouched); synchronises s.
 hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const float* val, int m,
-                      int nnz, int S, void* mem, int* h_pin, Csr5* out);
-// Same gather modes as launch_row_merge (x == nullptr: bit array only).
+                      int n, int nnz, int S, void* mem, int* h_pin, const ColumnOrderArgs* order,
+                      Csr5* out);
+// Same gather modes as launch_row_merge (x == nullptr: bit array only, for
+// a copy without a column order). x is indexed by the copy's column order.
 // Without empty rows the results go to y (one mark byte per row with hits).
 // With empty rows they go to the non-empty-row space: vals_out = y_ne,
 // marks indexed by non-empty row, keymap_out = rows_ne (for the

@@ -397,3 +397,32 @@ def test_autotune_then_parity(rmat12):
     M.A.bfs(0, lev, policy=5, use_graphs=1)
     ref, _ = oracle.bfs(M.n, cp, ri, 0)
     np.testing.assert_array_equal(lev.to_host(), ref)
+
+
+def test_main_kernel_timing(rmat12):
+    """With tile-kernel timing on, a K5/K7 call reports its tile kernel's
+    duration (positive, within the call's own); a column slot's call
+    reports none, not the previous call's."""
+    M, (cp, ri, cv) = rmat12
+    key, val = sa.gen_sparse_vector(M.n, 800, 123)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    M.A.time_main_kernel(True)
+    try:
+        for slot in (sa.K_ROW_LB, sa.K_COL_LB_ATOMIC, sa.K_SPMV_LB, sa.K_COL_NAIVE_SORT):
+            M.A.set_sparse_x(dk, dv, key.size)
+            M.A.timer_start()
+            nnz = M.A.run(slot, y_key=yk, y_val=yv)
+            call_ms = M.A.timer_stop()
+            check_result(M.m, cp, ri, cv, key, val, yk.to_host(nnz), yv.to_host(nnz), f"timed slot {slot}")
+            if slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
+                km = M.A.main_kernel_ms()
+                say(f"slot {slot}: tile kernel {km:.4f} ms of {call_ms:.4f} ms")
+                assert 0.0 < km <= call_ms
+            else:
+                with pytest.raises(sa.SpmspvError):
+                    M.A.main_kernel_ms()
+    finally:
+        M.A.time_main_kernel(False)
