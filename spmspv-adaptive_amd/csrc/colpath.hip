@@ -199,10 +199,12 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
 
 // Features-only selector launch: bin_len and the largest and smallest
 // column length over x's support (a plain reduction: no scan, no tile
-// chain), then the selector in the last workgroup. For calls whose slot is
-// a row slot, which needs none of the column scan. The last workgroup waits
-// for the partials of the others: the grid is at most kSelMaxGrid
-// workgroups, all of them resident, and it is dispatched last.
+// chain), then the selector. For calls whose slot is a row slot, which
+// needs none of the column scan. Every workgroup publishes its partials and
+// counts itself done; the one that finishes last (the count it reads is
+// the grid size less one) reduces the partials and runs the selector, so no
+// workgroup waits for another. Every workgroup loads the selector weights
+// at its start, since any of them may be the last.
 template <int NT, int KPT>
 __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colptr,
                                                     const int* __restrict__ xkey,
@@ -214,8 +216,8 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
   __shared__ MatFeat s_mf;
   __shared__ long long s_l[NT / kWave];
   __shared__ int s_i[NT / kWave];
-  const bool last = blockIdx.x == gridDim.x - 1;
-  const bool selects = last && sel.mat != nullptr;
+  __shared__ unsigned s_last;
+  const bool selects = sel.mat != nullptr;
   constexpr int kWPT = (kNWeights + NT - 1) / NT;
   constexpr int kMF = sizeof(MatFeat) / sizeof(double);
   float wr[kWPT];
@@ -233,10 +235,29 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
   const int w = threadIdx.x / kWave;
   long long sum = 0;
   int mx = 0, mn = INT_MAX;
+  // the support encoding of a round is scattered at the start of the next
+  // one, and that of the last round after this workgroup has published its
+  // partials: the publication then waits for no scatter store, and the
+  // selector runs while the stores drain
+  int ok[KPT], pc[KPT];
+  float xv[KPT];
+  bool pending = false;
+  auto scatter = [&]() {
+    // one wait for every load here; otherwise each store behind a branch
+    // below waits for the whole memory counter, the stores before it
+    // included
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) asm volatile("" ::"v"(pc[j]), "v"(xv[j]));
+    // support encoding (as launch_scatter with encode): -0.0 marks a zero
+    // value inside the support
+#pragma unroll
+    for (int j = 0; j < KPT; ++j)
+      if (ok[j]) enc[pc[j]] = xv[j] == 0.f ? -0.f : xv[j];
+  };
   for (long long t0 = (long long)blockIdx.x * NT * KPT; t0 < xnnz; t0 += (long long)gridDim.x * NT * KPT) {
+    if (pending) scatter();
     const long long b0 = t0 + (long long)w * kWave * KPT + lane();
-    int c[KPT], beg[KPT], end[KPT], ok[KPT];
-    float xv[KPT];
+    int c[KPT], beg[KPT], end[KPT];
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
       const long long i = b0 + (long long)j * kWave;
@@ -250,6 +271,9 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
       const int ci = c[j] & -ok[j];
       beg[j] = colptr[ci];
       end[j] = colptr[ci + ok[j]];
+      // the permuted position, loaded unconditionally beside the column
+      // bounds (a load behind the store's branch would be waited for alone)
+      pc[j] = enc_perm ? enc_perm[ci] : ci;
     }
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
@@ -260,39 +284,35 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
         mn = min(mn, len);
       }
     }
-    if (enc) {
-      // support encoding (as launch_scatter with encode): -0.0 marks a zero
-      // value inside the support
-#pragma unroll
-      for (int j = 0; j < KPT; ++j)
-        if (ok[j]) enc[enc_perm ? enc_perm[c[j]] : c[j]] = xv[j] == 0.f ? -0.f : xv[j];
-    }
+    pending = enc != nullptr;
   }
   sum = block_sum<NT, long long>(sum, s_l);
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
-  if (!last) {
-    if (threadIdx.x == 0) {
-      __hip_atomic_store(&part[blockIdx.x], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&part[kSelMaxGrid + blockIdx.x], (long long)mx, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&part[2 * kSelMaxGrid + blockIdx.x], (long long)mn, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // partials in place before the count
-      __hip_atomic_fetch_add(&ctl->done[kSlotSelect], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return;
-  }
   if (threadIdx.x == 0) {
-    while (__hip_atomic_load(&ctl->done[kSlotSelect], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-           gridDim.x - 1)
-      __builtin_amdgcn_s_sleep(2);
-    __hip_atomic_store(&ctl->done[kSlotSelect], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&part[blockIdx.x], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&part[kSelMaxGrid + blockIdx.x], (long long)mx, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&part[2 * kSelMaxGrid + blockIdx.x], (long long)mn, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // partials in place before the count
+    const unsigned before =
+        __hip_atomic_fetch_add(&ctl->done[kSlotSelect], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = before == gridDim.x - 1 ? 1u : 0u;
+    // the last one resets the count for the next launch (stream-ordered)
+    if (before == gridDim.x - 1)
+      __hip_atomic_store(&ctl->done[kSlotSelect], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
+  if (!s_last) {
+    if (pending) scatter();
+    return;
+  }
+  // the other workgroups' partials (this one's own are in registers)
   long long osum = 0;
   int omx = 0, omn = INT_MAX;
-  for (unsigned u = threadIdx.x; u + 1 < gridDim.x; u += NT) {
+  for (unsigned u = threadIdx.x; u < gridDim.x; u += NT) {
+    if (u == blockIdx.x) continue;
     osum += __hip_atomic_load(&part[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     omx = max(omx, (int)__hip_atomic_load(&part[kSelMaxGrid + u], __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT));
@@ -308,7 +328,10 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
     ctl->min_len = mn;
     ctl->xnnz = xnnz;
   }
-  if (!selects) return;
+  if (!selects) {
+    if (pending) scatter();
+    return;
+  }
   if (sel.weights) {
 #pragma unroll
     for (int i = 0; i < kWPT; ++i) {
@@ -323,6 +346,7 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
   sl.mat = &s_mf;
   select_in_wave(ctl, sl, (double)xnnz, (double)sum, (double)mx);
   if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(sum, ctl->kernel_choice);
+  if (pending) scatter();
 }
 
 void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
@@ -542,15 +566,17 @@ void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, 
 }
 
 // -------------------------------------------------------------- compact
-// One word of 32 rows per thread, 8192 rows per tile, ordered tiles. The
-// hits come either from a bit array (row kernels) or from one mark byte per
-// row (column atomic kernels); both are cleared on the way.
-__device__ __forceinline__ unsigned bytes_to_bits(uint4 a, uint4 b) {
+// 2,048 rows per tile (kCmpNT bit-array words), 8 rows per thread, ordered
+// tiles. The hits come either from a bit array (row kernels) or from one
+// mark byte per row (column atomic kernels); both are cleared on the way.
+// Eight rows per thread (not one word of 32) puts 256 threads on a tile, so
+// that at high hit rates each thread writes at most a few outputs, all of
+// whose loads go out in one round.
+__device__ __forceinline__ unsigned bytes_to_bits8(uint2 a) {
   // byte k (0 or 1) of a 32-bit lane -> bit k: the four partial products of
   // x * 0x01020408 land on distinct bits, bits 24..27 hold bytes 0..3
   auto nib = [](unsigned x) { return (((x & 0x01010101u) * 0x01020408u) >> 24) & 0xFu; };
-  return nib(a.x) | (nib(a.y) << 4) | (nib(a.z) << 8) | (nib(a.w) << 12) | (nib(b.x) << 16) |
-         (nib(b.y) << 20) | (nib(b.z) << 24) | (nib(b.w) << 28);
+  return nib(a.x) | (nib(a.y) << 4);
 }
 
 template <int NT, bool BYTES>
@@ -561,23 +587,22 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
                                                 unsigned long long* state, Ctl* ctl, Ep ep,
                                                 const int* __restrict__ keymap, Gate g) {
   if (gate_shut(g)) return;
+  constexpr int kRows = 8, kTile = NT * kRows;
+  static_assert(kTile == kCmpNT * 32, "a tile is kCmpNT bit-array words");
   __shared__ unsigned long long s_pre;
   __shared__ int s_i[NT / kWave];
-  __shared__ int s_rows[NT * 32];
-  const int nwords = (m + 31) >> 5;
-  const unsigned ntiles = (unsigned)((nwords + NT - 1) / NT);
+  __shared__ int s_rows[kTile];
+  const unsigned ntiles = (unsigned)(((long long)m + kTile - 1) / kTile);
   const unsigned t = blockIdx.x;
   if (t >= ntiles) return;
   const unsigned epoch = ep_of(ctl, ep);
-  const int w = (int)t * NT + threadIdx.x;
+  // first row of this thread; the mark bytes are padded to 32 rows and the
+  // bit array to whole words, so the 8 rows are readable whenever r0 < m
+  const int r0 = (int)t * kTile + (int)threadIdx.x * kRows;
   unsigned b = 0u;
-  if (w < nwords) {
-    if (BYTES) {
-      const uint4* p = reinterpret_cast<const uint4*>(marks) + 2 * (size_t)w;
-      b = bytes_to_bits(p[0], p[1]);
-    } else {
-      b = bitmap[w];
-    }
+  if (r0 < m) {
+    if (BYTES) b = bytes_to_bits8(*reinterpret_cast<const uint2*>(marks + r0));
+    else b = (bitmap[r0 >> 5] >> (r0 & 31)) & 0xFFu;
   }
   const int cnt = __popc(b);
   int btot;
@@ -588,16 +613,16 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
   int loc = bex;
   if (b) {
     if (BYTES) {
-      uint4* p = reinterpret_cast<uint4*>(marks) + 2 * (size_t)w;
-      p[0] = make_uint4(0u, 0u, 0u, 0u);
-      p[1] = make_uint4(0u, 0u, 0u, 0u);
+      *reinterpret_cast<uint2*>(marks + r0) = make_uint2(0u, 0u);
     } else {
-      bitmap[w] = 0u;
+      // the four threads of a word are lanes of one wave, which all read
+      // the word before any of them clears it
+      bitmap[r0 >> 5] = 0u;
     }
     while (b) {
       const int bit = __ffs(b) - 1;
       b &= b - 1;
-      s_rows[loc++] = (w << 5) + bit;
+      s_rows[loc++] = r0 + bit;
     }
   }
   __syncthreads();
@@ -641,13 +666,13 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
                     Ep ep, const int* keymap, Gate g) {
-  const int nwords = (m + 31) >> 5;
-  const int grid = std::max(1, (nwords + kCmpNT - 1) / kCmpNT);
+  constexpr int nt = kCmpNT * 4;
+  const int grid = std::max(1, (int)(((long long)m + kCmpNT * 32 - 1) / (kCmpNT * 32)));
   if (marks)
-    hipLaunchKernelGGL((k_compact<kCmpNT, true>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
+    hipLaunchKernelGGL((k_compact<nt, true>), dim3(grid), dim3(nt), 0, s, bitmap, marks, m, acc,
                        reset_acc, ykey, yval, state, ctl, ep, keymap, g);
   else
-    hipLaunchKernelGGL((k_compact<kCmpNT, false>), dim3(grid), dim3(kCmpNT), 0, s, bitmap, marks, m, acc,
+    hipLaunchKernelGGL((k_compact<nt, false>), dim3(grid), dim3(nt), 0, s, bitmap, marks, m, acc,
                        reset_acc, ykey, yval, state, ctl, ep, keymap, g);
 }
 

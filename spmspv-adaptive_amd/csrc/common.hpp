@@ -322,7 +322,9 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
 // consecutive tiles with the same row form one run. For the first tile of
 // a run, sums the run in tile order into (s, h) and returns true. The
 // loads of the tile, its neighbours and its own carry are issued together:
-// one round trip to memory for the common run of one or two tiles.
+// one round trip to memory for the common run of one or two tiles; longer
+// runs (a row over many tiles) are read kCarryBatch tiles per round trip.
+constexpr int kCarryBatch = 8;
 template <typename Idx>
 __device__ __forceinline__ bool carry_run(const float* __restrict__ cv, const int* __restrict__ crow,
                                           const int* __restrict__ chit, Idx t, Idx ntiles, int& r,
@@ -338,15 +340,32 @@ __device__ __forceinline__ bool carry_run(const float* __restrict__ cv, const in
   asm volatile("" ::"v"(v), "v"(rn), "v"(h), "v"(rp));
   if (r < 0 || (t > 0 && rp == r)) return false;
   s = 0.f + v;
-  Idx u = t + 1;
-  int ru = u < ntiles ? rn : -1;
-  while (ru == r) {
-    s += cv[u];
-    if (chit) h |= chit[u];
-    ++u;
-    ru = u < ntiles ? crow[u] : -1;
+  if (!(t + 1 < ntiles && rn == r)) return true;
+  // the run goes on at t + 1: batches of tiles, loads issued together,
+  // summed in tile order until the row changes
+  for (Idx u = t + 1;; u += kCarryBatch) {
+    int br[kCarryBatch], bh[kCarryBatch];
+    float bv[kCarryBatch];
+#pragma unroll
+    for (int k = 0; k < kCarryBatch; ++k) {
+      const Idx q = u + k < ntiles ? u + k : ntiles - 1;
+      br[k] = crow[q];
+      bv[k] = cv[q];
+      bh[k] = chit ? chit[q] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kCarryBatch; ++k) asm volatile("" ::"v"(br[k]), "v"(bv[k]), "v"(bh[k]));
+    bool go = true;
+#pragma unroll
+    for (int k = 0; k < kCarryBatch; ++k) {
+      go = go && u + k < ntiles && br[k] == r;
+      if (go) {
+        s += bv[k];
+        h |= bh[k];
+      }
+    }
+    if (!go) return true;
   }
-  return true;
 }
 
 }  // namespace spmspv

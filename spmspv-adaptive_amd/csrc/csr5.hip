@@ -28,6 +28,17 @@ namespace spmspv {
 namespace {
 
 constexpr int kXPlain5 = 0, kXEnc5 = 1, kXBits5 = 2;
+// columns (after renumbering, the ones with the most nonzeros) whose x
+// entries the tile kernel keeps in LDS: all 160 KB of a CU's LDS, shared by
+// one block per CU that holds as many waves as the registers allow. At R-MAT
+// s20 they take 72 % of the nonzeros, and the distinct cache lines per
+// wave-wide gather fall to under a third.
+constexpr int kCsr5Hot = 40960;
+// threads per block of the tile kernel: 256 without the LDS copy; with it,
+// all the waves a CU can hold at the register use of each width (168
+// registers at S = 24: 3 waves per SIMD; 241 at S = 32: 2; <= 128 below: 4)
+constexpr int csr5_nt(int S, int HOT) { return HOT == 0 ? 256 : S >= 32 ? 512 : S >= 24 ? 768 : 1024; }
+static_assert(kCsr5Hot * 4 <= 160 * 1024, "LDS of one CU");
 
 // ---- preprocessing --------------------------------------------------------
 constexpr int kScanPart = 2048;
@@ -145,8 +156,11 @@ __global__ void k_csr5_transpose(const int* __restrict__ col, const float* __res
 }
 
 // ---- SpMV / masked SpMSpV ---------------------------------------------------
-template <int S, int XIN, bool MASKED, bool HITS>
-__global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
+// HOT > 0: the first `hot` (<= HOT) entries of x, the columns with the most
+// nonzeros once the columns are renumbered, are copied into LDS at kernel
+// start and gathered from there; the other lanes of a gather read global x.
+template <int S, int XIN, bool MASKED, bool HITS, int HOT>
+__global__ __launch_bounds__(csr5_nt(S, HOT)) void k_csr5(const int* __restrict__ ccol,
                                               const float* __restrict__ cval,
                                               const unsigned* __restrict__ flags,
                                               const unsigned* __restrict__ tinfo,
@@ -155,11 +169,33 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
                                               float* __restrict__ y, unsigned char* __restrict__ marks,
                                               float* __restrict__ carry_val,
                                               int* __restrict__ carry_row,
-                                              int* __restrict__ carry_hit, Gate g) {
+                                              int* __restrict__ carry_hit, int hot, int xn, Gate g) {
   if (gate_shut(g)) return;
   constexpr int T = kWave * S;
   const int lanes = lane();
   const int waves = gridDim.x * (blockDim.x / kWave);
+  __shared__ float sx[HOT > 0 ? HOT : 1];
+  // x through a buffer descriptor: one 32-bit offset per gather, and an
+  // offset past x reads 0 instead of faulting
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, xn * 4, 0x00020000);
+  if (HOT > 0) {
+    // every wave of the block takes part (before any of them may leave)
+    constexpr int NT = csr5_nt(S, HOT);
+    constexpr int PT = HOT > 0 ? (HOT + NT - 1) / NT : 1;
+    float v[PT];
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int c = k * NT + (int)threadIdx.x;
+      v[k] = x[c < hot ? c : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int c = k * NT + (int)threadIdx.x;
+      if (HOT % NT == 0 || c < HOT) sx[c] = v[k];
+    }
+    __syncthreads();
+  }
   // wave-uniform, so that per-tile words come through scalar loads
   int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
   if (t >= ntiles) return;
@@ -168,6 +204,9 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
   float vN[S];
   float vG[S], xG[S];  // stage 1: values and gathered x (or bit words)
   int cG[S];
+  // hot items: their product (masked) is formed as soon as LDS answers and
+  // kept in vG; bit i of hG marks item i hot, bit i of bG its support bit
+  unsigned hG = 0u, bG = 0u;
   unsigned fG;  // the tile's marker word
 
   auto load_cv = [&](int tt) {
@@ -180,18 +219,53 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
     }
   };
   auto gather = [&](int tt) {
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      if (XIN == kXBits5) xG[i] = __builtin_bit_cast(float, xbits[(unsigned)cN[i] >> 5]);
-      else xG[i] = x[(unsigned)cN[i]];
-    }
-    fG = flags[(size_t)tt * kWave + lanes];
     // real moves rather than renames (see the row-merge kernel): the stage
-    // registers keep their roles and no in-flight load is copied
+    // registers keep their roles and no in-flight load is copied. They come
+    // first so that vN is free before the gathers' registers are taken (its
+    // loads are a tile old; the column loads issued beside them are waited
+    // for by the gathers anyway)
 #pragma unroll
     for (int i = 0; i < S; ++i) {
       asm volatile("v_mov_b32 %0, %1" : "=v"(vG[i]) : "v"(vN[i]));
       if (XIN == kXBits5) asm volatile("v_mov_b32 %0, %1" : "=v"(cG[i]) : "v"(cN[i]));
+    }
+    unsigned hm = 0u;
+    if (HOT > 0) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) hm |= ((unsigned)cN[i] < (unsigned)hot ? 1u : 0u) << i;
+    }
+    float xL[HOT > 0 ? S : 1];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      if (XIN == kXBits5) {
+        xG[i] = __builtin_bit_cast(float, xbits[(unsigned)cN[i] >> 5]);
+      } else if (HOT > 0) {
+        // both reads unconditional: the hot lanes send the global read to
+        // x[0] (one shared line), the others read LDS word 0 (a broadcast)
+        const unsigned c = (unsigned)cN[i];
+        const bool hc = (hm >> i) & 1u;
+        xG[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, (hc ? 0u : c) * 4u, 0, 0));
+        xL[i] = sx[hc ? c : 0u];
+      } else {
+        xG[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, (unsigned)cN[i] * 4u, 0, 0));
+      }
+    }
+    fG = flags[(size_t)tt * kWave + lanes];
+    if (HOT > 0) {
+      // LDS answers within a few hundred cycles: the hot products are formed
+      // here, so that xL dies at once and only two mask words live on
+      unsigned bm = 0u;
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const bool hc = (hm >> i) & 1u;
+        const bool xb = __builtin_bit_cast(unsigned, xL[i]) != 0u;
+        float p = vG[i] * xL[i];
+        if (MASKED) p = xb ? p : 0.f;
+        vG[i] = hc ? p : vG[i];
+        bm |= (hc && xb ? 1u : 0u) << i;
+      }
+      hG = hm;
+      bG = bm;
     }
   };
 
@@ -201,6 +275,7 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
 
   for (; t < ntiles; t += waves) {
     const unsigned f = fG;
+    const unsigned hh = hG, hb = bG;
     const unsigned ti = tinfo[t];
     const unsigned tn = tinfo[t + 1];
     const int base = (int)(ti & 0x7fffffffu);
@@ -250,6 +325,10 @@ __global__ __launch_bounds__(256) void k_csr5(const int* __restrict__ ccol,
         b = __builtin_bit_cast(unsigned, xG[i]) != 0u;
         q = vG[i] * xG[i];
         if (MASKED) q = b ? q : 0.f;
+        if (HOT > 0 && ((hh >> i) & 1u)) {  // a select, not a branch
+          b = (hb >> i) & 1u;
+          q = vG[i];
+        }
       }
       s += ok ? q : 0.f;
       if (HITS) h |= (ok && b) ? 1 : 0;
@@ -536,20 +615,28 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
     if (y_dense_rows) launch_fill_f32(s, y, m, 0.f, g);
     return;
   }
-#define SPMSPV_CSR5(SS, XI, MK, HT)                                                              \
+  // the LDS copy holds the hot columns once they come first (column order),
+  // or the whole of x when it fits
+  const bool lds_hot = c.n >= 1 && (c.cperm != nullptr || c.n <= kCsr5Hot);
+#define SPMSPV_CSR5(SS, XI, MK, HT, HOTN)                                                        \
   {                                                                                              \
-    const void* fn = (const void*)k_csr5<SS, XI, MK, HT>;                                        \
-    const int waves = std::max(4, std::min(c.ntiles, resident_blocks(fn, 256) * 4));             \
-    hipLaunchKernelGGL((k_csr5<SS, XI, MK, HT>), dim3((waves + 3) / 4), dim3(256), 0, s, c.ccol,  \
-                       c.cval, c.flags, c.tinfo, x, xbits, nnz, c.ntiles, yk, marks,             \
-                       c.carry_val, c.carry_row, c.carry_hit, g);                                \
+    const void* fn = (const void*)k_csr5<SS, XI, MK, HT, HOTN>;                                  \
+    constexpr int nt = csr5_nt(SS, HOTN), wpb = nt / kWave;                                      \
+    const int waves = std::max(wpb, std::min(c.ntiles, resident_blocks(fn, nt) * wpb));          \
+    hipLaunchKernelGGL((k_csr5<SS, XI, MK, HT, HOTN>), dim3((waves + wpb - 1) / wpb), dim3(nt), 0, \
+                       s,                                                                        \
+                       c.ccol, c.cval, c.flags, c.tinfo, x, xbits, nnz, c.ntiles, yk, marks,     \
+                       c.carry_val, c.carry_row, c.carry_hit, std::min(c.n, HOTN), c.n, g);      \
   }
-#define SPMSPV_CSR5_S(SS)                                        \
-  if (!x) SPMSPV_CSR5(SS, kXBits5, true, true)                   \
-  else if (masked && hits) SPMSPV_CSR5(SS, kXEnc5, true, true)   \
-  else if (masked) SPMSPV_CSR5(SS, kXEnc5, true, false)          \
-  else if (hits) SPMSPV_CSR5(SS, kXEnc5, false, true)            \
-  else SPMSPV_CSR5(SS, kXPlain5, false, false)
+#define SPMSPV_CSR5_H(SS, XI, MK, HT)      \
+  if (lds_hot) SPMSPV_CSR5(SS, XI, MK, HT, kCsr5Hot) \
+  else SPMSPV_CSR5(SS, XI, MK, HT, 0)
+#define SPMSPV_CSR5_S(SS)                                          \
+  if (!x) SPMSPV_CSR5(SS, kXBits5, true, true, 0)                  \
+  else if (masked && hits) SPMSPV_CSR5_H(SS, kXEnc5, true, true)   \
+  else if (masked) SPMSPV_CSR5_H(SS, kXEnc5, true, false)          \
+  else if (hits) SPMSPV_CSR5_H(SS, kXEnc5, false, true)            \
+  else SPMSPV_CSR5_H(SS, kXPlain5, false, false)
   if (ev_begin) (void)hipEventRecord(ev_begin, s);
   if (c.S == 8) {
     SPMSPV_CSR5_S(8)
@@ -561,6 +648,7 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
     SPMSPV_CSR5_S(32)
   }
 #undef SPMSPV_CSR5_S
+#undef SPMSPV_CSR5_H
 #undef SPMSPV_CSR5
   if (ev_end) (void)hipEventRecord(ev_end, s);
   hipLaunchKernelGGL(k_csr5_fixup, dim3(std::min(1024, (c.ntiles + 255) / 256)), dim3(256), 0, s,

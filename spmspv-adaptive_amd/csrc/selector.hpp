@@ -20,7 +20,11 @@ static_assert((kF & (kF - 1)) == 0 && (kH1 & (kH1 - 1)) == 0 && (kH2 & (kH2 - 1)
 
 // Feature i of the selector input (0 <= i < kF). One function per feature
 // index, so that the device computes the 16 features in 16 lanes at once
-// (one double log2 per lane) and the host gets the same arithmetic.
+// and the host gets the same arithmetic. Every log2 feature is written as
+// log2(num / den + add) (den = 1, add = 0 where the formula has no division
+// or sum; both are exact then), so that the lanes pick their operands by
+// selects and share one double division and one log2 instead of running
+// their own branch each.
 __host__ __device__ inline float feature_value(int i, double m, double n, double nnz, double max_row,
                                                double avg_row, double std_row, double equality,
                                                double gini, double xnnz, double bin_len,
@@ -31,23 +35,23 @@ __host__ __device__ inline float feature_value(int i, double m, double n, double
   const double gm1 = 4.0 * (3.0 * xnnz + 3.0 * bin_len);                   // spmspv.h:2727-2731
   const double gm2 = 4.0 * ((m + 1) + 2.0 * bin_len + nnz + m);            // spmspv.h:2737-2740
   const double gm3 = 4.0 * ((m + 1) + 3.0 * nnz + m);                      // spmspv.h:2746-2749
-  double a;
+  double num, den = 1.0, add = 0.0;
   switch (i) {
-    case 0: a = m > 1 ? m : 1.0; break;
-    case 1: a = n > 1 ? n : 1.0; break;
-    case 2: a = 1.0 + nnz; break;
-    case 3: a = 1.0 + max_row; break;
-    case 4: a = 1.0 + avg_row; break;
-    case 8: a = 1.0 + xnnz; break;
-    case 9: a = 1.0 + bin_len; break;
-    case 10: a = 1.0 + max_len; break;
-    case 11: a = xnnz / (n > 0 ? n : 1.0) + 1e-7; break;
-    case 12: a = bin_len / (nnz > 0 ? nnz : 1.0) + 1e-7; break;
-    case 13: a = 1.0 + bin_len / (xnnz > 1 ? xnnz : 1.0); break;
-    case 14: a = (gm1 + 1.0) / gm3; break;
-    default: a = gm2 / gm3; break;
+    case 0: num = m > 1 ? m : 1.0; break;
+    case 1: num = n > 1 ? n : 1.0; break;
+    case 2: num = 1.0 + nnz; break;
+    case 3: num = 1.0 + max_row; break;
+    case 4: num = 1.0 + avg_row; break;
+    case 8: num = 1.0 + xnnz; break;
+    case 9: num = 1.0 + bin_len; break;
+    case 10: num = 1.0 + max_len; break;
+    case 11: num = xnnz; den = n > 0 ? n : 1.0; add = 1e-7; break;
+    case 12: num = bin_len; den = nnz > 0 ? nnz : 1.0; add = 1e-7; break;
+    case 13: num = bin_len; den = xnnz > 1 ? xnnz : 1.0; add = 1.0; break;
+    case 14: num = gm1 + 1.0; den = gm3; break;
+    default: num = gm2; den = gm3; break;
   }
-  return (float)log2(a);
+  return (float)log2(num / den + add);
 }
 
 __host__ __device__ inline void build_features(double m, double n, double nnz, double max_row,

@@ -186,6 +186,18 @@ def test_hub_column_spans_many_tiles(gpu):
             say(f"hub xnnz {key.size} slot {slot}")
             yk, yv, _ = run_slot(M, slot, key, val)
             check_result(m, cp, ri, cv, key, val, yk, yv, f"hub slot {slot}")
+    # the CSR5 tiles: row 123 runs through 59 tiles of 512 items at sigma 8
+    # (its carries are joined in several batches), 20 at sigma 24
+    for sigma in (8, 24):
+        M.A.set_sigma(sigma)
+        M.A.set_spmv_type(1)
+        for key in (np.arange(0, n, 3, dtype=np.int32), np.arange(n, dtype=np.int32)):
+            val = rng.uniform(-1, 1, key.size).astype(np.float32)
+            for slot in (sa.K_ROW_LB, sa.K_SPMV_LB):
+                say(f"hub csr5 sigma {sigma} xnnz {key.size} slot {slot}")
+                yk, yv, _ = run_slot(M, slot, key, val)
+                check_result(m, cp, ri, cv, key, val, yk, yv, f"hub csr5 s={sigma} slot {slot}")
+    M.A.set_spmv_type(0)
 
 
 def test_repeat_calls_keep_state_clean(rmat12):
