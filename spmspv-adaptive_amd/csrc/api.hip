@@ -97,7 +97,8 @@ static size_t carve(spmspv_handle_t h, char* base) {
   float* yvt = c.take<float>(m);
   float* xdi = c.take<float>(n);
   float* xen = c.take<float>(n);
-  float* xen5 = c.take<float>(n);
+  float* xen5a = c.take<float>(n);
+  float* xen5b = c.take<float>(n);
   unsigned* bi = c.take<unsigned>(nwords + 2);
   int* xki = c.take<int>(n);
   float* xvi = c.take<float>(n);
@@ -130,7 +131,9 @@ static size_t carve(spmspv_handle_t h, char* base) {
     h->yv_tmp = yvt;
     h->x_dense_int = xdi;
     h->x_enc = xen;
-    h->x_enc5 = xen5;
+    h->x_enc5b[0] = xen5a;
+    h->x_enc5b[1] = xen5b;
+    h->enc5_clean[0] = h->enc5_clean[1] = false;
     h->bits_int = bi;
     h->xk_int = xki;
     h->xv_int = xvi;
@@ -260,6 +263,20 @@ static int need_bits(spmspv_handle_t h) {
   return set_error(h, SPMSPV_VEC_FORMAT_ERR, "x is not set");
 }
 
+// The x_enc5 buffer for scattering a new x: one known to be all +0.0 if
+// there is one, otherwise the current one, filled first.
+static float* enc5_for_scatter(spmspv_handle_t h) {
+  int cur = h->enc5_cur;
+  if (!h->enc5_clean[cur] && h->enc5_clean[1 - cur]) cur = 1 - cur;
+  if (!h->enc5_clean[cur]) {
+    launch_fill_f32(h->stream, h->x_enc5b[cur], h->n, 0.f);
+    h->launches++;
+  }
+  h->enc5_cur = cur;
+  h->enc5_clean[cur] = false;
+  return h->x_enc5b[cur];
+}
+
 // Support encoding of x in the CSR5 copy's column order (x_enc itself when
 // the copy keeps the columns' order).
 static int need_enc_x(spmspv_handle_t h);
@@ -268,17 +285,25 @@ static int need_enc_x5(spmspv_handle_t h, const float** out) {
     *out = h->x_enc;
     return need_enc_x(h);
   }
-  *out = h->x_enc5;
-  if (h->enc5_valid) return SPMSPV_SUCCESS;
+  if (h->enc5_valid) {
+    *out = h->x_enc5b[h->enc5_cur];
+    return SPMSPV_SUCCESS;
+  }
   if (h->sparse_valid) {
-    launch_fill_f32(h->stream, h->x_enc5, h->n, 0.f);
-    launch_scatter(h->stream, h->x_key, h->x_val, h->xnnz, nullptr, h->n, h->x_enc5, true, Gate{},
+    float* e = enc5_for_scatter(h);
+    launch_scatter(h->stream, h->x_key, h->x_val, h->xnnz, nullptr, h->n, e, true, Gate{},
                    h->csr5.cperm);
   } else {
     int rc;
     if ((rc = need_dense_x(h)) || (rc = need_bits(h))) return rc;
-    launch_encode_x(h->stream, h->dense_x(), h->bits(), h->n, h->x_enc5, h->csr5.cperm);
+    // every entry is written: take a buffer that is not clean, if any
+    int cur = h->enc5_cur;
+    if (h->enc5_clean[cur] && !h->enc5_clean[1 - cur]) cur = 1 - cur;
+    h->enc5_cur = cur;
+    h->enc5_clean[cur] = false;
+    launch_encode_x(h->stream, h->dense_x(), h->bits(), h->n, h->x_enc5b[cur], h->csr5.cperm);
   }
+  *out = h->x_enc5b[h->enc5_cur];
   h->enc5_valid = true;
   return check_launch(h, "encode x (CSR5 order)");
 }
@@ -442,9 +467,19 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         float* vals = nullptr;
         const int* kmap = nullptr;
         int nk = 0;
+        // the idle x_enc5 buffer is cleared by the tile kernel on the way
+        const int idle = 1 - h->enc5_cur;
+        float* clr = h->csr5.cperm && xin == h->x_enc5b[h->enc5_cur] && !h->enc5_clean[idle]
+                         ? h->x_enc5b[idle]
+                         : nullptr;
         launch_csr5(s, h->csr5, masked, hits, xin, nullptr, m, h->nnz, yout, yout != h->acc, h->marks,
                     &vals, &kmap, &nk, h->time_main ? h->evk0 : nullptr,
-                    h->time_main ? h->evk1 : nullptr, h->gate);
+                    h->time_main ? h->evk1 : nullptr, h->gate, clr);
+        if (clr) {
+          // a gated launch clears only if it runs: settled with the choice
+          if (h->gate.word) h->enc5_clear_pending = idle;
+          else h->enc5_clean[idle] = true;
+        }
         h->main_timed = h->time_main;
         if (hits)
           launch_compact(s, nullptr, h->marks, nk, vals, vals == h->acc ? 1 : 0, y_key, y_val,
@@ -1102,6 +1137,7 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
     return rc;
   SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
                    h->sig_dev + 1};
+  h->enc5_clear_pending = -1;
   __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
   const int cls = size_class(h->xnnz);
   const int guess = h->spec_last[cls] - 1;
@@ -1115,12 +1151,17 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
                         (guess == SPMSPV_K_ROW_LB || guess == SPMSPV_K_SPMV_LB);
     float* enc = nullptr;
     if (row && !(order5 ? h->enc5_valid : h->enc_valid)) {
-      enc = order5 ? h->x_enc5 : h->x_enc;
-      launch_fill_f32(h->stream, enc, h->n, 0.f);
+      if (order5) {
+        enc = enc5_for_scatter(h);
+      } else {
+        enc = h->x_enc;
+        launch_fill_f32(h->stream, enc, h->n, 0.f);
+        h->launches++;
+      }
     }
     launch_select_feat(h->stream, h->csc_colptr, h->x_key, h->x_val, h->n, h->xnnz, h->sel_part, h->ctl,
                        sel, enc, order5 ? h->csr5.cperm : nullptr);
-    h->launches += enc ? 2 : 1;
+    h->launches++;
     if ((rc = check_launch(h, "selector"))) return rc;
     if (enc) (order5 ? h->enc5_valid : h->enc_valid) = true;
   } else if ((rc = run_prep(h, true, &sel))) {
@@ -1140,6 +1181,8 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
     }
   }
   long long word = 0;
+  const int clear_pending = h->enc5_clear_pending;
+  h->enc5_clear_pending = -1;
   if ((rc = wait_signal(h, 1, &word))) return rc;
   h->binlen_host = sel_word_binlen(word);
   const int kid = sel_word_choice(word);
@@ -1149,6 +1192,8 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   if (kernel_used) *kernel_used = kid;
   if (spec && kid == guess) {
     h->spec_hits++;
+    // the gated tile kernel ran, and with it the clearing of the idle buffer
+    if (clear_pending >= 0) h->enc5_clean[clear_pending] = true;
     if (y_nnz) {
       long long v = 0;
       if ((rc = wait_signal(h, 3, &v))) return rc;

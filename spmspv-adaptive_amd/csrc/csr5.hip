@@ -169,8 +169,17 @@ __global__ __launch_bounds__(csr5_nt(S, HOT)) void k_csr5(const int* __restrict_
                                               float* __restrict__ y, unsigned char* __restrict__ marks,
                                               float* __restrict__ carry_val,
                                               int* __restrict__ carry_row,
-                                              int* __restrict__ carry_hit, int hot, int xn, Gate g) {
+                                              int* __restrict__ carry_hit, int hot, int xn,
+                                              float* __restrict__ clr, Gate g) {
   if (gate_shut(g)) return;
+  if (clr) {
+    // a buffer the caller wants all +0.0 (xn floats, 16-byte aligned)
+    const int n4 = xn >> 2, gt = (int)(gridDim.x * blockDim.x);
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n4; i += gt)
+      reinterpret_cast<float4*>(clr)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int i = (n4 << 2) + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < xn) clr[i] = 0.f;
+  }
   constexpr int T = kWave * S;
   const int lanes = lane();
   const int waves = gridDim.x * (blockDim.x / kWave);
@@ -240,11 +249,12 @@ __global__ __launch_bounds__(csr5_nt(S, HOT)) void k_csr5(const int* __restrict_
       if (XIN == kXBits5) {
         xG[i] = __builtin_bit_cast(float, xbits[(unsigned)cN[i] >> 5]);
       } else if (HOT > 0) {
-        // both reads unconditional: the hot lanes send the global read to
-        // x[0] (one shared line), the others read LDS word 0 (a broadcast)
+        // both reads unconditional: the hot lanes give the global read an
+        // offset past the end of x (the buffer read returns 0 without a
+        // memory access), the others read LDS word 0 (a broadcast)
         const unsigned c = (unsigned)cN[i];
         const bool hc = (hm >> i) & 1u;
-        xG[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, (hc ? 0u : c) * 4u, 0, 0));
+        xG[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, hc ? 0xFFFFFFF0u : c * 4u, 0, 0));
         xL[i] = sx[hc ? c : 0u];
       } else {
         xG[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, (unsigned)cN[i] * 4u, 0, 0));
@@ -604,15 +614,16 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
 void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
                  const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
                  unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out,
-                 hipEvent_t ev_begin, hipEvent_t ev_end, Gate g) {
+                 hipEvent_t ev_begin, hipEvent_t ev_end, Gate g, float* clear) {
   const bool compressed = c.n_empty > 0;
   float* yk = compressed ? c.y_ne : y;  // results indexed by non-empty row
   *vals_out = yk;
   *keymap_out = compressed ? c.rows_ne : nullptr;
   *n_out = compressed ? c.n_ne : m;
-  if (m <= 0) return;
-  if (c.ntiles == 0) {
-    if (y_dense_rows) launch_fill_f32(s, y, m, 0.f, g);
+  if (m <= 0 || c.ntiles == 0) {
+    // no tile kernel: the buffer is cleared by a fill (gated alike)
+    if (clear && c.n > 0) launch_fill_f32(s, clear, c.n, 0.f, g);
+    if (m > 0 && y_dense_rows) launch_fill_f32(s, y, m, 0.f, g);
     return;
   }
   // the LDS copy holds the hot columns once they come first (column order),
@@ -626,7 +637,7 @@ void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const flo
     hipLaunchKernelGGL((k_csr5<SS, XI, MK, HT, HOTN>), dim3((waves + wpb - 1) / wpb), dim3(nt), 0, \
                        s,                                                                        \
                        c.ccol, c.cval, c.flags, c.tinfo, x, xbits, nnz, c.ntiles, yk, marks,     \
-                       c.carry_val, c.carry_row, c.carry_hit, std::min(c.n, HOTN), c.n, g);      \
+                       c.carry_val, c.carry_row, c.carry_hit, std::min(c.n, HOTN), c.n, clear, g); \
   }
 #define SPMSPV_CSR5_H(SS, XI, MK, HT)      \
   if (lds_hot) SPMSPV_CSR5(SS, XI, MK, HT, kCsr5Hot) \

@@ -41,7 +41,7 @@ struct spmspv_handle_s {
   bool bits_valid = false;    // bit array holds the current x's support
   bool prep_valid = false;    // ptr/xbeg/split/bin_len computed for the current x
   bool enc_valid = false;     // x_enc holds the support encoding of the current x
-  bool enc5_valid = false;    // x_enc5 holds it in the CSR5 column order
+  bool enc5_valid = false;    // x_enc5b[enc5_cur] holds it in the CSR5 column order
   long long binlen_host = -1; // bin_len read back for the current x (or -1)
   int spmv_type = 0;          // 0 merge (Hola role), 1 CSR5
   int sigma = -1;
@@ -68,7 +68,13 @@ struct spmspv_handle_s {
   float* yv_tmp = nullptr;   // m
   float* x_dense_int = nullptr;  // n
   float* x_enc = nullptr;        // n: +0 outside x's support, nonzero bits inside
-  float* x_enc5 = nullptr;       // n: the same in the CSR5 copy's column order
+  // n each: the same in the CSR5 copy's column order, two buffers. The
+  // tile kernel reading one clears the other (its x is no longer needed),
+  // so the next x is scattered into a buffer that is already all +0.0
+  float* x_enc5b[2] = {nullptr, nullptr};
+  int enc5_cur = 0;                      // the buffer of the current x
+  bool enc5_clean[2] = {false, false};   // buffer known to be all +0.0
+  int enc5_clear_pending = -1;           // cleared by a gated tile kernel if it runs
   unsigned* bits_int = nullptr;  // ceil(n/32)
   int* xk_int = nullptr;     // n (x converted from dense)
   float* xv_int = nullptr;   // n

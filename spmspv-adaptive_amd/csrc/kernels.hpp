@@ -197,10 +197,13 @@ hipError_t csr5_build(hipStream_t s, const int* row_ptr, const int* col, const f
 // compaction), and y is filled only when y_dense_rows (a dense output).
 // n_out = entries of that space (m or n_ne).
 // ev_begin / ev_end (nullable) are recorded around the tile kernel alone.
+// clear (nullable, c.n floats, 16-byte aligned, not x) is set to +0.0 by the
+// tile kernel as it runs.
 void launch_csr5(hipStream_t s, const Csr5& c, bool masked, bool hits, const float* x,
                  const unsigned* xbits, int m, int nnz, float* y, bool y_dense_rows,
                  unsigned char* marks, float** vals_out, const int** keymap_out, int* n_out,
-                 hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, Gate g = {});
+                 hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, Gate g = {},
+                 float* clear = nullptr);
 
 // Workgroups of `block` threads of `kernel` resident on the whole device
 // at once (occupancy x CUs), cached per kernel.

@@ -143,3 +143,61 @@ def test_edge_sizes_under_speculation(rmat12_csr5, slot):
                       f"slot {slot} xnnz {k} call {rep}")
     finally:
         A.selector_mode(1)
+
+
+def test_enc5_buffers_across_inputs(rmat12_csr5):
+    """The CSR5 slots read x through one of two encoding buffers; each tile
+    kernel clears the other one, into which the next x is scattered without
+    a fill. A sequence of fixed-slot and selector-dispatched calls (right
+    and wrong guesses), sparse and dense x, a repeated x and changed ones,
+    each checked against the FP64 oracle."""
+    M, (cp, ri, cv) = rmat12_csr5
+    A = M.A
+    n = M.n
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    dx = sa.DeviceArray(n, np.float32)
+    rng = np.random.default_rng(77)
+    # (how, slot, density): run = fixed slot; auto = selector forced to the
+    # slot (the guess is the class's previous choice); dense = dense x;
+    # again = the same x as the step before, not set again
+    plan = [("run", 7, 0.3), ("run", 5, 0.02), ("again", 7, None), ("auto", 7, 0.5),
+            ("auto", 7, 0.4), ("auto", 2, 0.45), ("auto", 5, 0.35), ("dense", 7, 0.2),
+            ("dense", 5, 0.6), ("run", 7, 1.0), ("auto", 5, 0.3), ("again", 5, None),
+            ("auto", 7, 0.33), ("run", 1, 0.01), ("run", 5, 0.05)]
+    A.set_speculation(True)
+    key = val = None
+    try:
+        for i, (how, slot, d) in enumerate(plan):
+            if how != "again":
+                k = max(1, int(round(d * n)))
+                key, val = sa.gen_sparse_vector(n, k, 7000 + i)
+            if how == "dense":
+                A.set_vector_type(1)
+                xd = np.zeros(n, np.float32)
+                xd[key] = val
+                dx.upload(xd)
+                A.set_x(dx)
+                ynnz = A.run(slot, y_key=yk, y_val=yv)
+                A.set_x(None)
+                A.set_vector_type(0)
+                ek = np.nonzero(xd)[0].astype(np.int32)
+                kk, vv = ek, xd[ek]
+            else:
+                if how != "again":
+                    dk = sa.DeviceArray.from_host(key, np.int32)
+                    dv = sa.DeviceArray.from_host(val, np.float32)
+                    A.set_sparse_x(dk, dv, key.size)
+                if how == "auto":
+                    A.selector_mode(2, slot)
+                    ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+                    assert used == slot
+                else:
+                    ynnz = A.run(slot, y_key=yk, y_val=yv)
+                kk, vv = key, val
+            print(f"step {i}: {how} slot {slot} xnnz {kk.size} ynnz {ynnz} stats {A.speculation_stats()}",
+                  flush=True)
+            check(M.m, cp, ri, cv, kk, vv, yk.to_host(ynnz), yv.to_host(ynnz), f"step {i} {how} slot {slot}")
+    finally:
+        A.selector_mode(1)
+        A.set_vector_type(0)
