@@ -8,8 +8,8 @@
 //          unvisited rows (naive team per vertex / merge-path balanced);
 //   K1/K3  top-down gather of all neighbours, in-house radix sort + unique,
 //          then claim (the next frontier comes out sorted);
-//   K4     bottom-up: each unvisited row scans its neighbours and stops at
-//          the first frontier member;
+//   K4     bottom-up: each unvisited row (one lane) scans its neighbours
+//          and stops at the first frontier member;
 //   K5/K7  merge-path row tiles testing the frontier bit array, then claim
 //          (on booleans the masked and unmasked row kernels coincide);
 //   K6     naive row scan without early exit, then claim.
@@ -34,7 +34,8 @@ struct BfsState {
   unsigned* visited = nullptr;              // n bits
   unsigned* fbits[2] = {nullptr, nullptr};  // frontier bit arrays
   int* front[2] = {nullptr, nullptr};       // frontier lists
-  int* cnt = nullptr;                       // [0],[1] sizes, [2] level
+  int* cnt = nullptr;                       // [0],[1] sizes, [2] level, [3] bottom-up queue size
+  int* queue = nullptr;                     // m: rows the bottom-up head left undecided
   unsigned long long* reached = nullptr;    // [0] vertices, [1] degree sum
   float* ybuf = nullptr;                    // m, scratch output of row kernels
   float* ones = nullptr;                    // n ones: x values of a frontier
@@ -49,6 +50,8 @@ struct BfsState {
 namespace {
 
 constexpr int kBfsGrid = 2048;
+constexpr int kBuBatch = 8;  // neighbours each row tests in the bottom-up head pass
+constexpr int kBuTeam = 8;   // lanes per queued row in the bottom-up tail pass
 constexpr int kFeatPin = 32;  // h_pin words 32..47: the next level's selector features
 
 __device__ __forceinline__ int wave_append(int* counter, bool pred) {
@@ -96,6 +99,7 @@ __global__ void k_bfs_init(int* levels, int n, unsigned* visited, unsigned* fb0,
     cnt[0] = 1;
     cnt[1] = 0;
     cnt[2] = 0;
+    cnt[3] = 0;
     reached[0] = 0;
     reached[1] = 0;
   }
@@ -108,6 +112,7 @@ __global__ void k_bfs_begin(int* cnt, int nxt, unsigned* nbits, int words, Ctl* 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[nxt] = 0;
     cnt[2] += 1;
+    cnt[3] = 0;
     ctl->epoch += 16;
   }
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x)
@@ -230,44 +235,90 @@ __global__ void k_claim_list(const int* __restrict__ keys, const Ctl* ctl, int* 
   }
 }
 
-// K4: bottom-up with early exit, a team per unvisited row.
-__global__ __launch_bounds__(256) void k_bu_naive(const int* __restrict__ row_ptr,
-                                                  const int* __restrict__ col, int m,
-                                                  const unsigned* __restrict__ fbits, int* cnt,
-                                                  int cur, int lg, unsigned* visited, int* levels,
-                                                  int* nf, unsigned* nbits) {
+// K4 as the bottom-up step of a direction-optimising BFS, in two passes.
+// Head: one lane per unvisited row tests its first B neighbours (the column
+// loads, then the frontier-bit loads, each issued together) and claims the
+// row at a hit; a row with no hit among more than B neighbours goes to a
+// queue. Tail: a team of kBuTeam lanes per queued row tests the rest, kBuTeam
+// neighbours per step, and stops at the first hit. A team per row for every
+// row keeps a whole wave on a row until each of its lanes is done (64 ms for
+// a small-frontier level of R-MAT s22 with one wave per row), and one lane
+// per row for every row leaves a wave waiting on its longest row (86 ms);
+// in R-MAT rows the hubs come first (low ids), so once the frontier holds
+// them most rows stop in the head.
+template <int B>
+__global__ __launch_bounds__(256) void k_bu_head(const int* __restrict__ row_ptr,
+                                                 const int* __restrict__ col, int m,
+                                                 const unsigned* __restrict__ fbits, int* cnt, int cur,
+                                                 unsigned* visited, int* levels, int* nf, unsigned* nbits,
+                                                 int* queue) {
   const int level = cnt[2];
   int* ncnt = cnt + (cur ^ 1);
-  const int team = 1 << lg, tpw = kWave >> lg;
-  const int tl = threadIdx.x & (team - 1);
-  const int tin = (threadIdx.x & (kWave - 1)) >> lg;
-  const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-  for (long long base = gw * tpw; base < m; base += nw * tpw) {
-    const long long r = base + tin;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < m;
+       base += (long long)gridDim.x * blockDim.x) {
+    const long long r = base + threadIdx.x;
     bool active = r < m;
     if (active) active = !((visited[r >> 5] >> (r & 31)) & 1u);
-    int b = 0, len = 0;
+    int b = 0, e = 0;
     if (active) {
       b = row_ptr[r];
-      len = row_ptr[r + 1] - b;
+      e = row_ptr[r + 1];
     }
-    const int mx = wave_max(len);
+    const bool go = active && b < e;
+    int u[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) u[j] = col[go ? min(b + j, e - 1) : 0];
+    unsigned w[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) w[j] = fbits[(unsigned)u[j] >> 5];
     bool found = false;
-    for (int k0 = 0; k0 < mx; k0 += team) {
-      const int k = k0 + tl;
-      int h = 0;
-      if (k < len && !found) {
-        const int u = col[b + k];
-        h = (fbits[u >> 5] >> (u & 31)) & 1u;
-      }
-      for (int o = team >> 1; o > 0; o >>= 1) h |= __shfl_xor(h, o, kWave);
-      found = found || h;
-      if (__ballot(!found && k0 + team < len) == 0ull) break;
-    }
-    const bool cl = active && found && tl == 0;
+#pragma unroll
+    for (int j = 0; j < B; ++j) found = found || (go && b + j < e && ((w[j] >> (u[j] & 31)) & 1u));
+    const bool cl = go && found;
     if (cl) atomicOr(visited + (r >> 5), 1u << (r & 31));
     record((int)r, cl, level, levels, nf, ncnt, nbits);
+    const bool more = go && !found && e - b > B;
+    const int q = wave_append(cnt + 3, more);
+    if (more) queue[q] = (int)r;
+  }
+}
+
+template <int B, int TEAM>
+__global__ __launch_bounds__(256) void k_bu_tail(const int* __restrict__ row_ptr,
+                                                 const int* __restrict__ col,
+                                                 const unsigned* __restrict__ fbits, int* cnt, int cur,
+                                                 unsigned* visited, int* levels, int* nf, unsigned* nbits,
+                                                 const int* __restrict__ queue) {
+  const int level = cnt[2];
+  int* ncnt = cnt + (cur ^ 1);
+  const int nq = cnt[3];
+  constexpr int TPW = kWave / TEAM;
+  const int tl = threadIdx.x & (TEAM - 1);
+  const int tin = (threadIdx.x & (kWave - 1)) / TEAM;
+  const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long q0 = gw * TPW; q0 < nq; q0 += nw * TPW) {
+    const long long qi = q0 + tin;
+    int r = 0, b = 0, e = 0;
+    if (qi < nq) {
+      r = queue[qi];
+      b = row_ptr[r] + B;
+      e = row_ptr[r + 1];
+    }
+    bool found = false;
+    // wave-uniform loop: every team of the wave steps until all are done
+    for (int k0 = b; __ballot(!found && k0 < e) != 0ull; k0 += TEAM) {
+      const int k = k0 + tl;
+      const bool ok = !found && k < e;
+      const int v = col[ok ? k : 0];
+      int h = ok ? (int)((fbits[(unsigned)v >> 5] >> (v & 31)) & 1u) : 0;
+#pragma unroll
+      for (int o = TEAM >> 1; o > 0; o >>= 1) h |= __shfl_xor(h, o, kWave);
+      found = found || h;
+    }
+    const bool cl = qi < nq && found && tl == 0;
+    if (cl) atomicOr(visited + (r >> 5), 1u << (r & 31));
+    record(r, cl, level, levels, nf, ncnt, nbits);
   }
 }
 
@@ -343,7 +394,7 @@ int bfs_alloc(spmspv_handle_t h) {
   };
   const size_t o_vis = take(words * 4), o_fb0 = take(words * 4), o_fb1 = take(words * 4);
   const size_t o_f0 = take(n * 4 + 4), o_f1 = take(n * 4 + 4), o_cnt = take(64), o_r = take(64);
-  const size_t o_y = take(m * 4 + 4), o_ones = take(n * 4 + 4);
+  const size_t o_y = take(m * 4 + 4), o_ones = take(n * 4 + 4), o_q = take(m * 4 + 4);
   if (hipMalloc(&b->alloc, off) != hipSuccess) {
     delete b;
     return set_error(h, SPMSPV_OUT_OF_MEMORY, "bfs buffers");
@@ -358,6 +409,7 @@ int bfs_alloc(spmspv_handle_t h) {
   b->reached = reinterpret_cast<unsigned long long*>(p + o_r);
   b->ybuf = reinterpret_cast<float*>(p + o_y);
   b->ones = reinterpret_cast<float*>(p + o_ones);
+  b->queue = reinterpret_cast<int*>(p + o_q);
   if (!h->stream) {
     if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
       (void)hipFree(b->alloc);
@@ -410,8 +462,12 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
       break;
     }
     case SPMSPV_K_ROW_NAIVE:
-      hipLaunchKernelGGL(k_bu_naive, dim3(kBfsGrid), dim3(256), 0, s, h->csr_row, h->csr_col, h->m,
-                         b->fbits[cur], b->cnt, cur, lg2(h->team_row), b->visited, levels, nf, nbits);
+      if (h->nnz > 0) {
+        hipLaunchKernelGGL((k_bu_head<kBuBatch>), dim3(kBfsGrid), dim3(256), 0, s, h->csr_row, h->csr_col,
+                           h->m, b->fbits[cur], b->cnt, cur, b->visited, levels, nf, nbits, b->queue);
+        hipLaunchKernelGGL((k_bu_tail<kBuBatch, kBuTeam>), dim3(kBfsGrid), dim3(256), 0, s, h->csr_row,
+                           h->csr_col, b->fbits[cur], b->cnt, cur, b->visited, levels, nf, nbits, b->queue);
+      }
       break;
     default: {  // 5, 6, 7: row kernels with hit tracking, then claim
       if (slot == SPMSPV_K_SPMV_NAIVE)
