@@ -240,6 +240,10 @@ class SpmspvHandle:
         self.h = _vp()
         _check(lib().spmspv_create(ctypes.byref(self.h), m, n, nnz))
         self.m, self.n, self.nnz = m, n, nnz
+        # The handle borrows the device arrays it is given (the C-ABI keeps
+        # only their addresses): the wrapper holds the arrays, so that they
+        # cannot be freed while the handle may still read them.
+        self._borrowed = {}
 
     def _c(self, rc):
         return _check(rc, self.h)
@@ -248,6 +252,7 @@ class SpmspvHandle:
         if self.h is not None and self.h.value:
             lib().spmspv_destroy(self.h)
         self.h = None
+        self._borrowed = {}
 
     def __del__(self):
         try:
@@ -257,10 +262,14 @@ class SpmspvHandle:
 
     # ---- set-up (spmspv.h:353-419, 1653-1763)
     def InputCSR(self, nnz, row_ptr, col, val):
-        return self._c(lib().spmspv_input_csr(self.h, nnz, row_ptr.ptr, col.ptr, val.ptr))
+        rc = self._c(lib().spmspv_input_csr(self.h, nnz, row_ptr.ptr, col.ptr, val.ptr))
+        self._borrowed["csr"] = (row_ptr, col, val)
+        return rc
 
     def InputCSC(self, nnz, row, col_ptr, val):
-        return self._c(lib().spmspv_input_csc(self.h, nnz, row.ptr, col_ptr.ptr, val.ptr))
+        rc = self._c(lib().spmspv_input_csc(self.h, nnz, row.ptr, col_ptr.ptr, val.ptr))
+        self._borrowed["csc"] = (row, col_ptr, val)
+        return rc
 
     def ToCSC(self):
         return self._c(lib().spmspv_to_csc(self.h))
@@ -296,18 +305,24 @@ class SpmspvHandle:
         return self._c(lib().spmspv_set_vector_type(self.h, code))
 
     def set_sparse_x(self, key, val, xnnz):
-        return self._c(lib().spmspv_set_sparse_x(self.h, key.ptr if key else None,
-                                                 val.ptr if val else None, xnnz))
+        rc = self._c(lib().spmspv_set_sparse_x(self.h, key.ptr if key else None,
+                                               val.ptr if val else None, xnnz))
+        self._borrowed["x"] = (key, val)
+        return rc
 
     def set_sparse_x_ptr(self, key_addr, val_addr, xnnz):
         """set_sparse_x with raw device addresses (slices of larger arrays)."""
         return self._c(lib().spmspv_set_sparse_x(self.h, _vp(key_addr), _vp(val_addr), xnnz))
 
     def set_x(self, x):
-        return self._c(lib().spmspv_set_x(self.h, x.ptr if x else None))
+        rc = self._c(lib().spmspv_set_x(self.h, x.ptr if x else None))
+        self._borrowed["dense_x"] = x
+        return rc
 
     def set_bitvector(self, bits, num_ints):
-        return self._c(lib().spmspv_set_bitvector(self.h, bits.ptr if bits else None, num_ints))
+        rc = self._c(lib().spmspv_set_bitvector(self.h, bits.ptr if bits else None, num_ints))
+        self._borrowed["bits"] = bits
+        return rc
 
     def sparse2dense(self):
         return self._c(lib().spmspv_sparse2dense(self.h))

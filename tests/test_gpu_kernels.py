@@ -438,3 +438,28 @@ def test_main_kernel_timing(rmat12):
                     M.A.main_kernel_ms()
     finally:
         M.A.time_main_kernel(False)
+
+
+def test_handle_holds_borrowed_arrays(gpu):
+    """The handle borrows the device arrays it is given; the wrapper keeps
+    them alive, so a handle taken from a Matrix that is then dropped still
+    reads valid arrays (keeping only `.A` once freed the CSR under it)."""
+    import gc
+    rp, col, val = sa.gen_rmat_csr(12, 16, seed=43)
+    n = 1 << 12
+    A = sa.Matrix(n, n, rp, col, val).A
+    gc.collect()
+    assert A._borrowed["csr"][0].ptr is not None and A._borrowed["csr"][0].ptr.value
+    cp, ri, cv = csc_of(n, n, rp, col, val)
+    key, xv = sa.gen_sparse_vector(n, 500, 17)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(xv, np.float32)
+    A.set_sparse_x(dk, dv, key.size)
+    del dk, dv
+    gc.collect()
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    A.set_spmv_type(1)
+    for slot in (sa.K_COL_LB_ATOMIC, sa.K_SPMV_LB):
+        nnz = A.run(slot, y_key=yk, y_val=yv)
+        check_result(n, cp, ri, cv, key, xv, yk.to_host(nnz), yv.to_host(nnz), f"dropped owner slot {slot}")
