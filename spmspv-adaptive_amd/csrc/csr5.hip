@@ -15,10 +15,14 @@
 //     holds the tile's first item, with bit 31 set when that item continues
 //     a row from the previous tile. The k-th marker of tile t is non-empty
 //     row rows_ne[base + k], which replaces the reference's per-tile
-//     empty_offset arrays; empty rows are written from their own list.
-// One wave per tile, no LDS and no barriers; each wave walks tiles
+//     empty_offset arrays; empty rows are written from their own list;
+//   * the copy's columns are renumbered by nonzero count and every row's
+//     items stored in that order, so that the densest columns come first.
+// One wave per tile, no barrier inside the tile loop; each wave walks tiles
 // w, w + waves, ... with the next tile's loads in flight (column indices
-// and values two tiles ahead, x gathered one tile ahead).
+// and values two tiles ahead, x gathered one tile ahead). The only LDS use
+// is a copy of the first x entries (the densest columns), made by each block
+// at its start: one block per CU then serves most gathers from LDS.
 #include <algorithm>
 
 #include "handle.hpp"
