@@ -324,7 +324,7 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
 // loads of the tile, its neighbours and its own carry are issued together:
 // one round trip to memory for the common run of one or two tiles; longer
 // runs (a row over many tiles) are read kCarryBatch tiles per round trip.
-constexpr int kCarryBatch = 8;
+constexpr int kCarryBatch = 16;
 template <typename Idx>
 __device__ __forceinline__ bool carry_run(const float* __restrict__ cv, const int* __restrict__ crow,
                                           const int* __restrict__ chit, Idx t, Idx ntiles, int& r,
