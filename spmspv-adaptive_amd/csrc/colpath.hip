@@ -352,6 +352,16 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
 void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
                         int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
                         float* enc, const int* enc_perm) {
+  // 16 keys per thread fill at most 512 workgroups up to 2 M keys; below
+  // 256 K keys, 8 per thread give twice the workgroups and half the chain
+  // per thread (3 µs less at 100 K keys, 4 µs more at 1 M)
+  if (xnnz < (1 << 18)) {
+    constexpr int kPer = 256 * 8;
+    const int grid = std::max(1, (xnnz + kPer - 1) / kPer);
+    hipLaunchKernelGGL((k_select_feat<256, 8>), dim3(grid), dim3(256), 0, s, colptr, xkey, xval, ncols,
+                       xnnz, part, ctl, sel, enc, enc_perm);
+    return;
+  }
   constexpr int kPer = 256 * 16;
   const int grid = (int)std::max<long long>(
       1, std::min<long long>(kSelMaxGrid, ((long long)xnnz + kPer - 1) / kPer));
