@@ -576,12 +576,14 @@ void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, 
 }
 
 // -------------------------------------------------------------- compact
-// 2,048 rows per tile (kCmpNT bit-array words), 8 rows per thread, ordered
+// 8,192 rows per tile (kCmpNT bit-array words), 8 rows per thread, ordered
 // tiles. The hits come either from a bit array (row kernels) or from one
 // mark byte per row (column atomic kernels); both are cleared on the way.
-// Eight rows per thread (not one word of 32) puts 256 threads on a tile, so
-// that at high hit rates each thread writes at most a few outputs, all of
-// whose loads go out in one round.
+// Eight rows per thread (not one word of 32) puts 1,024 threads on a tile,
+// so that at high hit rates each thread writes at most 8 outputs, whose
+// loads go out in one round; 8,192-row tiles keep the look-back short (a
+// tile reads 64 predecessors per round: 2,048-row tiles took up to 8 rounds
+// at a million rows).
 __device__ __forceinline__ unsigned bytes_to_bits8(uint2 a) {
   // byte k (0 or 1) of a 32-bit lane -> bit k: the four partial products of
   // x * 0x01020408 land on distinct bits, bits 24..27 hold bytes 0..3

@@ -14,7 +14,7 @@ constexpr int kPrepNT = 256, kPrepIPT = 8, kPrepTile = kPrepNT * kPrepIPT;
 // odd items per thread: a lane walking VT consecutive LDS words then hits
 // a different bank than its neighbours (stride 7 is coprime with 32 banks)
 constexpr int kLbNT = 256, kLbVT = 7, kLbTile = kLbNT * kLbVT;
-constexpr int kCmpNT = 64;  // bit-array words per compaction tile (2,048 rows, 8 per thread)
+constexpr int kCmpNT = 256;  // bit-array words per compaction tile (8,192 rows, 8 per thread)
 constexpr int kSortNT = 256, kSortIPT = 12, kSortTile = kSortNT * kSortIPT;
 constexpr int kDScanNT = 256, kDScanIPT = 8, kDScanTile = kDScanNT * kDScanIPT;
 constexpr int kRedNT = 256, kRedIPT = 7, kRedTile = kRedNT * kRedIPT;
