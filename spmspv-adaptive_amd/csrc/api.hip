@@ -304,8 +304,10 @@ static int need_enc_x5(spmspv_handle_t h, const float** out) {
     launch_encode_x(h->stream, h->dense_x(), h->bits(), h->n, h->x_enc5b[cur], h->csr5.cperm);
   }
   *out = h->x_enc5b[h->enc5_cur];
+  int rc = check_launch(h, "encode x (CSR5 order)");
+  if (rc) return rc;
   h->enc5_valid = true;
-  return check_launch(h, "encode x (CSR5 order)");
+  return SPMSPV_SUCCESS;
 }
 
 // Support encoding of x for the row-merge kernels: from the sparse form
@@ -475,17 +477,19 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         launch_csr5(s, h->csr5, masked, hits, xin, nullptr, m, h->nnz, yout, yout != h->acc, h->marks,
                     &vals, &kmap, &nk, h->time_main ? h->evk0 : nullptr,
                     h->time_main ? h->evk1 : nullptr, h->gate, clr);
-        if (clr) {
-          // a gated launch clears only if it runs: settled with the choice
-          if (h->gate.word) h->enc5_clear_pending = idle;
-          else h->enc5_clean[idle] = true;
-        }
         h->main_timed = h->time_main;
         if (hits)
           launch_compact(s, nullptr, h->marks, nk, vals, vals == h->acc ? 1 : 0, y_key, y_val,
                          h->scan_state, h->ctl, next_ep(h, 1), kmap, h->gate);
         h->launches += 3;
-        return check_launch(h, "CSR5 row kernel");
+        if ((rc = check_launch(h, "CSR5 row kernel"))) return rc;
+        if (clr) {
+          // the buffer counts as clean once the launches went in; a gated
+          // launch clears only if it runs, which the choice settles
+          if (h->gate.word) h->enc5_clear_pending = idle;
+          else h->enc5_clean[idle] = true;
+        }
+        return SPMSPV_SUCCESS;
       }
       if (lb) {
         const float* xin = nullptr;
