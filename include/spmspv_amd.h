@@ -22,8 +22,11 @@
  *  - No allocation after spmspv_alloc_pre_buffer(); no host exit(); errors
  *    come back as SPMSPV_* codes and spmspv_last_error() text.
  *  - All work is queued on the handle's stream (spmspv_set_stream). Calls
- *    that return a host scalar (y_nnz, bin_len, features) synchronise that
- *    stream, as the reference's blocking D2H reads did.
+ *    that return a host scalar (y_nnz, bin_len, features) wait for it, as
+ *    the reference's blocking D2H reads did; the results it describes are
+ *    in place by then (y_nnz may arrive while the call's last kernel still
+ *    restores the handle's scratch state, so other streams read the results
+ *    after an event or a synchronisation).
  */
 #ifndef SPMSPV_AMD_H_
 #define SPMSPV_AMD_H_

@@ -373,7 +373,16 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
   int rc = ensure_arena(h);
   if (rc) return rc;
   h->main_timed = false;  // set again below when this slot's tile kernel is timed
+  h->count_sent = false;
   hipStream_t s = h->stream;
+  // signal word 3 for the final compaction, when the caller wants the count
+  // (marked pending just before the launch that writes it)
+  auto count_sig = [&]() -> long long* {
+    if (!h->count_to_host) return nullptr;
+    __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
+    h->count_sent = true;
+    return h->sig_dev + 3;
+  };
   const bool want_sparse = y_key || y_val || want_nnz;
   const int m = h->m;
   switch (kid) {
@@ -392,7 +401,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
                          h->xnnz, nullptr, acc, h->marks, nullptr, nullptr, lb_tiles_hint(h), h->gate);
       }
       launch_compact(s, nullptr, h->marks, m, acc, y_dense ? 0 : 1, y_key, y_val, h->scan_state, h->ctl,
-                     next_ep(h, 1), nullptr, h->gate);
+                     next_ep(h, 1), nullptr, h->gate, count_sig());
       h->launches += 2;
       return check_launch(h, "column atomic kernel");
     }
@@ -480,7 +489,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         h->main_timed = h->time_main;
         if (hits)
           launch_compact(s, nullptr, h->marks, nk, vals, vals == h->acc ? 1 : 0, y_key, y_val,
-                         h->scan_state, h->ctl, next_ep(h, 1), kmap, h->gate);
+                         h->scan_state, h->ctl, next_ep(h, 1), kmap, h->gate, count_sig());
         h->launches += 3;
         if ((rc = check_launch(h, "CSR5 row kernel"))) return rc;
         if (clr) {
@@ -514,7 +523,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       }
       if (hits)
         launch_compact(s, h->bitmap, nullptr, m, yout, yout == h->acc ? 1 : 0, y_key, y_val,
-                       h->scan_state, h->ctl, next_ep(h, 1), nullptr, h->gate);
+                       h->scan_state, h->ctl, next_ep(h, 1), nullptr, h->gate, count_sig());
       h->launches += 2;
       return check_launch(h, "row kernel");
     }
@@ -611,6 +620,10 @@ int spmspv_destroy(spmspv_handle_t h) {
 
 int spmspv_set_stream(spmspv_handle_t h, void* stream) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  // the old stream's work drains first: a call may return once its count
+  // is in host memory while its last kernel still restores the handle's
+  // scratch state (marks, accumulator), which the new stream's work reads
+  if (h->stream != static_cast<hipStream_t>(stream)) HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->stream = static_cast<hipStream_t>(stream);
   return SPMSPV_SUCCESS;
 }
@@ -970,9 +983,15 @@ int spmspv_run(spmspv_handle_t h, int kid, float alpha, float* d_y_dense, int* d
                float* d_y_val, int* y_nnz) {
   (void)alpha;
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->count_to_host = y_nnz != nullptr;
   int rc = run_kernel(h, kid, d_y_dense, d_y_key, d_y_val, y_nnz != nullptr);
+  h->count_to_host = false;
   if (rc) return rc;
-  if (y_nnz) return readback_ynnz(h, y_nnz);
+  if (!y_nnz) return SPMSPV_SUCCESS;
+  if (!h->count_sent) return readback_ynnz(h, y_nnz);
+  long long v = 0;
+  if ((rc = wait_signal(h, 3, &v))) return rc;
+  *y_nnz = (int)v;
   return SPMSPV_SUCCESS;
 }
 
@@ -1177,9 +1196,12 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
       ~Ungate() { h->gate = Gate{}; }
     } ungate{h};
     h->gate = Gate{&h->ctl->kernel_choice, guess};
-    if (y_nnz) __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
-    if ((rc = run_kernel(h, guess, d_y_dense, d_y_key, d_y_val, y_nnz != nullptr))) return rc;
-    if (y_nnz) {
+    h->count_to_host = y_nnz != nullptr;
+    rc = run_kernel(h, guess, d_y_dense, d_y_key, d_y_val, y_nnz != nullptr);
+    h->count_to_host = false;
+    if (rc) return rc;
+    if (y_nnz && !h->count_sent) {
+      __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
       launch_signal_int(h->stream, &h->ctl->ynnz, h->sig_dev + 3, h->gate);
       if ((rc = check_launch(h, "result count"))) return rc;
     }

@@ -597,7 +597,8 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
                                                 float* __restrict__ acc, int reset_acc,
                                                 int* __restrict__ ykey, float* __restrict__ yval,
                                                 unsigned long long* state, Ctl* ctl, Ep ep,
-                                                const int* __restrict__ keymap, Gate g) {
+                                                const int* __restrict__ keymap, Gate g,
+                                                long long* sig) {
   if (gate_shut(g)) return;
   constexpr int kRows = 8, kTile = NT * kRows;
   static_assert(kTile == kCmpNT * 32, "a tile is kCmpNT bit-array words");
@@ -606,7 +607,17 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
   __shared__ int s_rows[kTile];
   const unsigned ntiles = (unsigned)(((long long)m + kTile - 1) / kTile);
   const unsigned t = blockIdx.x;
-  if (t >= ntiles) return;
+  if (t >= ntiles) {
+    // no rows: an empty result
+    if (ntiles == 0 && t == 0 && threadIdx.x == 0) {
+      ctl->ynnz = 0;
+      if (sig) {
+        __threadfence_system();
+        *sig = 0;
+      }
+    }
+    return;
+  }
   const unsigned epoch = ep_of(ctl, ep);
   // first row of this thread; the mark bytes are padded to 32 rows and the
   // bit array to whole words, so the 8 rows are readable whenever r0 < m
@@ -666,26 +677,53 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
     for (int u = 0; u < U; ++u) {
       if ((ok >> u) & 1u) {
         const long long o = (long long)pre + i0 + u * NT;
-        if (ykey) ykey[o] = key[u];
-        if (yval) yval[o] = v[u];
+        if (sig) {
+          // device-coherent stores: visible once acknowledged, with no
+          // write-back of this workgroup's L2 before the count goes out
+          if (ykey) __hip_atomic_store(ykey + o, key[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (yval) __hip_atomic_store(yval + o, v[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          if (ykey) ykey[o] = key[u];
+          if (yval) yval[o] = v[u];
+        }
         if (reset_acc) acc[(unsigned)r[u]] = 0.0f;
       }
     }
   }
-  if (t == ntiles - 1 && threadIdx.x == 0) ctl->ynnz = (int)(pre + btot);
+  if (t == ntiles - 1 && threadIdx.x == 0)
+    __hip_atomic_store(&ctl->ynnz, (int)(pre + btot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (sig) {
+    // the result count goes to the host word from the workgroup that
+    // finishes last, once every workgroup's output stores (device-coherent
+    // above) are acknowledged: the count never arrives ahead of the result.
+    // The cleared state (marks, accumulator) is read by later launches only,
+    // after this one has ended.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned d = __hip_atomic_fetch_add(&ctl->done[kSlotCompact], 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+      if (d == ntiles - 1) {
+        __hip_atomic_store(&ctl->done[kSlotCompact], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const long long v = __hip_atomic_load(&ctl->ynnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence_system();
+        *sig = v;
+      }
+    }
+  }
 }
 
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep, const int* keymap, Gate g) {
+                    Ep ep, const int* keymap, Gate g, long long* sig) {
   constexpr int nt = kCmpNT * 4;
   const int grid = std::max(1, (int)(((long long)m + kCmpNT * 32 - 1) / (kCmpNT * 32)));
   if (marks)
     hipLaunchKernelGGL((k_compact<nt, true>), dim3(grid), dim3(nt), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep, keymap, g);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap, g, sig);
   else
     hipLaunchKernelGGL((k_compact<nt, false>), dim3(grid), dim3(nt), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep, keymap, g);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap, g, sig);
 }
 
 // ------------------------------------------------------------ utilities

@@ -112,6 +112,11 @@ struct spmspv_handle_s {
   hipEvent_t evk0 = nullptr, evk1 = nullptr;
   bool time_main = false;
   bool main_timed = false;
+  // set by the caller around run_kernel: the slot's final compaction, where
+  // one ends the call, writes the result count to signal word 3 itself
+  // (count_sent then tells that it did)
+  bool count_to_host = false;
+  bool count_sent = false;
   long long launches = 0;
   unsigned epoch_host = 1;   // next host-side look-back epoch (bit 23 clear)
   long long dev_levels = 0;  // device-epoch sequences run (captured BFS levels)

@@ -87,10 +87,12 @@ void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, 
 // hits (bit array, or mark bytes when marks != null) -> sorted (key, val);
 // clears the hits (and acc entries when reset_acc) so the accumulator
 // returns to all-zero. With keymap, hit k is reported as key keymap[k]
-// (keymap ascending keeps the keys sorted).
+// (keymap ascending keeps the keys sorted). With sig (may be host-mapped),
+// the result count is also written to *sig once every workgroup's stores
+// are visible, as launch_signal_int after this launch would.
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep, const int* keymap = nullptr, Gate g = {});
+                    Ep ep, const int* keymap = nullptr, Gate g = {}, long long* sig = nullptr);
 // bitmap of (v != 0) over a dense vector (whole words, no atomics)
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
 void launch_fill_f32(hipStream_t s, float* p, long long n, float v, Gate g = {});

@@ -201,3 +201,55 @@ def test_enc5_buffers_across_inputs(rmat12_csr5):
     finally:
         A.selector_mode(1)
         A.set_vector_type(0)
+
+
+@pytest.mark.parametrize("slot", range(9))
+def test_result_count_paths(rmat12_csr5, slot):
+    """The result count reaches the host from the compaction that ends the
+    call (slots 0, 2 and 4-7) or from a one-thread launch after the call
+    (the sorting slots 1, 3 and the bucket slot 8): with the sorted output,
+    with the count alone, with a dense output, through run and, for the
+    selector's eight classes (the bucket slot is not one), through run_auto
+    (its second call of the size class takes the speculative path), each
+    against the FP64 oracle."""
+    M, (cp, ri, cv) = rmat12_csr5
+    A = M.A
+    n = M.n
+    key, val = sa.gen_sparse_vector(n, 400, 6100 + slot)
+    ek, ev, eb = oracle.spmspv_f64(M.m, cp, ri, cv, key, val)
+    ref = np.zeros(M.m)
+    ref[ek] = ev
+    bound = np.zeros(M.m)
+    bound[ek] = eb
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    yd = sa.DeviceArray(M.m, np.float32)
+
+    def dense_ok(tag):
+        d = yd.to_host()
+        assert np.all(np.abs(d - ref) <= TOL * bound + 1e-30), f"slot {slot} {tag}: dense output"
+
+    A.set_sparse_x(dk, dv, key.size)
+    print(f"slot {slot}: run", flush=True)
+    ynnz = A.run(slot, y_key=yk, y_val=yv)
+    check(M.m, cp, ri, cv, key, val, yk.to_host(ynnz), yv.to_host(ynnz), f"slot {slot} run")
+    assert A.run(slot) == ek.size, f"slot {slot}: count alone"
+    assert A.run(slot, y_dense=yd) == ek.size, f"slot {slot}: count with dense output"
+    dense_ok("run")
+    if slot == sa.K_COL_BUCKET:
+        return
+    A.set_speculation(True)
+    try:
+        A.selector_mode(2, slot)
+        for rep in range(2):
+            print(f"slot {slot}: run_auto call {rep} stats {A.speculation_stats()}", flush=True)
+            ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+            assert used == slot
+            check(M.m, cp, ri, cv, key, val, yk.to_host(ynnz), yv.to_host(ynnz), f"slot {slot} auto {rep}")
+            ynnz, used = A.run_auto(y_dense=yd)
+            assert (ynnz, used) == (ek.size, slot), f"slot {slot} auto {rep}: count with dense output"
+            dense_ok(f"auto {rep}")
+    finally:
+        A.selector_mode(1)
