@@ -62,8 +62,8 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
     if (threadIdx.x < kMF) mfr = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
   }
-  auto run_selector = [&](long long xn, long long bl, int mxl) {
-    if (!selects) return;
+  // weights and matrix features into LDS (visible after the next barrier)
+  auto stage = [&]() {
     if (sel.weights) {
 #pragma unroll
       for (int i = 0; i < kWPT; ++i) {
@@ -72,7 +72,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
       }
     }
     if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
-    __syncthreads();
+  };
+  // after a barrier that follows stage()
+  auto run_selector = [&](long long xn, long long bl, int mxl) {
+    if (!selects) return;
     SelectorArgs sl = sel;
     if (sel.weights) sl.weights = s_w;
     sl.mat = &s_mf;
@@ -90,6 +93,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
       ctl->n_tiles = 0;
       if (split) split[0] = 0;
     }
+    if (selects) stage();
     __syncthreads();
     run_selector(0, 0, 0);
     if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(0, sel.mat ? ctl->kernel_choice : -1);
@@ -130,6 +134,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
     tsum += len[j];
   }
+  // the selecting tile stages the selector's inputs now, while the loads
+  // above are in, instead of after its look-back (the barrier below
+  // publishes them)
+  if (selects) stage();
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
   // a single tile (small x) needs neither the look-back nor the tile max/min
@@ -286,6 +294,18 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
     }
     pending = enc != nullptr;
   }
+  // the selector's inputs into LDS in every workgroup, any of which may be
+  // the last (the reduction's barriers publish them)
+  if (selects) {
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        if (idx < kNWeights) s_w[wlds_index(idx)] = wr[i];
+      }
+    }
+    if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
+  }
   sum = block_sum<NT, long long>(sum, s_l);
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
@@ -332,15 +352,6 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
     if (pending) scatter();
     return;
   }
-  if (sel.weights) {
-#pragma unroll
-    for (int i = 0; i < kWPT; ++i) {
-      const int idx = threadIdx.x + i * NT;
-      if (idx < kNWeights) s_w[wlds_index(idx)] = wr[i];
-    }
-  }
-  if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
-  __syncthreads();
   SelectorArgs sl = sel;
   if (sel.weights) sl.weights = s_w;
   sl.mat = &s_mf;
