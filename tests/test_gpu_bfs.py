@@ -55,3 +55,29 @@ def test_bfs_disconnected_and_path(gpu):
             M.A.bfs(src, lev, policy=policy, use_graphs=1)
             ref, _ = oracle.bfs(n, cp, ri, src)
             np.testing.assert_array_equal(lev.to_host(), ref)
+
+
+def test_bfs_rmat22_full_size(gpu):
+    """The BFS workload at full size: R-MAT scale 22, edge factor 16,
+    symmetrised (4,194,304 vertices, about 128 M stored edges), the first 4
+    of the 64 sources the benchmark draws, levels bit-exact against the CPU
+    BFS for the learned per-level policy (with and without captured levels)
+    and two fixed slots."""
+    import os
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "spmspv-adaptive_amd")
+    n = 1 << 22
+    rp, col, val = sa.gen_rmat_csr(22, 16, 0.57, 0.19, 0.19, seed=0x5EED0004)
+    M = sa.Matrix(n, n, rp, col, val)
+    M.A.selector_set_weights(np.load(os.path.join(root, "selector_weights.npy")).astype(np.float32))
+    M.A.bfs_selector_set_weights(np.load(os.path.join(root, "bfs_selector_weights.npy")).astype(np.float32))
+    deg = np.diff(rp)
+    cand = np.nonzero(deg > 0)[0]
+    srcs = np.random.default_rng(0x5EED0003).choice(cand, size=64, replace=False)[:4]
+    lev = sa.DeviceArray(n, np.int32)
+    for src in srcs:
+        ref, nl = oracle.bfs(n, rp, col, int(src))  # symmetric: the CSR is the CSC
+        for policy, graphs in ((-1, 1), (-1, 0), (sa.K_ROW_LB, 1), (sa.K_COL_LB_SORT, 1)):
+            print(f"rmat22 bfs src {src} policy {policy} graphs {graphs}", flush=True)
+            st = M.A.bfs(int(src), lev, policy=policy, use_graphs=graphs)
+            np.testing.assert_array_equal(lev.to_host(), ref, err_msg=f"src {src} policy {policy}")
+            assert int(st[0]) == nl
