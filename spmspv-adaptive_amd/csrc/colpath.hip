@@ -709,16 +709,22 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
     // above) are acknowledged: the count never arrives ahead of the result.
     // The cleared state (marks, accumulator) is read by later launches only,
     // after this one has ended.
+    // One 64-bit atomic per workgroup: an arrival in the high half and,
+    // from the last tile, the count in the low half (below 2^31: no carry),
+    // so the workgroup that completes the arrivals also holds the count.
+    static_assert(offsetof(Ctl, done) % 8 == 0 && kSlotCount % 2 == 0, "64-bit count word");
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
-      const unsigned d = __hip_atomic_fetch_add(&ctl->done[kSlotCompact], 1u, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-      if (d == ntiles - 1) {
-        __hip_atomic_store(&ctl->done[kSlotCompact], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const long long v = __hip_atomic_load(&ctl->ynnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      auto* word = reinterpret_cast<unsigned long long*>(&ctl->done[kSlotCount]);
+      const unsigned long long mine =
+          (1ull << 32) | (t == ntiles - 1 ? (unsigned long long)(pre + btot) : 0ull);
+      const unsigned long long now =
+          __hip_atomic_fetch_add(word, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + mine;
+      if ((unsigned)(now >> 32) == ntiles) {
+        __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence_system();
-        *sig = v;
+        *sig = (long long)(unsigned)now;
       }
     }
   }

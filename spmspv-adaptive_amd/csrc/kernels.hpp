@@ -26,7 +26,10 @@ constexpr int kPersistGrid = 1024;        // workgroups of a persistent launch
 
 // Ticket slots in Ctl (one per persistent kernel kind).
 enum Slot { kSlotPrep = 0, kSlotCompact, kSlotHist, kSlotSort, kSlotReduce, kSlotMat, kSlotBfs,
-            kSlotBatch, kSlotBucket, kSlotSelect };
+            kSlotBatch, kSlotBucket, kSlotSelect,
+            // done[10..11] as one 64-bit word: the compaction's arrivals
+            // (high half) and result count (low half)
+            kSlotCount = 10 };
 constexpr int kSelMaxGrid = 512;  // workgroups of the features-only selector launch
 
 struct MatFeat {  // device-resident matrix features (computeMatFeture_serial)
