@@ -19,8 +19,15 @@
  *    dropped exact zeros, spmspv.h:1882-1928, while its sort kernels kept
  *    them). All kernels therefore return identical index sets.
  *  - alpha is accepted and ignored, as in the reference (csr5_spmspv.h:34).
- *  - No allocation after spmspv_alloc_pre_buffer(); no host exit(); errors
- *    come back as SPMSPV_* codes and spmspv_last_error() text.
+ *  - No allocation after spmspv_alloc_pre_buffer(): it reserves the arena,
+ *    the CSR5 copy at its largest width, the BFS buffers (square matrices)
+ *    and the selector stash of a 4,096-vector batch. The exceptions are
+ *    stated where they apply: spmspv_to_csc (handle-owned CSC, once per
+ *    matrix), a spmspv_run_batched call of more than 4,096 vectors (grows
+ *    the stash once) and spmspv_flush_cache (a measurement helper). Without
+ *    alloc_pre_buffer, the first call that needs a buffer allocates it.
+ *  - No host exit(); errors come back as SPMSPV_* codes and
+ *    spmspv_last_error() text.
  *  - All work is queued on the handle's stream (spmspv_set_stream). Calls
  *    that return a host scalar (y_nnz, bin_len, features) wait for it, as
  *    the reference's blocking D2H reads did; the results it describes are
@@ -182,6 +189,9 @@ int spmspv_csc_nosort_merge_keyval(spmspv_handle_t h, int is_binlen_computed, fl
                                    int* ynnz, float* d_y, int* d_y_key, float* d_y_val);
 int spmspv_csc_sort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
                           float* d_y, int* d_y_key, float* d_y_val);
+/* CscBasedBucketSpmspv (csc_bucket_spmspv.cuh:766-1086): buckets = number
+ * of row buckets, a power of two in [2, 256]; 0 keeps the handle's current
+ * count (256, or the autotuner's pick). Other values: INVALID_ARGUMENT. */
 int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* d_y_key, float* d_y_val,
                       int buckets);
 int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y);
@@ -312,6 +322,37 @@ int spmspv_time_main_kernel(spmspv_handle_t h, int on);
 int spmspv_main_kernel_ms(spmspv_handle_t h, float* ms);
 /* Writes `bytes` through a scratch buffer to evict the 256 MB MALL. */
 int spmspv_flush_cache(spmspv_handle_t h, size_t bytes);
+
+/* ---- matrix and frontier files (host; la/include/spmspv/mtx.hpp and
+ * readsvmdata.hpp) -------------------------------------------------------- */
+/* ReadMTX + SortMTX (mtx.hpp:218-402) and the harness's coo2csr
+ * (la/test/new_test_app.cu:547-559): a MatrixMarket coordinate file as CSR
+ * with rows' entries in column order. Pattern and complex entries read as
+ * 1.0; a "symmetric" file is mirrored (off-diagonal entries twice, same
+ * value); other symmetry tags read as general, as in the reference;
+ * duplicates are kept. Call with row_ptr == NULL to get m, n and nnz first;
+ * then row_ptr[m+1], col[nnz], val[nnz] (host) are filled. Out-of-range
+ * indices, truncated entries and dense (array) files are
+ * SPMSPV_MAT_FORMAT_ERR (the reference read past them). */
+int spmspv_mtx_read_csr(const char* path, int* m, int* n, long long* nnz, int* row_ptr, int* col,
+                        float* val);
+/* loadCSR_header (mtx.hpp:750-773): sizes from a binary CSR file (24-byte
+ * header: "Hola\1CSR\1", 3 padding bytes, int32 rows, columns, entries). */
+int spmspv_csr_load_header(const char* path, int* m, int* n, int* nnz);
+/* loadCSR (mtx.hpp:775-804): values, column indices and row pointers
+ * (host arrays) of a binary CSR file whose header matches m, n, nnz. */
+int spmspv_csr_load(const char* path, int m, int n, int nnz, int* row_ptr, int* col, float* val);
+/* storeCSR (mtx.hpp:806-819): writes that file (padding bytes zero). */
+int spmspv_csr_store(const char* path, int m, int n, int nnz, const int* row_ptr, const int* col,
+                     const float* val);
+/* readNNZXFromFile + readSparseXInxFromFile (readsvmdata.hpp:223-258): the
+ * x size of every iteration of a recorded BFS / PageRank run (whitespace
+ * separated integers). sizes == NULL: *count only. */
+int spmspv_frontier_sizes(const char* path, int* count, int* sizes);
+/* extractSparseXfromFile (readsvmdata.hpp:308-348): the keys on line `line`
+ * (0-based) of a frontier file, every value 1.0; at most cap keys, else
+ * SPMSPV_INVALID_ARGUMENT. A line past the end gives xnnz = 0. */
+int spmspv_frontier_read(const char* path, int line, int cap, int* xnnz, int* keys, float* vals);
 
 /* ---- synthetic inputs (host; the reference read .mtx files) ------------- */
 /* Square n x n CSC with exactly k distinct rows per column (Floyd sampling,

@@ -188,7 +188,7 @@ int ensure_csr5(spmspv_handle_t h) {
   int rc = ensure_arena(h);
   if (rc) return rc;
   const size_t bytes = csr5_bytes(h->m, h->n, h->nnz, S);
-  if (h->csr5_bytes < bytes) {
+  if (h->csr5_bytes < bytes) {  // not reserved by alloc_pre_buffer
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     if (h->csr5_mem) HIP_TRY(h, hipFree(h->csr5_mem));
     h->csr5_mem = nullptr;
@@ -198,7 +198,8 @@ int ensure_csr5(spmspv_handle_t h) {
   }
   // columns ordered by nonzero count for the gathers' cache locality (the
   // sort buffers hold nnz items, so the order needs n <= nnz)
-  const ColumnOrderArgs order{h->sb, (long long)h->nnz, h->ctl, h->n_dev + 3, next_ep(h, 12), h->csc_colptr};
+  const ColumnOrderArgs order{h->sb, (long long)h->nnz, h->ctl, h->n_dev + 3, next_ep(h, 12),
+                              h->csc_stale ? nullptr : h->csc_colptr};
   HIP_TRY(h, csr5_build(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->n, h->nnz, S, h->csr5_mem,
                         h->h_pin, &order, &h->csr5));
   h->enc5_valid = false;
@@ -228,7 +229,7 @@ int ensure_mat_features(spmspv_handle_t h) {
 }
 
 int need_csc(spmspv_handle_t h) {
-  if (!h->csc_colptr) {
+  if (!h->csc_colptr || h->csc_stale) {
     if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "no matrix input");
     int rc = spmspv_to_csc(h);
     if (rc) return rc;
@@ -444,7 +445,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       int* ok = y_key ? y_key : h->yk_tmp;
       float* ov = y_val ? y_val : h->yv_tmp;
       if ((rc = ensure_binlen_host(h))) return rc;
-      launch_bucket(s, h->sb, m, h->ctl, h->acc, h->marks, h->binlen_host, h->nnz, next_ep(h, 1));
+      launch_bucket(s, h->sb, m, h->bucket_bits, h->ctl, h->acc, h->marks, h->binlen_host, h->nnz, next_ep(h, 1));
       launch_compact(s, nullptr, h->marks, m, h->acc, 1, ok, ov, h->scan_state, h->ctl, next_ep(h, 1));
       h->launches += 6;
       if (y_dense) {
@@ -585,7 +586,6 @@ int spmspv_create(spmspv_handle_t* out, int m, int n, int nnz) {
   if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&h->sig_dev, h->sig, 0);
   if (e != hipSuccess) {
     if (h->h_pin) (void)hipHostFree(h->h_pin);
-  if (h->sig) (void)hipHostFree(h->sig);
     if (h->sig) (void)hipHostFree(h->sig);
     delete h;
     return set_error(nullptr, SPMSPV_DEVICE_ERROR, std::string("pinned buffer: ") + hipGetErrorString(e));
@@ -610,6 +610,7 @@ int spmspv_destroy(spmspv_handle_t h) {
   if (h->batch_stash) (void)hipFree(h->batch_stash);
   if (h->csr5_mem) (void)hipFree(h->csr5_mem);
   if (h->h_pin) (void)hipHostFree(h->h_pin);
+  if (h->sig) (void)hipHostFree(h->sig);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->evk0) (void)hipEventDestroy(h->evk0);
@@ -638,12 +639,19 @@ int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* 
   if (!h || !row_ptr || (nnz > 0 && (!col || !val)))
     return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSR: null argument");
   if (nnz != h->nnz) return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSR: nnz differs from handle");
+  (void)hipStreamSynchronize(h->stream);
+  bfs_drop_graphs(h);
   h->csr_row = row_ptr;
   h->csr_col = col;
   h->csr_val = val;
   h->row_ready = false;
   h->csr5_ready = false;
   h->mat_ready = false;
+  // a CSC this handle transposed from the previous CSR no longer matches
+  if (h->csc_owned) h->csc_stale = true;
+  h->prep_valid = false;
+  h->binlen_host = -1;
+  h->enc5_valid = false;
   return SPMSPV_SUCCESS;
 }
 
@@ -651,6 +659,8 @@ int spmspv_input_csc(spmspv_handle_t h, int nnz, int* row, int* col_ptr, float* 
   if (!h || !col_ptr || (nnz > 0 && (!row || !val)))
     return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSC: null argument");
   if (nnz != h->nnz) return set_error(h, SPMSPV_INVALID_ARGUMENT, "InputCSC: nnz differs from handle");
+  (void)hipStreamSynchronize(h->stream);
+  bfs_drop_graphs(h);
   if (h->csc_alloc) {
     (void)hipFree(h->csc_alloc);
     h->csc_alloc = nullptr;
@@ -659,7 +669,10 @@ int spmspv_input_csc(spmspv_handle_t h, int nnz, int* row, int* col_ptr, float* 
   h->csc_colptr = col_ptr;
   h->csc_val = val;
   h->csc_owned = false;
+  h->csc_stale = false;
   h->mat_ready = false;
+  h->prep_valid = false;
+  h->binlen_host = -1;
   return SPMSPV_SUCCESS;
 }
 
@@ -679,12 +692,16 @@ int spmspv_to_csc(spmspv_handle_t h) {
     h->csc_val = reinterpret_cast<float*>(p + a + b);
     h->csc_owned = true;
   }
+  bfs_drop_graphs(h);
+  h->prep_valid = false;
+  h->binlen_host = -1;
   HIP_TRY(h, launch_csr_to_csc_sorted(h->stream, h->csr_row, h->csr_col, h->csr_val, h->m, h->n, h->nnz,
                            h->csc_colptr, h->csc_row, h->csc_val, h->sb, h->n_dev, h->ctl, next_ep(h, 4)));
   h->launches += 6;
   rc = check_launch(h, "ToCSC");
   if (rc) return rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->csc_stale = false;
   return SPMSPV_SUCCESS;
 }
 
@@ -692,6 +709,31 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   int rc = ensure_arena(h);
   if (rc) return rc;
+  // Everything a later call could allocate is reserved here: the CSR5 copy
+  // at its largest width, the BFS buffers and a batch stash of
+  // kBatchReserve vectors.
+  if (h->csr_row) {
+    size_t most = 0;
+    for (int S = 1; S <= 32; ++S) most = std::max(most, csr5_bytes(h->m, h->n, h->nnz, S));
+    if (h->csr5_bytes < most) {
+      HIP_TRY(h, hipStreamSynchronize(h->stream));
+      if (h->csr5_mem) HIP_TRY(h, hipFree(h->csr5_mem));
+      h->csr5_mem = nullptr;
+      h->csr5_bytes = 0;
+      h->csr5_ready = false;
+      HIP_TRY(h, hipMalloc(&h->csr5_mem, most));
+      h->csr5_bytes = most;
+    }
+  }
+  if (h->m == h->n && (rc = bfs_alloc(h))) return rc;
+  if (h->batch_cap < spmspv_handle_s::kBatchReserve) {
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->batch_stash) HIP_TRY(h, hipFree(h->batch_stash));
+    h->batch_stash = nullptr;
+    h->batch_cap = 0;
+    HIP_TRY(h, hipMalloc(&h->batch_stash, sizeof(long long) * spmspv_handle_s::kBatchReserve));
+    h->batch_cap = spmspv_handle_s::kBatchReserve;
+  }
   if (h->csr_row && (rc = ensure_row_tiles(h))) return rc;
   if ((h->csr_row || h->csc_colptr) && (rc = ensure_mat_features(h))) return rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -701,6 +743,7 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
 int spmspv_dealloc_pre_buffer(spmspv_handle_t h) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   (void)hipStreamSynchronize(h->stream);
+  bfs_drop_graphs(h);
   if (h->arena) (void)hipFree(h->arena);
   h->arena = nullptr;
   h->reserved = false;
@@ -860,7 +903,9 @@ int spmspv_dense2sparse(spmspv_handle_t h) {
   if (rc) return rc;
   int* k = h->x_key ? h->x_key : h->xk_int;
   float* v = h->x_val ? h->x_val : h->xv_int;
-  unsigned* b = h->bits();
+  // the handle's own bit array as scratch (the compaction clears it): a
+  // bit array the caller lent with set_bitvector keeps its contents
+  unsigned* b = h->bits_int;
   launch_bitmap_from_dense(h->stream, h->dense_x(), h->n, b);
   launch_compact(h->stream, b, nullptr, h->n, h->dense_x(), 0, k, v, h->scan_state, h->ctl,
                  next_ep(h, 1));
@@ -875,7 +920,7 @@ int spmspv_dense2sparse(spmspv_handle_t h) {
   h->x_val = v;
   h->xnnz = cnt;
   h->sparse_valid = true;
-  h->bits_valid = false;
+  if (!h->bits_user) h->bits_valid = false;  // the scratch above was the internal array
   h->enc_valid = h->enc5_valid = false;
   h->prep_valid = false;
   h->binlen_host = -1;
@@ -1027,7 +1072,14 @@ int spmspv_csc_sort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha
   return spmspv_run(h, SPMSPV_K_COL_LB_SORT, alpha, d_y, k, v, ynnz);
 }
 int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* k, float* v, int buckets) {
-  (void)buckets;
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (buckets != 0) {
+    int bits = 1;
+    while (bits < 8 && (1 << bits) < buckets) ++bits;
+    if (buckets < 2 || buckets > 256 || (1 << bits) != buckets)
+      return set_error(h, SPMSPV_INVALID_ARGUMENT, "buckets must be 0 or a power of two in [2, 256]");
+    h->bucket_bits = bits;
+  }
   return spmspv_run(h, SPMSPV_K_COL_BUCKET, alpha, nullptr, k, v, ynnz);
 }
 int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y) {

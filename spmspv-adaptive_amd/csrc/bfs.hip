@@ -382,49 +382,6 @@ int lg2(int t) {
   return l;
 }
 
-int bfs_alloc(spmspv_handle_t h) {
-  if (h->bfs) return SPMSPV_SUCCESS;
-  auto* b = new BfsState();
-  const size_t n = h->n, m = h->m, words = (std::max(n, m) + 31) / 32 + 2;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    const size_t o = off;
-    off = (off + bytes + 255) / 256 * 256;
-    return o;
-  };
-  const size_t o_vis = take(words * 4), o_fb0 = take(words * 4), o_fb1 = take(words * 4);
-  const size_t o_f0 = take(n * 4 + 4), o_f1 = take(n * 4 + 4), o_cnt = take(64), o_r = take(64);
-  const size_t o_y = take(m * 4 + 4), o_ones = take(n * 4 + 4), o_q = take(m * 4 + 4);
-  if (hipMalloc(&b->alloc, off) != hipSuccess) {
-    delete b;
-    return set_error(h, SPMSPV_OUT_OF_MEMORY, "bfs buffers");
-  }
-  char* p = static_cast<char*>(b->alloc);
-  b->visited = reinterpret_cast<unsigned*>(p + o_vis);
-  b->fbits[0] = reinterpret_cast<unsigned*>(p + o_fb0);
-  b->fbits[1] = reinterpret_cast<unsigned*>(p + o_fb1);
-  b->front[0] = reinterpret_cast<int*>(p + o_f0);
-  b->front[1] = reinterpret_cast<int*>(p + o_f1);
-  b->cnt = reinterpret_cast<int*>(p + o_cnt);
-  b->reached = reinterpret_cast<unsigned long long*>(p + o_r);
-  b->ybuf = reinterpret_cast<float*>(p + o_y);
-  b->ones = reinterpret_cast<float*>(p + o_ones);
-  b->queue = reinterpret_cast<int*>(p + o_q);
-  if (!h->stream) {
-    if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
-      (void)hipFree(b->alloc);
-      delete b;
-      return set_error(h, SPMSPV_DEVICE_ERROR, "bfs stream");
-    }
-  }
-  hipStream_t s = h->stream ? h->stream : b->own_stream;
-  launch_fill_f32(s, b->ones, (long long)n, 1.0f);
-  launch_fill_f32(s, b->ybuf, (long long)m, 0.0f);
-  if (hipStreamSynchronize(s) != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, "bfs init");
-  h->bfs = b;
-  return SPMSPV_SUCCESS;
-}
-
 // Enqueues one level on stream s: begin, the slot's kernels on frontier
 // `cur`, the selector on the new frontier, and the readback of
 // (new frontier size, next slot) into pinned host memory.
@@ -491,6 +448,66 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
 
 }  // namespace
 
+namespace spmspv {
+// BFS buffers (visited and frontier bit arrays, frontier lists, the
+// bottom-up queue): sized by m and n, so allocated once per handle.
+int bfs_alloc(spmspv_handle_t h) {
+  if (h->bfs) return SPMSPV_SUCCESS;
+  auto* b = new BfsState();
+  const size_t n = h->n, m = h->m, words = (std::max(n, m) + 31) / 32 + 2;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = (off + bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_vis = take(words * 4), o_fb0 = take(words * 4), o_fb1 = take(words * 4);
+  const size_t o_f0 = take(n * 4 + 4), o_f1 = take(n * 4 + 4), o_cnt = take(64), o_r = take(64);
+  const size_t o_y = take(m * 4 + 4), o_ones = take(n * 4 + 4), o_q = take(m * 4 + 4);
+  if (hipMalloc(&b->alloc, off) != hipSuccess) {
+    delete b;
+    return set_error(h, SPMSPV_OUT_OF_MEMORY, "bfs buffers");
+  }
+  char* p = static_cast<char*>(b->alloc);
+  b->visited = reinterpret_cast<unsigned*>(p + o_vis);
+  b->fbits[0] = reinterpret_cast<unsigned*>(p + o_fb0);
+  b->fbits[1] = reinterpret_cast<unsigned*>(p + o_fb1);
+  b->front[0] = reinterpret_cast<int*>(p + o_f0);
+  b->front[1] = reinterpret_cast<int*>(p + o_f1);
+  b->cnt = reinterpret_cast<int*>(p + o_cnt);
+  b->reached = reinterpret_cast<unsigned long long*>(p + o_r);
+  b->ybuf = reinterpret_cast<float*>(p + o_y);
+  b->ones = reinterpret_cast<float*>(p + o_ones);
+  b->queue = reinterpret_cast<int*>(p + o_q);
+  if (!h->stream) {
+    if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipFree(b->alloc);
+      delete b;
+      return set_error(h, SPMSPV_DEVICE_ERROR, "bfs stream");
+    }
+  }
+  hipStream_t s = h->stream ? h->stream : b->own_stream;
+  launch_fill_f32(s, b->ones, (long long)n, 1.0f);
+  launch_fill_f32(s, b->ybuf, (long long)m, 0.0f);
+  if (hipStreamSynchronize(s) != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, "bfs init");
+  h->bfs = b;
+  return SPMSPV_SUCCESS;
+}
+
+// The captured level graphs hold the matrix, arena and row-tiling pointers
+// and sizes they were captured with; every change of those drops them.
+void bfs_drop_graphs(spmspv_handle_t h) {
+  if (!h || !h->bfs) return;
+  for (auto& g : h->bfs->graph)
+    for (auto& e : g)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  h->bfs->levels_bound = nullptr;
+}
+}  // namespace spmspv
+
 extern "C" {
 
 int spmspv_bfs_release(spmspv_handle_t h) {
@@ -516,8 +533,7 @@ static int bfs_run(spmspv_handle_t h, int src, int* d_levels, int policy, int us
   if (h->m != h->n) return set_error(h, SPMSPV_INVALID_ARGUMENT, "bfs needs a square matrix");
   if (!h->csr_row) return set_error(h, SPMSPV_MAT_FORMAT_ERR, "bfs needs CSR input");
   int rc;
-  if ((rc = ensure_arena(h))) return rc;
-  if (!h->csc_colptr && (rc = spmspv_to_csc(h))) return rc;
+  if ((rc = ensure_arena(h)) || (rc = need_csc(h))) return rc;
   if ((rc = ensure_row_tiles(h)) || (rc = ensure_mat_features(h)) || (rc = bfs_alloc(h))) return rc;
   BfsState* b = h->bfs;
   hipStream_t s = h->stream ? h->stream : b->own_stream;
@@ -527,12 +543,7 @@ static int bfs_run(spmspv_handle_t h, int src, int* d_levels, int policy, int us
   const float* wsel = h->have_bfs_weights ? h->bfs_weights_dev : (h->have_weights ? h->weights_dev : nullptr);
   if (use_graphs && (b->levels_bound != d_levels || b->bound_mode != smode || b->bound_fixed != sfixed ||
                      b->bound_weights != wsel)) {
-    for (auto& g : b->graph)
-      for (auto& e : g)
-        if (e) {
-          (void)hipGraphExecDestroy(e);
-          e = nullptr;
-        }
+    bfs_drop_graphs(h);
     b->levels_bound = d_levels;
     b->bound_mode = smode;
     b->bound_fixed = sfixed;

@@ -6,8 +6,9 @@
 // (:684). Here:
 //   1. merge-path load-balanced gather writes every product (row, a*x);
 //   2. one stable counted partition pass (per-tile digit counts, their
-//      scan, scatter) on the top 8 bits of the row groups the products into
-//      256 row buckets (the count/scan/extract);
+//      scan, scatter) on the top bucket_bits bits of the row groups the
+//      products into 2^bucket_bits row buckets, 256 unless the caller or the
+//      autotuner chose fewer (the count/scan/extract);
 //   3. the bucketed products are cut into fixed chunks of 4,096 (so a
 //      bucket holding most products -- R-MAT hub rows -- is spread over many
 //      workgroups): a chunk sums its products per row in an LDS window over
@@ -92,19 +93,20 @@ __global__ __launch_bounds__(NT) void k_bucket_chunk(const unsigned* __restrict_
   }
 }
 
-// Bucket geometry for m rows: 256 buckets over the row range, windows of
-// at most 8192 rows inside each bucket.
-static void bucket_geometry(int m, int* bshift, int* wbits) {
+// Bucket geometry for m rows: 2^bucket_bits buckets (1..8 bits, the
+// partition pass's one 8-bit digit) over the row range, windows of at most
+// 8192 rows inside each bucket.
+static void bucket_geometry(int m, int bucket_bits, int* bshift, int* wbits) {
   int bits = 1;
   while (bits < 31 && (1u << bits) < (unsigned)m) ++bits;
-  *bshift = bits > 8 ? bits - 8 : 0;
+  *bshift = bits > bucket_bits ? bits - bucket_bits : 0;
   *wbits = std::min(*bshift, kWinBits);
 }
 
-void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, float* acc, unsigned char* marks,
-                   long long binlen_hint, long long n_cap, Ep ep) {
+void launch_bucket(hipStream_t s, const SortBufs& b, int m, int bucket_bits, Ctl* ctl, float* acc,
+                   unsigned char* marks, long long binlen_hint, long long n_cap, Ep ep) {
   int bshift, wbits;
-  bucket_geometry(m, &bshift, &wbits);
+  bucket_geometry(m, bucket_bits, &bshift, &wbits);
   const long long* n_dev = &ctl->bin_len;
   launch_partition_pass(s, b, bshift, n_dev, ctl, binlen_hint, n_cap, ep);
   const long long n = binlen_hint >= 0 ? binlen_hint : n_cap;

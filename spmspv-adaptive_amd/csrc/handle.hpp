@@ -26,6 +26,7 @@ struct spmspv_handle_s {
   int* csc_row = nullptr;
   float* csc_val = nullptr;
   bool csc_owned = false;
+  bool csc_stale = false;  // handle-owned CSC transposed from an earlier CSR
   void* csc_alloc = nullptr;
 
   // x
@@ -99,6 +100,7 @@ struct spmspv_handle_s {
   int team_col = 16;
   int team_row = 8;
   int row_vt = 7;  // row-merge items per thread (autotuned)
+  int bucket_bits = 8;  // bucket slot: 2^bucket_bits row buckets (autotuned, or set by csc_bucket)
 
   // host mirrors
   int* h_pin = nullptr;       // small pinned readback area (64 ints)
@@ -135,6 +137,8 @@ struct spmspv_handle_s {
   signed char spec_last[33] = {};
   long long spec_hits = 0, spec_misses = 0;
   int batch_cap = 0;
+  // batch stash reserved by alloc_pre_buffer; a larger batch grows it
+  static constexpr int kBatchReserve = 4096;
 
   float* dense_x() { return x_dense_user ? x_dense_user : x_dense_int; }
   unsigned* bits() { return bits_user ? bits_user : bits_int; }
@@ -158,8 +162,13 @@ int ensure_row_tiles(spmspv_handle_t h);
 int ensure_mat_features(spmspv_handle_t h);
 // Builds the CSR5 copy (sigma from set_sigma or nonzeros per row).
 int ensure_csr5(spmspv_handle_t h);
-// Builds the handle-owned CSC from the CSR when no CSC was given.
+// Builds the handle-owned CSC from the CSR when no CSC was given or the
+// CSR changed since it was built.
 int need_csc(spmspv_handle_t h);
+// Allocates the BFS buffers once per handle (square matrices).
+int bfs_alloc(spmspv_handle_t h);
+// Drops the captured BFS level graphs (they hold matrix and arena pointers).
+void bfs_drop_graphs(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);
 int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_val, bool want_nnz);
 int readback_ynnz(spmspv_handle_t h, int* y_nnz);

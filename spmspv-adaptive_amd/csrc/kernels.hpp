@@ -139,12 +139,12 @@ void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* 
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
                       long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g = {});
 
-// Bucket slot: partition the gathered bins by the top 8 row bits, then
-// fixed chunks of the bucketed products sum per row in LDS windows and add
-// each row's sum once into acc (marks set); the caller's compaction gives
-// the sorted result and clears acc and marks.
-void launch_bucket(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, float* acc, unsigned char* marks,
-                   long long binlen_hint, long long n_cap, Ep ep);
+// Bucket slot: partition the gathered bins by the top bucket_bits (1..8)
+// row bits, then fixed chunks of the bucketed products sum per row in LDS
+// windows and add each row's sum once into acc (marks set); the caller's
+// compaction gives the sorted result and clears acc and marks.
+void launch_bucket(hipStream_t s, const SortBufs& b, int m, int bucket_bits, Ctl* ctl, float* acc,
+                   unsigned char* marks, long long binlen_hint, long long n_cap, Ep ep);
 
 // --- row path ------------------------------------------------------------
 // --- CSR5 for wave64 (spmv_type 1) ----------------------------------------

@@ -31,6 +31,8 @@ K_SPMV_NAIVE = 6
 K_SPMV_LB = 7
 K_COL_BUCKET = 8
 NUM_SLOTS = 8
+# vector types (set_vector_type)
+SPMSPV_VEC_SPARSE, SPMSPV_VEC_DENSE, SPMSPV_VEC_BITARRAY = 0, 1, 2
 KERNEL_NAMES = ["col-naive-atomic", "col-naive-sort", "col-lb-atomic", "col-lb-sort",
                 "row-naive", "row-lb", "spmv-naive", "spmv-lb", "col-bucket"]
 MLP_F, MLP_H1, MLP_H2, MLP_C = 16, 64, 64, 8
@@ -139,6 +141,13 @@ _SIGS = {
     "spmspv_gen_banded_csr": [_i, _i, _u64, ctypes.POINTER(_ll), _vp, _vp, _vp],
     "spmspv_gen_blockdiag_csr": [_i, _i, _u64, ctypes.POINTER(_ll), _vp, _vp, _vp],
     "spmspv_gen_random_csr": [_i, _i, _d, _u64, ctypes.POINTER(_ll), _vp, _vp, _vp],
+    "spmspv_mtx_read_csr": [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_ll),
+                            _vp, _vp, _vp],
+    "spmspv_csr_load_header": [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "spmspv_csr_load": [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp],
+    "spmspv_csr_store": [ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp],
+    "spmspv_frontier_sizes": [ctypes.c_char_p, ctypes.POINTER(_i), _vp],
+    "spmspv_frontier_read": [ctypes.c_char_p, _i, _i, ctypes.POINTER(_i), _vp, _vp],
 }
 
 
@@ -335,6 +344,9 @@ class SpmspvHandle:
 
     def dense2bitarray(self):
         return self._c(lib().spmspv_dense2bitarray(self.h))
+
+    def bitarray2sparse(self):
+        return self._c(lib().spmspv_bitarray2sparse(self.h))
 
     def dense2sparse_outer(self, m, dense, key, val):
         c = _i(0)
@@ -538,6 +550,63 @@ def gen_random_csr(m, n, mean, seed):
     val = np.zeros(max(nnz, 1), np.float32)
     _check(lib().spmspv_gen_random_csr(m, n, mean, seed, ctypes.byref(_ll(0)), _np(rp), _np(col), _np(val)))
     return rp, col[:nnz], val[:nnz]
+
+
+# ---- matrix and frontier files (host)
+def _path(p):
+    return os.fsencode(p)
+
+
+def read_mtx(path):
+    """MatrixMarket file -> (m, n, row_ptr, col, val) with the reference's
+    ReadMTX semantics (mtx.hpp:218-402)."""
+    m, n, nnz = _i(0), _i(0), _ll(0)
+    _check(lib().spmspv_mtx_read_csr(_path(path), ctypes.byref(m), ctypes.byref(n), ctypes.byref(nnz),
+                                     None, None, None))
+    rp = np.zeros(m.value + 1, np.int32)
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    val = np.zeros(max(nnz.value, 1), np.float32)
+    _check(lib().spmspv_mtx_read_csr(_path(path), ctypes.byref(m), ctypes.byref(n), ctypes.byref(nnz),
+                                     _np(rp), _np(col), _np(val)))
+    return m.value, n.value, rp, col[: nnz.value], val[: nnz.value]
+
+
+def store_csr(path, m, n, row_ptr, col, val):
+    """storeCSR (mtx.hpp:806-819): the binary CSR cache file."""
+    row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float32)
+    _check(lib().spmspv_csr_store(_path(path), m, n, int(row_ptr[-1]), _np(row_ptr),
+                                  _np(col) if col.size else None, _np(val) if val.size else None))
+
+
+def load_csr(path):
+    """loadCSR_header + loadCSR (mtx.hpp:750-804) -> (m, n, row_ptr, col, val)."""
+    m, n, nnz = _i(0), _i(0), _i(0)
+    _check(lib().spmspv_csr_load_header(_path(path), ctypes.byref(m), ctypes.byref(n), ctypes.byref(nnz)))
+    rp = np.zeros(m.value + 1, np.int32)
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    val = np.zeros(max(nnz.value, 1), np.float32)
+    _check(lib().spmspv_csr_load(_path(path), m.value, n.value, nnz.value, _np(rp), _np(col), _np(val)))
+    return m.value, n.value, rp, col[: nnz.value], val[: nnz.value]
+
+
+def frontier_sizes(path):
+    """readNNZXFromFile + readSparseXInxFromFile (readsvmdata.hpp:223-258)."""
+    c = _i(0)
+    _check(lib().spmspv_frontier_sizes(_path(path), ctypes.byref(c), None))
+    out = np.zeros(max(c.value, 1), np.int32)
+    _check(lib().spmspv_frontier_sizes(_path(path), ctypes.byref(c), _np(out)))
+    return out[: c.value]
+
+
+def frontier_read(path, line, cap):
+    """extractSparseXfromFile (readsvmdata.hpp:308-348): (keys, values=1.0)."""
+    k = np.zeros(max(cap, 1), np.int32)
+    v = np.zeros(max(cap, 1), np.float32)
+    c = _i(0)
+    _check(lib().spmspv_frontier_read(_path(path), line, cap, ctypes.byref(c), _np(k), _np(v)))
+    return k[: c.value], v[: c.value]
 
 
 class Matrix:

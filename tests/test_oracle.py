@@ -253,3 +253,63 @@ def test_mlp_gradient_finite_differences():
     z = np.zeros(npar)
     l0, _ = oracle.mlp_loss_grad(F, H1, H2, C, z, X, labels=lab, want_grad=False)
     assert abs(l0 - np.log(C)) < 1e-12
+
+
+def test_mlp_forward_hand_network():
+    """A 3->2->3->2 network small enough to evaluate by hand. Sample 0 has
+    one dead hidden unit per layer and a negative logit, so a ReLU applied to
+    the logits, a transposed weight matrix or swapped layer offsets each
+    change the result."""
+    F, H1, H2, C = 3, 2, 3, 2
+    W1 = [1, 1, 1, 2, 0, -1]
+    b1 = [0.5, -4]
+    W2 = [1, 3, -2, 1, 0.5, 0.5]
+    b2 = [0, 1, -1]
+    W3 = [1, 2, 4, -1, 0, -2]
+    b3 = [0.25, 0]
+    p = np.array(W1 + b1 + W2 + b2 + W3 + b3, np.float64)
+    assert p.size == oracle.mlp_param_count(F, H1, H2, C)
+    X = np.array([[1, 2, -1], [0, 0, 0]], np.float64)
+    # sample 0: z1 = (2.5, -2) -> h1 = (2.5, 0); z2 = (2.5, -4, 0.25) ->
+    # h2 = (2.5, 0, 0.25); logits = (2.5 + 1 + 0.25, -2.5 - 0.5) = (3.75, -3)
+    # sample 1: h1 = relu(b1) = (0.5, 0); z2 = (0.5, 0, -0.75) ->
+    # h2 = (0.5, 0, 0); logits = (0.75, -0.5)
+    out = oracle.mlp_forward(F, H1, H2, C, p, X)
+    np.testing.assert_array_equal(out, [[3.75, -3.0], [0.75, -0.5]])
+
+
+def test_mlp_forward_affine_and_dead_regions():
+    """At the selector's own shape (16->64->64->8): with positive inputs and
+    positive first/second-layer parameters no ReLU clips, so the network is
+    the composition of its three affine maps; with a very negative second
+    bias every second-layer unit is dead and the logits are exactly b3."""
+    F, H1, H2, C, B = 16, 64, 64, 8, 6
+    rng = np.random.default_rng(7)
+    W1 = rng.uniform(0.01, 0.5, (H1, F))
+    b1 = rng.uniform(0.0, 0.2, H1)
+    W2 = rng.uniform(0.01, 0.5, (H2, H1))
+    b2 = rng.uniform(0.0, 0.2, H2)
+    W3 = rng.normal(0, 0.5, (C, H2))
+    b3 = rng.normal(0, 0.5, C)
+    X = rng.uniform(0.0, 1.0, (B, F))
+    pack = lambda w2b: np.concatenate([W1.ravel(), b1, W2.ravel(), w2b, W3.ravel(), b3])
+    out = oracle.mlp_forward(F, H1, H2, C, pack(b2), X)
+    want = ((X @ W1.T + b1) @ W2.T + b2) @ W3.T + b3
+    np.testing.assert_allclose(out, want, rtol=1e-13, atol=1e-12)
+    dead = oracle.mlp_forward(F, H1, H2, C, pack(np.full(H2, -1e6)), X)
+    np.testing.assert_array_equal(dead, np.tile(b3, (B, 1)))
+
+
+def test_mlp_forward_log_softmax_is_per_sample_loss():
+    """The per-sample loss of the finite-difference-pinned loss function at
+    label c equals -log_softmax(forward logits)[c], for every label."""
+    F, H1, H2, C, B = 16, 64, 64, 8, 4
+    rng = np.random.default_rng(11)
+    p = rng.normal(0, 0.3, oracle.mlp_param_count(F, H1, H2, C))
+    X = rng.normal(0, 1, (B, F))
+    z = oracle.mlp_forward(F, H1, H2, C, p, X)
+    lse = z.max(axis=1) + np.log(np.exp(z - z.max(axis=1, keepdims=True)).sum(axis=1))
+    for s in range(B):
+        for c in range(C):
+            loss, _ = oracle.mlp_loss_grad(F, H1, H2, C, p, X[s:s + 1], labels=np.array([c]), want_grad=False)
+            assert abs(loss - (lse[s] - z[s, c])) < 1e-12, (s, c)
