@@ -20,11 +20,13 @@
  *    them). All kernels therefore return identical index sets.
  *  - alpha is accepted and ignored, as in the reference (csr5_spmspv.h:34).
  *  - No allocation after spmspv_alloc_pre_buffer(): it reserves the arena,
- *    the CSR5 copy at its largest width, the BFS buffers (square matrices)
+ *    the CSR5 copy at its largest width, the BFS and PageRank buffers (square
+ *    matrices)
  *    and the selector stash of a 4,096-vector batch. The exceptions are
  *    stated where they apply: spmspv_to_csc (handle-owned CSC, once per
  *    matrix), a spmspv_run_batched call of more than 4,096 vectors (grows
- *    the stash once) and spmspv_flush_cache (a measurement helper). Without
+ *    the stash once), spmspv_flush_cache and spmspv_measure_ceilings
+ *    (measurement helpers). Without
  *    alloc_pre_buffer, the first call that needs a buffer allocates it.
  *  - No host exit(); errors come back as SPMSPV_* codes and
  *    spmspv_last_error() text.
@@ -288,6 +290,21 @@ int spmspv_bfs_trace(spmspv_handle_t h, int src, int* d_levels, int policy, int 
  * weights. */
 int spmspv_bfs_selector_set_weights(spmspv_handle_t h, const float* w, int count);
 
+/* ---- PageRank-style driver (the reference replays frontiers recorded from
+ * a PageRank run, la/test/test_application.cu:206-210, 252-256; this runs
+ * the iteration that produces them) ---------------------------------------- */
+/* Residual-push PageRank over the CSC (column u = u's out-edges, weights
+ * a_vu; the walk leaves u with probability a_vu / sum_v a_vu): every
+ * iteration's active set is a sparse x for one selector-dispatched SpMSpV
+ * (policy -1) or a fixed slot (0..7). damping d in (0,1); a vertex stays
+ * active while its residual exceeds eps * (1 - d) / n; at most max_iter
+ * iterations. d_rank[n] (device) receives the ranks, the solution of
+ * (I - d A W^-1) p = (1 - d)/n within sum|p - p*| <= eps * d once converged.
+ * stats (nullable) = {iterations, seconds, sum of x sizes, vertices still
+ * active}; sizes (host, nullable, max_iter ints) = x size per iteration. */
+int spmspv_pagerank(spmspv_handle_t h, float damping, float eps, int max_iter, int policy,
+                    float* d_rank, double stats[4], int* sizes);
+
 /* ---- batched multi-vector SpMSpV (no reference counterpart) ------------- */
 /* nvec sparse vectors in CSR-of-vectors form: d_vptr[nvec+1] (device),
  * d_vkey, d_vval. Vector v's sorted result goes to d_ykey/d_yval at the
@@ -320,6 +337,13 @@ int spmspv_timer_stop(spmspv_handle_t h, float* ms);
  * the last call ran a slot without one. */
 int spmspv_time_main_kernel(spmspv_handle_t h, int on);
 int spmspv_main_kernel_ms(spmspv_handle_t h, float* ms);
+/* No counterpart in the reference (measurement): the GPU's ceilings for
+ * the roofline, each launch after a 512 MiB eviction write, best of reps:
+ * out[0] streaming read GB/s (512 MiB, 16-byte nontemporal loads, 8 in
+ * flight per lane), out[1] device-to-device copy GB/s (256 MiB read + 256
+ * MiB written), out[2] random 4-byte gathers per second (in G) over an x of
+ * the handle's n columns. Allocates and frees 1 GiB of scratch. */
+int spmspv_measure_ceilings(spmspv_handle_t h, int reps, double out[3]);
 /* Writes `bytes` through a scratch buffer to evict the 256 MB MALL. */
 int spmspv_flush_cache(spmspv_handle_t h, size_t bytes);
 

@@ -229,6 +229,62 @@ int64_t oracle_scan_col_lengths(const int* col_ptr, int xnnz, const int* x_key,
   return s;
 }
 
+/* ---------------- PageRank, FP64 ---------------- */
+
+int oracle_pagerank_f64(int n, const int* col_ptr, const int* row_idx, const float* cval, double damping,
+                        double eps, int max_iter, double* rank, int* sizes) {
+  double* w = (double*)malloc(sizeof(double) * (size_t)n);
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* y = (double*)calloc((size_t)n, sizeof(double));
+  double* x = (double*)malloc(sizeof(double) * (size_t)n);
+  int* act = (int*)malloc(sizeof(int) * (size_t)n);
+  unsigned char* hit = (unsigned char*)calloc((size_t)n, 1);
+  for (int u = 0; u < n; ++u) {
+    double s = 0.0;
+    for (int k = col_ptr[u]; k < col_ptr[u + 1]; ++k) s += cval[k];
+    w[u] = s;
+    rank[u] = 0.0;
+    r[u] = (1.0 - damping) / n;
+    act[u] = u;
+  }
+  const double thr = eps * (1.0 - damping) / n;
+  int cnt = n, it = 0;
+  while (cnt > 0 && it < max_iter) {
+    if (sizes) sizes[it] = cnt;
+    for (int i = 0; i < cnt; ++i) {
+      const int u = act[i];
+      rank[u] += r[u];
+      x[i] = w[u] > 0.0 ? r[u] / w[u] : 0.0;
+      r[u] = 0.0;
+    }
+    for (int i = 0; i < cnt; ++i) {
+      const int u = act[i];
+      for (int k = col_ptr[u]; k < col_ptr[u + 1]; ++k) {
+        y[row_idx[k]] += (double)cval[k] * x[i];
+        hit[row_idx[k]] = 1;
+      }
+    }
+    int nc = 0;
+    for (int v = 0; v < n; ++v) {
+      if (!hit[v]) continue;
+      hit[v] = 0;
+      r[v] += damping * y[v];
+      y[v] = 0.0;
+      if (r[v] > thr) act[nc++] = v;
+    }
+    cnt = nc;
+    ++it;
+  }
+  for (int v = 0; v < n; ++v) rank[v] += r[v];
+  free(w);
+  free(r);
+  free(y);
+  free(x);
+  free(act);
+  free(hit);
+  return it;
+}
+
 /* ---------------- selector MLP, FP64 ---------------- */
 
 static void mlp_offsets(int F, int H1, int H2, int C, size_t off[6]) {

@@ -100,6 +100,19 @@ void oracle_gm(int64_t m, int64_t nnz, int64_t xnnz, int64_t bin_len,
 int64_t oracle_scan_col_lengths(const int* col_ptr, int xnnz, const int* x_key,
                                 int64_t* ptr);
 
+/* Residual-push PageRank in FP64 (the reference has no PageRank; its
+ * harnesses replay frontiers recorded from one, la/test/test_application.cu:
+ * 206-210, 252-256). Column u of the CSC holds u's out-edges with weights
+ * a_vu; w_u = sum_v a_vu. p = 0, r = (1-d)/n, every vertex active; per
+ * iteration, for the active set S in ascending order: p[u] += r[u],
+ * x_u = r[u]/w_u (0 when w_u = 0), r[u] = 0; y = A x column by column;
+ * r[v] += d*y[v] for every row v that a column of S reaches, and v joins
+ * the next S (ascending) if r[v] > eps*(1-d)/n. At the end p += r.
+ * rank[n] out; sizes (nullable, max_iter ints) = |S| per iteration.
+ * Returns the number of iterations run. */
+int oracle_pagerank_f64(int n, const int* col_ptr, const int* row_idx, const float* cval, double damping,
+                        double eps, int max_iter, double* rank, int* sizes);
+
 /* FP64 MLP 16->H1->H2->C with ReLU hidden layers and softmax cross-entropy
  * (the selector; the reference's selectors are sklearn trees,
  * model/1-test_dtree.py:170-196, so this has no reference counterpart).

@@ -51,6 +51,9 @@ def lib():
         L.oracle_scan_col_lengths.restype = ctypes.c_int64
         L.oracle_mlp_loss_grad.argtypes = [_c_int] * 5 + [_f64p, _f64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_mlp_loss_grad.restype = ctypes.c_double
+        L.oracle_pagerank_f64.argtypes = [_c_int, _i32p, _i32p, _f32p, ctypes.c_double, ctypes.c_double, _c_int,
+                                          _f64p, ctypes.c_void_p]
+        L.oracle_pagerank_f64.restype = _c_int
         L.oracle_mlp_forward.argtypes = [_c_int] * 5 + [_f64p, _f64p, _f64p]
         _LIB = L
     return _LIB
@@ -171,6 +174,17 @@ def scan_col_lengths(col_ptr, x_key):
     xk = _i32(x_key) if len(x_key) else np.zeros(1, np.int32)
     lib().oracle_scan_col_lengths(_i32(col_ptr), len(x_key), xk, ptr)
     return ptr
+
+
+def pagerank_f64(n, col_ptr, row_idx, cval, damping=0.85, eps=1e-4, max_iter=500):
+    """Residual-push PageRank in FP64: (rank, x size per iteration)."""
+    rank = np.zeros(n, np.float64)
+    sizes = np.zeros(max_iter, np.int32)
+    ri = _i32(row_idx) if len(row_idx) else np.zeros(1, np.int32)
+    cv = _f32(cval) if len(cval) else np.zeros(1, np.float32)
+    it = lib().oracle_pagerank_f64(n, _i32(col_ptr), ri, cv, damping, eps, max_iter, rank,
+                                   sizes.ctypes.data)
+    return rank, sizes[:it]
 
 
 def mlp_param_count(F, H1, H2, C):

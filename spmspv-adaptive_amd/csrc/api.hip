@@ -604,6 +604,7 @@ int spmspv_destroy(spmspv_handle_t h) {
   if (!h) return SPMSPV_SUCCESS;
   (void)hipStreamSynchronize(h->stream);
   spmspv_bfs_release(h);
+  pr_release(h);
   if (h->arena) (void)hipFree(h->arena);
   if (h->csc_alloc) (void)hipFree(h->csc_alloc);
   if (h->flush_buf) (void)hipFree(h->flush_buf);
@@ -710,7 +711,7 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
   int rc = ensure_arena(h);
   if (rc) return rc;
   // Everything a later call could allocate is reserved here: the CSR5 copy
-  // at its largest width, the BFS buffers and a batch stash of
+  // at its largest width, the BFS and PageRank buffers and a batch stash of
   // kBatchReserve vectors.
   if (h->csr_row) {
     size_t most = 0;
@@ -725,7 +726,7 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
       h->csr5_bytes = most;
     }
   }
-  if (h->m == h->n && (rc = bfs_alloc(h))) return rc;
+  if (h->m == h->n && ((rc = bfs_alloc(h)) || (rc = pr_alloc(h)))) return rc;
   if (h->batch_cap < spmspv_handle_s::kBatchReserve) {
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     if (h->batch_stash) HIP_TRY(h, hipFree(h->batch_stash));

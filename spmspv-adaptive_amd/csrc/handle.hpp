@@ -12,6 +12,7 @@
 #include "kernels.hpp"
 
 struct BfsState;
+struct PrState;
 
 struct spmspv_handle_s {
   int m = 0, n = 0, nnz = 0;
@@ -124,6 +125,7 @@ struct spmspv_handle_s {
   long long dev_levels = 0;  // device-epoch sequences run (captured BFS levels)
 
   BfsState* bfs = nullptr;
+  PrState* pr = nullptr;  // PageRank driver buffers
   void* csr5_mem = nullptr;          // CSR5 copy of the CSR (spmv_type 1)
   size_t csr5_bytes = 0;
   bool csr5_ready = false;
@@ -167,6 +169,9 @@ int ensure_csr5(spmspv_handle_t h);
 int need_csc(spmspv_handle_t h);
 // Allocates the BFS buffers once per handle (square matrices).
 int bfs_alloc(spmspv_handle_t h);
+// Allocates / frees the PageRank driver's buffers (square matrices).
+int pr_alloc(spmspv_handle_t h);
+void pr_release(spmspv_handle_t h);
 // Drops the captured BFS level graphs (they hold matrix and arena pointers).
 void bfs_drop_graphs(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);

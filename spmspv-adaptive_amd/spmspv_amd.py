@@ -124,6 +124,7 @@ _SIGS = {
     "spmspv_mlp_forward": [_vp, _vp, _i, _vp],
     "spmspv_bfs": [_vp, _i, _vp, _i, _i, _vp],
     "spmspv_run_batched": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "spmspv_pagerank": [_vp, _f, _f, _i, _i, _vp, _vp, _vp],
     "spmspv_device_count": [ctypes.POINTER(_i)],
     "spmspv_malloc": [ctypes.POINTER(_vp), _sz],
     "spmspv_free": [_vp],
@@ -135,6 +136,7 @@ _SIGS = {
     "spmspv_timer_start": [_vp],
     "spmspv_timer_stop": [_vp, ctypes.POINTER(_f)],
     "spmspv_flush_cache": [_vp, _sz],
+    "spmspv_measure_ceilings": [_vp, _i, _vp],
     "spmspv_gen_uniform_csc": [_i, _i, _u64, _vp, _vp, _vp],
     "spmspv_gen_rmat_csr": [_i, _i, _d, _d, _d, _u64, ctypes.POINTER(_ll), _vp, _vp, _vp],
     "spmspv_gen_sparse_vector": [_i, _i, _u64, _f, _f, _vp, _vp],
@@ -451,6 +453,14 @@ class SpmspvHandle:
         w = np.ascontiguousarray(w, np.float32)
         return self._c(lib().spmspv_bfs_selector_set_weights(self.h, _np(w), w.size))
 
+    def pagerank(self, rank, damping=0.85, eps=1e-4, max_iter=500, policy=-1):
+        """Residual-push PageRank (each iteration one SpMSpV); returns
+        (stats, x size per iteration)."""
+        st = np.zeros(4, np.float64)
+        sizes = np.zeros(max_iter, np.int32)
+        self._c(lib().spmspv_pagerank(self.h, damping, eps, max_iter, policy, rank.ptr, _np(st), _np(sizes)))
+        return st, sizes[: int(st[0])]
+
     def run_batched(self, vptr, vkey, vval, yoff, ykey, yval, ynnz):
         """Batched SpMSpV over the vectors vptr/vkey/vval (device arrays);
         results at yoff (device int64 offsets) in ykey/yval, counts in ynnz.
@@ -487,6 +497,13 @@ class SpmspvHandle:
         v = _f(0)
         self._c(lib().spmspv_timer_stop(self.h, ctypes.byref(v)))
         return v.value
+
+    def measure_ceilings(self, reps=3):
+        """{read_gbs, copy_gbs, gather_gps}: the GPU's streaming read, copy
+        and random-gather ceilings (each launch cold)."""
+        out = np.zeros(3, np.float64)
+        self._c(lib().spmspv_measure_ceilings(self.h, reps, _np(out)))
+        return {"read_gbs": float(out[0]), "copy_gbs": float(out[1]), "gather_gps": float(out[2])}
 
     def flush_cache(self, nbytes=512 << 20):
         return self._c(lib().spmspv_flush_cache(self.h, nbytes))
@@ -607,6 +624,35 @@ def frontier_read(path, line, cap):
     c = _i(0)
     _check(lib().spmspv_frontier_read(_path(path), line, cap, ctypes.byref(c), _np(k), _np(v)))
     return k[: c.value], v[: c.value]
+
+
+def replay_frontiers(A, sizes_path, x_path, y_key, y_val, slots=None):
+    """The reference harness's frontier replay (la/test/test_application.cu:
+    240-415, new_test_app.cu:280-469): iteration i takes x = the keys on
+    line i of x_path (values 1.0; at most the i-th size of sizes_path,
+    clamped to n), sets it as the handle's sparse x and runs the selector's
+    slot (slots None) or each given slot, timed with the handle's events.
+    Returns one dict per iteration: xnnz, keys, and per run (slot, ynnz, ms)."""
+    sizes = frontier_sizes(sizes_path)
+    out = []
+    for i, want in enumerate(sizes):
+        want = min(int(want), A.n)
+        keys, vals = frontier_read(x_path, i, A.n)
+        keys, vals = keys[:want], vals[:want]
+        dk = DeviceArray.from_host(keys if keys.size else np.zeros(1, np.int32), np.int32)
+        dv = DeviceArray.from_host(vals if vals.size else np.zeros(1, np.float32), np.float32)
+        runs = []
+        for slot in (slots if slots is not None else [None]):
+            A.set_sparse_x(dk, dv, int(keys.size))
+            A.timer_start()
+            if slot is None:
+                ynnz, used = A.run_auto(y_key=y_key, y_val=y_val)
+            else:
+                ynnz, used = A.run(slot, y_key=y_key, y_val=y_val), slot
+            ms = A.timer_stop()
+            runs.append({"slot": used, "ynnz": ynnz, "ms": ms})
+        out.append({"xnnz": int(keys.size), "keys": keys, "runs": runs})
+    return out
 
 
 class Matrix:

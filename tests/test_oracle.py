@@ -313,3 +313,43 @@ def test_mlp_forward_log_softmax_is_per_sample_loss():
         for c in range(C):
             loss, _ = oracle.mlp_loss_grad(F, H1, H2, C, p, X[s:s + 1], labels=np.array([c]), want_grad=False)
             assert abs(loss - (lse[s] - z[s, c])) < 1e-12, (s, c)
+
+
+# ----------------------------------------------------------- PageRank
+def _pagerank_solve(n, A_csc, d):
+    """p* = (I - d A W^-1)^-1 (1-d)/n, W = diag of column sums (0 -> 0)."""
+    import scipy.sparse.linalg as spla
+    w = np.asarray(A_csc.sum(axis=0)).ravel()
+    winv = np.where(w > 0, 1.0 / np.where(w > 0, w, 1.0), 0.0)
+    P = A_csc @ sp.diags(winv)
+    return spla.spsolve((sp.identity(n) - d * P).tocsc(), np.full(n, (1 - d) / n))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_pagerank_matches_linear_solve(seed):
+    n = 300
+    A = rand_csr(n, n, 0.02, seed).tocsc()
+    A.data = np.abs(A.data) + np.float32(0.05)   # positive weights
+    A.sort_indices()
+    want = _pagerank_solve(n, A.astype(np.float64), 0.85)
+    got, sizes = oracle.pagerank_f64(n, A.indptr, A.indices, A.data, 0.85, 1e-13, 5000)
+    assert sizes[0] == n
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-14)
+
+
+def test_pagerank_cycle_is_uniform_and_eps_bound_holds():
+    n = 64
+    # directed cycle u -> u+1: every column has one entry, p* = 1/n
+    cp = np.arange(n + 1, dtype=np.int32)
+    ri = ((np.arange(n) + 1) % n).astype(np.int32)
+    cv = np.ones(n, np.float32)
+    got, _ = oracle.pagerank_f64(n, cp, ri, cv, 0.85, 1e-12, 5000)
+    np.testing.assert_allclose(got, 1.0 / n, rtol=1e-10)
+    # a coarse eps stops early, within sum|p - p*| <= eps * d
+    A = rand_csr(200, 200, 0.03, 9, values="ones").tocsc()
+    A.sort_indices()
+    want = _pagerank_solve(200, A.astype(np.float64), 0.85)
+    for eps in (1e-1, 1e-2, 1e-3):
+        got, sizes = oracle.pagerank_f64(200, A.indptr, A.indices, A.data, 0.85, eps, 5000)
+        assert np.abs(got - want).sum() <= eps * 0.85
+        assert sizes.size > 1 and sizes[0] == 200
