@@ -318,6 +318,24 @@ int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int
                        const float* d_vval, const long long* d_yoff, int* d_ykey, float* d_yval,
                        int* d_ynnz, int* kernel_used);
 
+/* ---- no-allocation entry point ------------------------------------------ */
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). The load-balanced atomic column SpMSpV (the kernels
+ * of spmspv_run with SPMSPV_K_COL_LB_ATOMIC) on a caller's CSC
+ * (d_col_ptr[n+1], d_row[nnz], d_val[nnz]) and sparse x (d_x_key[xnnz]
+ * distinct keys in [0,n), d_x_val[xnnz]), without a handle: every scratch
+ * buffer comes from one 32 MiB array in device memory that each GPU holds
+ * until the process ends (enough for m <= 2^21 rows with xnnz <= 2^20 at any
+ * nnz; larger sizes are SPMSPV_INVALID_ARGUMENT with the bytes needed). The
+ * sorted result goes to d_y_key/d_y_val (capacity m) and its size to the
+ * device int *d_y_nnz, all queued on `stream` (NULL: the default stream);
+ * nothing is allocated and the host does not wait. Calls that overlap on
+ * one GPU share its scratch array and race. Errors: spmspv_last_error(NULL). */
+int spmspv_csc_spmspv_noalloc(int m, int n, long long nnz, const int* d_col_ptr, const int* d_row,
+                              const float* d_val, const int* d_x_key, const float* d_x_val, int xnnz,
+                              int* d_y_key, float* d_y_val, int* d_y_nnz, void* stream);
+
 /* ---- utilities (the reference harness called the CUDA runtime directly) - */
 int spmspv_device_count(int* count);
 int spmspv_malloc(void** d_ptr, size_t bytes);

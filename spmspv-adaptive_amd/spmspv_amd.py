@@ -125,6 +125,7 @@ _SIGS = {
     "spmspv_bfs": [_vp, _i, _vp, _i, _i, _vp],
     "spmspv_run_batched": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "spmspv_pagerank": [_vp, _f, _f, _i, _i, _vp, _vp, _vp],
+    "spmspv_csc_spmspv_noalloc": [_i, _i, _ll, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "spmspv_device_count": [ctypes.POINTER(_i)],
     "spmspv_malloc": [ctypes.POINTER(_vp), _sz],
     "spmspv_free": [_vp],

@@ -90,3 +90,17 @@ def test_gen_sparse_vector_and_others():
     assert rp[-1] == 32 * 32 * 2 + 6 * 6
     rp, col, val = sa.gen_random_csr(300, 200, 8.0, 4)
     assert abs(rp[-1] / 300 - 8.0) < 1.5
+
+
+def test_noalloc_rejects_oversized_input_before_any_device_call():
+    """The no-allocation entry point checks its sizes against the 32 MiB
+    scratch before it touches the device (so this runs without a GPU)."""
+    import ctypes
+    L = sa.lib()
+    dummy = (ctypes.c_int * 4)()
+    big = 1 << 24  # the accumulator alone would need 64 MiB
+    p = ctypes.cast(dummy, ctypes.c_void_p)
+    assert L.spmspv_csc_spmspv_noalloc(big, big, 0, p, None, None, None, None, 0, p, p, p, None) == -20
+    assert b"maximum" in L.spmspv_last_error(None)
+    assert L.spmspv_csc_spmspv_noalloc(0, 4, 0, p, None, None, None, None, 0, p, p, p, None) == -20
+    assert L.spmspv_csc_spmspv_noalloc(4, 4, 0, None, None, None, None, None, 0, p, p, p, None) == -20
