@@ -13,11 +13,16 @@ before it, outside the timed interval. Calls are timed with events on the
 handle's stream. `value` = algorithmic bytes of the step / step time, with
 the kernel-independent byte count B = 16*xnnz + 8*bin_len + 8*y_nnz.
 
-Also reported: every fixed slot on the same inputs (best fixed kernel,
-oracle-best per density, selector regret), the roofline of the dominant
-kernel, a warm back-to-back wall-clock rate, BFS GTEPS on the same graph,
-and the reference's CPU path (the oracle's FP32 restatement of
-SpmSpvCSCHost, one core) on a bounded sample.
+Also reported, each outside the timed steps: every fixed slot on the same
+inputs (best fixed kernel, oracle-best per density, selector regret), the
+FP32 error of every density against the FP64 oracle, speculation hits and a
+mixed-order sequence, the roofline of the dominant kernel against the
+measured read / copy / gather ceilings, the R-MAT scale 22 target densities,
+the level-synchronous BFS (R-MAT scale 22, 64 sources) with a CPU BFS, the
+batched multi-vector configuration (R-MAT scale 24, 256 vectors), the
+PageRank driver, the config-1 uniform matrix, and the reference's CPU path
+(the oracle's FP32 restatement of SpmSpvCSCHost) on one core beside the
+build's all-core port.
 """
 import argparse
 import json
@@ -32,6 +37,7 @@ sys.path.insert(0, os.path.join(ROOT, "spmspv-adaptive_amd"))
 
 HBM_PEAK_GBS = 8000.0
 DENSITIES = [1e-4, 1e-3, 1e-2, 0.1, 0.5, 1.0]
+TARGET_DENSITIES = [0.01, 0.1, 1.0]
 WEIGHTS = os.path.join(ROOT, "spmspv-adaptive_amd", "selector_weights.npy")
 BFS_WEIGHTS = os.path.join(ROOT, "spmspv-adaptive_amd", "bfs_selector_weights.npy")
 AUTO_SLOT = 99  # --only-slot value that times the selector-dispatched call
@@ -42,8 +48,37 @@ def alg_bytes(xnnz, bin_len, ynnz):
 
 
 def profile_slot(sa, A, n, args, yk, yv):
-    """One slot at one density, cold, `reps` times (rocprofv3 runs read the
-    kernel durations and counters of exactly these launches)."""
+    """Profiling mode: every slot of --only-slot at every density of
+    --density, cold, `reps` times each, slots outer, densities inner (rocprofv3
+    runs read the kernel durations and counters of exactly these launches;
+    each call follows one 512 MB fill, which delimits the calls in a trace).
+    With --profile-autotune the per-matrix tuning runs first, on the dense x,
+    as in the sweep; the calibration copy (--calib-mb) follows it, so every
+    dispatch after the copy belongs to the profiled calls."""
+    slots = [int(v) for v in args.only_slot.split(",")]
+    dens = [float(v) for v in args.density.split(",")]
+    xs = {}
+    for d in dens:
+        k = max(1, int(round(d * n)))
+        key, xv = sa.gen_sparse_vector(n, k, 0x5EED0100 + DENSITIES.index(d) if d in DENSITIES else 0x5EED0200)
+        xs[d] = (k, sa.DeviceArray.from_host(key, np.int32), sa.DeviceArray.from_host(xv, np.float32))
+    tuned = None
+    if args.profile_autotune:
+        key = np.arange(n, dtype=np.int32)
+        dk = sa.DeviceArray.from_host(key, np.int32)
+        dv = sa.DeviceArray.from_host(np.ones(n, np.float32), np.float32)
+        A.set_sparse_x(dk, dv, n)
+        tuned = A.autotune(reps=3)
+    if args.spmv_type >= 0:  # the K5/K7 format the sweep's autotuner picked
+        A.set_spmv_type(args.spmv_type)
+        A.set_sigma(args.sigma)
+    if args.no_small_x:
+        A.set_small_x_path(False)
+    bls = {}
+    for d, (k, dk, dv) in xs.items():
+        A.set_sparse_x(dk, dv, k)
+        bls[d] = A.computeBinlen()
+    sa.synchronize()
     if args.calib_mb:
         nb = args.calib_mb << 20
         src = sa.DeviceArray(nb // 4, np.float32)
@@ -52,31 +87,60 @@ def profile_slot(sa, A, n, args, yk, yv):
         sa.synchronize()
         sa.lib().spmspv_memcpy_d2d(dst.ptr, src.ptr, nb)
         sa.synchronize()
-    k = max(1, int(round(args.density * n)))
-    key, xv = sa.gen_sparse_vector(n, k, 0x5EED0100 + DENSITIES.index(args.density)
-                                   if args.density in DENSITIES else 0x5EED0200)
-    dk = sa.DeviceArray.from_host(key, np.int32)
-    dv = sa.DeviceArray.from_host(xv, np.float32)
-    A.set_sparse_x(dk, dv, k)
-    bl = A.computeBinlen()
-    if args.spmv_type >= 0:  # the K5/K7 format the sweep's autotuner picked
-        A.set_spmv_type(args.spmv_type)
-        A.set_sigma(args.sigma)
-    ms = []
-    ynnz = 0
-    for _ in range(args.reps):
-        A.set_sparse_x(dk, dv, k)
-        A.flush_cache(args.flush_mb << 20)
-        A.timer_start()
-        if args.only_slot == AUTO_SLOT:
-            ynnz, _ = A.run_auto(y_key=yk, y_val=yv)
-        else:
-            ynnz = A.run(args.only_slot, y_key=yk, y_val=yv)
-        ms.append(A.timer_stop())
-    name = "auto" if args.only_slot == AUTO_SLOT else sa.KERNEL_NAMES[args.only_slot]
-    print(json.dumps({"slot": name, "density": args.density, "xnnz": k,
-                      "bin_len": int(bl), "ynnz": int(ynnz), "ms": ms,
-                      "alg_bytes": alg_bytes(k, bl, ynnz), "calib_bytes": args.calib_mb << 20}), flush=True)
+    for slot in slots:
+        for d in dens:
+            k, dk, dv = xs[d]
+            ms = []
+            ynnz = used = 0
+            for _ in range(args.reps):
+                A.set_sparse_x(dk, dv, k)
+                A.flush_cache(args.flush_mb << 20)
+                A.timer_start()
+                if slot == AUTO_SLOT:
+                    ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+                else:
+                    ynnz, used = A.run(slot, y_key=yk, y_val=yv), slot
+                ms.append(A.timer_stop())
+            name = "auto" if slot == AUTO_SLOT else sa.KERNEL_NAMES[slot]
+            print(json.dumps({"slot": name, "slot_id": slot, "ran": sa.KERNEL_NAMES[used], "density": d,
+                              "xnnz": k, "bin_len": int(bls[d]), "ynnz": int(ynnz), "ms": ms,
+                              "alg_bytes": alg_bytes(k, bls[d], ynnz), "calib_bytes": args.calib_mb << 20,
+                              "autotuned": tuned}), flush=True)
+
+
+def cpu_info():
+    """(model name, cores this process may use)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def load_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    return oracle
+
+
+def result_error(oracle, m, cp, ri, cv, key, val, yk, yv):
+    """Index sets equal to the FP64 oracle's, and max |y - y64| / sum|a x|."""
+    ek, ev, eb = oracle.spmspv_f64(m, cp, ri, cv, key, val)
+    same = bool(yk.size == ek.size and np.array_equal(yk, ek))
+    if not same:
+        return False, None
+    if ek.size == 0:
+        return True, 0.0
+    return True, float((np.abs(yv.astype(np.float64) - ev) / np.maximum(eb, 1e-300)).max())
 
 
 def run_bfs(sa, args, A, rp, col, n):
@@ -88,14 +152,8 @@ def run_bfs(sa, args, A, rp, col, n):
     if bscale == args.scale:
         B, brp, bcol, bn = A, rp, col, n
     else:
-        bn = 1 << bscale
-        brp, bcol, bval = sa.gen_rmat_csr(bscale, 16, 0.57, 0.19, 0.19, seed=0x5EED0004)
-        BM = sa.Matrix(bn, bn, brp, bcol, bval)
-        B = BM.A
-        if os.path.exists(WEIGHTS):
-            B.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
-        else:
-            B.selector_mode(1)
+        g = bfs_graph(sa, args)
+        B, brp, bcol, bn = g["M"].A, g["rp"], g["col"], g["n"]
     if os.path.exists(BFS_WEIGHTS):  # the BFS levels' own selector
         B.bfs_selector_set_weights(np.load(BFS_WEIGHTS).astype(np.float32))
     deg = np.diff(brp)
@@ -104,12 +162,17 @@ def run_bfs(sa, args, A, rp, col, n):
     srcs = rng.choice(cand, size=min(args.bfs_sources, cand.size), replace=False)
     lev = sa.DeviceArray(bn, np.int32)
     refs = {}
+    cpu_s, cpu_edges = 0.0, 0.0
     if args.bfs_check > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
+        oracle = load_oracle()
         for s_ in srcs[:args.bfs_check]:
-            refs[int(s_)] = oracle.bfs(bn, brp, bcol, int(s_))[0]  # symmetric: CSR = CSC
+            t0 = time.perf_counter()
+            lv = oracle.bfs(bn, brp, bcol, int(s_))[0]  # symmetric: CSR = CSC
+            cpu_s += time.perf_counter() - t0
+            refs[int(s_)] = lv
+            cpu_edges += float(deg[lv >= 0].sum()) / 2.0
     res, levels, checked = {}, {}, 0
+    roof_bytes, roof_s = 0.0, 0.0
     for pol in [-1] + list(range(sa.NUM_SLOTS)):
         B.bfs(int(srcs[0]), lev, policy=pol, use_graphs=1)  # warm-up + graph capture
         secs, edges = 0.0, 0.0
@@ -117,18 +180,167 @@ def run_bfs(sa, args, A, rp, col, n):
             st = B.bfs(int(s_), lev, policy=pol, use_graphs=1)
             secs += st[2]
             edges += st[1]
+            if pol < 0:
+                # boolean level bytes summed over the levels: 12 per frontier
+                # vertex + 4 per frontier edge + 4 per new vertex + n/8 per level
+                roof_bytes += 16.0 * st[3] + 8.0 * st[1] + st[0] * ((bn + 7) // 8)
+                roof_s += st[2]
             if int(s_) in refs:
                 if not np.array_equal(lev.to_host(), refs[int(s_)]):
                     raise SystemExit(f"BFS levels differ from the CPU BFS: policy {pol}, source {s_}")
                 checked += 1
             levels[int(s_)] = int(st[0])
         res["learned" if pol < 0 else sa.KERNEL_NAMES[pol]] = edges / secs / 1e9
+    # the same sources as one multi-source BFS (64 visited bits per vertex)
+    msb = None
+    if len(srcs) <= 64:
+        mlev = sa.DeviceArray(bn * len(srcs), np.int32)
+        B.bfs_multi(srcs, mlev)  # warm-up
+        msecs, medges, mreached = [], 0.0, 0
+        for _ in range(3):
+            st = B.bfs_multi(srcs, mlev)
+            msecs.append(st[2])
+            medges, mreached = st[1], int(st[3])
+        mh = mlev.to_host().reshape(bn, len(srcs))
+        mchecked = 0
+        for i, s_ in enumerate(srcs):
+            if int(s_) in refs:
+                if not np.array_equal(mh[:, i], refs[int(s_)]):
+                    raise SystemExit(f"multi-source BFS levels differ from the CPU BFS: source {s_}")
+                mchecked += 1
+        del mh, mlev
+        ms_s = float(np.median(msecs))
+        msb = {"gteps": round(medges / ms_s / 1e9, 3), "ms": round(1e3 * ms_s, 3), "sources": int(len(srcs)),
+               "reached_pairs": mreached, "levels_checked_vs_cpu": mchecked,
+               "edges": "undirected edges summed over the sources' components, as for the single-source runs"}
     fixed_best = max(v for kk, v in res.items() if kk != "learned")
     bfs = {"graph": f"rmat{bscale}", "sources": int(len(srcs)), "gteps": res,
            "learned_gteps": res["learned"], "best_fixed_gteps": fixed_best,
            "levels_checked_vs_cpu": checked,
-           "mean_levels": round(float(np.mean(list(levels.values()))), 2)}
+           "mean_levels": round(float(np.mean(list(levels.values()))), 2),
+           "roofline_learned": {"achieved_gbs": round(roof_bytes / roof_s / 1e9, 2),
+                                "frac": round(roof_bytes / roof_s / 1e9 / HBM_PEAK_GBS, 5),
+                                "bytes": "16*reached + 8*edges + levels*n/8 per BFS",
+                                "timed": "host wall clock of each BFS, level round trips included"}}
+    if msb is not None:
+        bfs["multi_source"] = msb
+    if cpu_s > 0:
+        bfs["cpu_baseline"] = {"gteps": round(cpu_edges / cpu_s / 1e9, 4), "cores": 1, "kind": "port",
+                               "sample": f"{len(refs)} sources of the same graph, textbook queue BFS "
+                                         f"(oracle), {cpu_s:.2f} s"}
     return bfs
+
+
+_BFS_GRAPH = {}
+
+
+def bfs_graph(sa, args):
+    """The R-MAT graph of the BFS workload (scale --bfs-scale, edge factor 16,
+    seed 0x5EED0004), built once: host CSR (= CSC by symmetry) and a handle
+    with the selector weights."""
+    if "M" not in _BFS_GRAPH:
+        bscale = args.bfs_scale
+        bn = 1 << bscale
+        brp, bcol, bval = sa.gen_rmat_csr(bscale, 16, 0.57, 0.19, 0.19, seed=0x5EED0004)
+        BM = sa.Matrix(bn, bn, brp, bcol, bval)
+        if os.path.exists(WEIGHTS):
+            BM.A.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
+        else:
+            BM.A.selector_mode(1)
+        _BFS_GRAPH.update(M=BM, rp=brp, col=bcol, val=bval, n=bn, scale=bscale)
+    return _BFS_GRAPH
+
+
+def run_c1(sa, args, oracle):
+    """The reference's CPU-runnable case: uniform n = 65,536 with 16 distinct
+    rows per column, x with 655 keys (seed 0x5EED0001). The sort-based
+    kernel (col-naive-sort) and the selector-dispatched call, cold, median
+    of --reps, against SpmSpvCSCHost on one core; FP32 error vs FP64."""
+    n, k = 65536, 16
+    cp, ri, cv = sa.gen_uniform_csc(n, k, 0x5EED0001)
+    rp, col, val = oracle.csr_to_csc(n, n, cp, ri, cv)  # CSR of A = CSC of its transpose
+    M = sa.Matrix(n, n, rp, col, val)
+    if os.path.exists(WEIGHTS):
+        M.A.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
+    key, xv = sa.gen_sparse_vector(n, 655, 0x5EED0001)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(xv, np.float32)
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    res = {"n": n, "nnz": int(cp[-1]), "xnnz": 655}
+    for name, slot in (("col-naive-sort", sa.K_COL_NAIVE_SORT), ("auto", None)):
+        ms = []
+        for r in range(2 + args.reps):
+            M.A.set_sparse_x(dk, dv, key.size)
+            M.A.flush_cache(args.flush_mb << 20)
+            M.A.timer_start()
+            if slot is None:
+                c, used = M.A.run_auto(y_key=yk, y_val=yv)
+            else:
+                c, used = M.A.run(slot, y_key=yk, y_val=yv), slot
+            t = M.A.timer_stop()
+            if r >= 2:
+                ms.append(t)
+        same, err = result_error(oracle, n, cp, ri, cv, key, xv, yk.to_host(c), yv.to_host(c))
+        res[name] = {"ms": round(float(np.median(ms)), 5), "slot": sa.KERNEL_NAMES[used], "ynnz": int(c),
+                     "index_sets_exact": same, "max_rel_err": err}
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        oracle.spmspv_csc_host_f32(n, cp, ri, cv, key, xv)
+        ts.append(time.perf_counter() - t0)
+    res["cpu_ms"] = round(1e3 * float(np.median(ts)), 4)
+    return res
+
+
+def run_target(sa, args, oracle):
+    """SpMSpV on the BFS graph (R-MAT scale 22) at densities 1%, 10% and 100%:
+    the selector-dispatched call, cold, median of --reps; the tile kernel's
+    own time where a row slot ran; FP32 error against the FP64 oracle."""
+    g = bfs_graph(sa, args)
+    A, n, rp, col, val = g["M"].A, g["n"], g["rp"], g["col"], g["val"]
+    yk = sa.DeviceArray(n, np.int32)
+    yv = sa.DeviceArray(n, np.float32)
+    xs = []
+    for i, d in enumerate(TARGET_DENSITIES):
+        k = max(1, int(round(d * n)))
+        key, xv = sa.gen_sparse_vector(n, k, 0x5EED0300 + i)
+        xs.append((k, sa.DeviceArray.from_host(key, np.int32), sa.DeviceArray.from_host(xv, np.float32), key, xv))
+    k, dk, dv = xs[-1][:3]
+    A.set_sparse_x(dk, dv, k)
+    tuned = A.autotune(reps=3)
+    out = []
+    A.time_main_kernel(True)
+    for i, d in enumerate(TARGET_DENSITIES):
+        k, dk, dv, key, xv = xs[i]
+        A.set_sparse_x(dk, dv, k)
+        bl = A.computeBinlen()
+        ms, kms, ynnz, used = [], [], 0, -1
+        for r in range(2 + args.reps):
+            A.set_sparse_x(dk, dv, k)
+            A.flush_cache(args.flush_mb << 20)
+            A.timer_start()
+            ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+            t = A.timer_stop()
+            if r >= 2:
+                ms.append(t)
+                if used in (sa.K_ROW_LB, sa.K_SPMV_LB):
+                    kms.append(A.main_kernel_ms())
+        same, err = result_error(oracle, n, rp, col, val, key, xv, yk.to_host(ynnz), yv.to_host(ynnz))
+        b = alg_bytes(k, bl, ynnz)
+        call = float(np.median(ms))
+        row = {"density": d, "xnnz": k, "bin_len": int(bl), "ynnz": int(ynnz), "slot": sa.KERNEL_NAMES[used],
+               "call_ms": round(call, 5), "call_gbs": round(b / (call * 1e-3) / 1e9, 2),
+               "call_frac": round(b / (call * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+               "index_sets_exact": same, "max_rel_err": err}
+        if kms:
+            km = float(np.median(kms))
+            row.update(kernel_ms=round(km, 5), kernel_gbs=round(b / (km * 1e-3) / 1e9, 2),
+                       kernel_frac=round(b / (km * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
+        out.append(row)
+    A.time_main_kernel(False)
+    return {"graph": f"rmat{g['scale']}", "nnz": int(g["M"].nnz), "autotuned": tuned, "per_density": out,
+            "target": "north star: >= 60% of HBM peak at >= 1% density"}
 
 
 def run_batched_bench(sa, args):
@@ -209,6 +421,26 @@ def run_batched_bench(sa, args):
     same = diff_nnz == 0 and diff_slot == 0 and diff_idx == 0
     b_ms, s_ms = 1e3 * float(np.median(bt)), 1e3 * float(np.median(st))
     prods = int(np.sum(bls))
+    # 8 vectors spanning the density range against the FP64 oracle (the
+    # symmetric CSR is the CSC), and the reference's CPU path on a sample
+    oracle = load_oracle()
+    order = np.argsort(dens)
+    pick = [int(order[j]) for j in np.linspace(0, nvec - 1, min(8, nvec)).astype(int)]
+    oracle_ok, oracle_err = True, 0.0
+    for v in pick:
+        o, c = int(yoff[v]), int(yn[v])
+        ok, err = result_error(oracle, n, rp, col, val, keys[v], vals[v], bk[o:o + c], bv[o:o + c])
+        oracle_ok &= ok
+        oracle_err = max(oracle_err, err if err is not None else float("inf"))
+    cpu_t, cpu_b, cpu_v = 0.0, 0.0, 0
+    for v in order:  # sparsest first: a bounded sample of about 5 s
+        t = time.perf_counter()
+        _, ck, _ = oracle.spmspv_csc_host_f32(n, rp, col, val, keys[v], vals[v])
+        cpu_t += time.perf_counter() - t
+        cpu_b += alg_bytes(keys[v].size, bls[v], ck.size)
+        cpu_v += 1
+        if cpu_t > 5.0:
+            break
     return {"graph": f"rmat{scale}", "nnz": int(M.nnz), "vectors": nvec,
             "density": "log-uniform [1e-4, 0.2]", "total_bin_len": prods,
             "batched_ms": round(b_ms, 3), "sequential_ms": round(s_ms, 3),
@@ -218,7 +450,12 @@ def run_batched_bench(sa, args):
             "slots_used": {sa.KERNEL_NAMES[k]: int((used == k).sum()) for k in np.unique(used)},
             "matches_sequential": same, "vectors_differing": {"ynnz": diff_nnz, "slot": diff_slot,
                                                               "index_set": diff_idx},
-            "max_abs_value_diff": max_dv, "setup_s": round(setup_s, 1)}
+            "max_abs_value_diff": max_dv, "setup_s": round(setup_s, 1),
+            "oracle_checked_vectors": len(pick), "oracle_index_sets_exact": bool(oracle_ok),
+            "oracle_max_rel_err": oracle_err,
+            "cpu_baseline": {"gbs": round(cpu_b / cpu_t / 1e9, 3), "cores": 1, "kind": "port",
+                             "sample": f"SpmSpvCSCHost on the {cpu_v} sparsest of the {nvec} vectors, "
+                                       f"{cpu_t:.1f} s"}}
 
 
 def main():
@@ -228,9 +465,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--reps-fixed", type=int, default=3)
-    ap.add_argument("--bfs-sources", type=int, default=8)
-    ap.add_argument("--bfs-scale", type=int, default=0,
+    ap.add_argument("--bfs-sources", type=int, default=64)
+    ap.add_argument("--bfs-scale", type=int, default=22,
                     help="R-MAT scale of the BFS graph (0: the SpMSpV graph); 22 with 64 sources is the BFS workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (R-MAT scale 24) section")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the sweep, its fixed slots and the CPU baseline (no scale-22/24 sections)")
     ap.add_argument("--batched", action="store_true",
                     help="run only the batched measurement (one JSON line)")
     ap.add_argument("--batched-scale", type=int, default=24)
@@ -247,10 +487,14 @@ def main():
                     help="profiling mode: K5/K7 format (0 merge-path, 1 CSR5), as reported under autotuned")
     ap.add_argument("--sigma", type=int, default=-1, help="profiling mode: CSR5 sigma")
     ap.add_argument("--flush-mb", type=int, default=512)
-    ap.add_argument("--only-slot", type=int, default=-1,
-                    help="profiling mode: time one slot at --density only (for rocprofv3 runs); "
-                         "99 = the selector-dispatched call")
-    ap.add_argument("--density", type=float, default=1.0)
+    ap.add_argument("--only-slot", default="",
+                    help="profiling mode: time these slots (comma list) at the --density values only "
+                         "(for rocprofv3 runs); 99 = the selector-dispatched call")
+    ap.add_argument("--density", default="1.0", help="profiling mode: comma list of x densities")
+    ap.add_argument("--no-small-x", action="store_true",
+                    help="profiling mode: turn the single-launch small-x column path off")
+    ap.add_argument("--profile-autotune", action="store_true",
+                    help="profiling mode: run the per-matrix tuning first, as the sweep does")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--calib-mb", type=int, default=0,
                     help="profiling mode: first copy this many MiB device-to-device (counter calibration)")
@@ -295,7 +539,7 @@ def main():
         xs.append((k, sa.DeviceArray.from_host(key, np.int32), sa.DeviceArray.from_host(xv, np.float32), key, xv))
     yk = sa.DeviceArray(n, np.int32)
     yv = sa.DeviceArray(n, np.float32)
-    if args.only_slot >= 0:
+    if args.only_slot:
         profile_slot(sa, A, n, args, yk, yv)
         return
     meta = []
@@ -309,6 +553,7 @@ def main():
     A.set_sparse_x(dk, dv, k)
     tuned = A.autotune(reps=3) if not args.no_autotune else None
     setup_s = time.time() - t_setup
+    ceilings = A.measure_ceilings(3)
 
     def timed_call(i, slot=None):
         k, dk, dv = xs[i][:3]
@@ -334,6 +579,7 @@ def main():
     ynnzs = [0] * len(DENSITIES)
     A.time_main_kernel(True)
     sa.synchronize()
+    spec0 = A.speculation_stats()
     for _ in range(args.steps):
         tot = 0.0
         for i in range(len(DENSITIES)):
@@ -347,6 +593,7 @@ def main():
         step_ms.append(tot)
     sa.synchronize()
     A.time_main_kernel(False)
+    spec1 = A.speculation_stats()
     step_bytes = sum(alg_bytes(xs[i][0], meta[i], ynnzs[i]) for i in range(len(DENSITIES)))
     ms_step = float(np.median(step_ms))
     value = step_bytes / (ms_step * 1e-3) / 1e9
@@ -361,6 +608,43 @@ def main():
     sa.synchronize()
     warm_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     warm_value = step_bytes / (warm_ms * 1e-3) / 1e9
+
+    # ---- FP32 error of every density against the FP64 oracle (symmetric
+    # matrix: the CSR is the CSC), outside the timed steps
+    oracle = load_oracle()
+    errors = []
+    for i in range(len(DENSITIES)):
+        k, dk, dv, key, xv = xs[i]
+        A.set_sparse_x(dk, dv, k)
+        ynnz, used = A.run_auto(y_key=yk, y_val=yv)
+        same, err = result_error(oracle, n, rp, col, val, key, xv, yk.to_host(ynnz), yv.to_host(ynnz))
+        errors.append({"density": DENSITIES[i], "index_sets_exact": same, "max_rel_err": err})
+
+    # ---- mixed order: fresh x of log-uniform density in random order, so
+    # consecutive calls of one size class may want different slots; timed
+    # with speculation on and off
+    rng = np.random.default_rng(0x5EED0400)
+    mixed = []
+    for j in range(24):
+        kk = max(1, int(round(float(np.exp(rng.uniform(np.log(1e-4), 0.0))) * n)))
+        key, xv = sa.gen_sparse_vector(n, kk, 0x5EED0500 + j)
+        mixed.append((kk, sa.DeviceArray.from_host(key, np.int32), sa.DeviceArray.from_host(xv, np.float32)))
+    mixed_res = {}
+    for spec_on in (True, False):
+        A.set_speculation(spec_on)
+        h0 = A.speculation_stats()
+        tot = 0.0
+        for kk, dk, dv in mixed:
+            A.set_sparse_x(dk, dv, kk)
+            if flush:
+                A.flush_cache(flush)
+            A.timer_start()
+            A.run_auto(y_key=yk, y_val=yv)
+            tot += A.timer_stop()
+        h1 = A.speculation_stats()
+        mixed_res["on" if spec_on else "off"] = {"ms": round(tot, 4), "hits": h1[0] - h0[0],
+                                                 "misses": h1[1] - h0[1]}
+    A.set_speculation(True)
 
     # ---- every fixed slot, same inputs, same cold start
     fixed = None
@@ -415,19 +699,34 @@ def main():
         traffic = float(np.mean([t for t, _ in traffic_src]))
         traffic_src = [p for _, p in traffic_src]
 
-    bfs = run_bfs(sa, args, A, rp, col, n) if args.bfs_sources > 0 else None
+    bfs = run_bfs(sa, args, A, rp, col, n) if args.bfs_sources > 0 and not args.no_extra else None
+    target = run_target(sa, args, oracle) if args.bfs_scale and not args.no_extra else None
+
+    # ---- PageRank driver on the sweep's graph: learned per-iteration slot
+    # vs fixed slots
+    pagerank = None
+    if not args.no_extra:
+        rank = sa.DeviceArray(n, np.float32)
+        pagerank = {"graph": f"rmat{args.scale}", "damping": 0.85, "eps": 1e-4}
+        for pol in (-1, sa.K_SPMV_LB, sa.K_COL_LB_SORT):
+            st, sizes = A.pagerank(rank, 0.85, 1e-4, 1000, pol)
+            pagerank["learned" if pol < 0 else sa.KERNEL_NAMES[pol]] = {
+                "iterations": int(st[0]), "seconds": round(float(st[1]), 4),
+                "x_entries": int(st[2]), "first_sizes": [int(v) for v in sizes[:3]],
+                "last_sizes": [int(v) for v in sizes[-3:]]}
+        del rank
+
+    c1 = run_c1(sa, args, oracle) if not args.no_extra else None
 
     # ---- CPU baseline: the reference's CPU SpMSpV (oracle port, 1 core)
-    cpu = None
+    cpu, cpu_all = None, None
+    model, cores = cpu_info()
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        cp, ri, cv = oracle.csr_to_csc(n, n, rp, col, val)
+        cp, ri, cv = rp, col, val  # symmetric: the CSR is the CSC
+        # a bounded sample: whole sweeps of the 6 densities until about 8 s
+        # of single-core CPU work, then the same calls on every core
         t_cpu, b_cpu, calls = 0.0, 0.0, 0
-        # a bounded sample: whole sweeps of the 6 densities until about 10 s
-        # of single-core CPU work
-        tlimit = 10.0
-        while (t_cpu < tlimit or calls % len(DENSITIES)) and calls < len(DENSITIES) * 5000:
+        while (t_cpu < 8.0 or calls % len(DENSITIES)) and calls < len(DENSITIES) * 5000:
             i = calls % len(DENSITIES)
             k, _, _, key, xv = xs[i]
             t0 = time.perf_counter()
@@ -437,7 +736,19 @@ def main():
             calls += 1
         cpu = {"value": b_cpu / t_cpu / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"{calls} calls of SpmSpvCSCHost (FP32, single thread) cycling the 6 densities "
-                         f"of the same R-MAT s{args.scale} workload, {t_cpu:.1f} s"}
+                         f"of the same R-MAT s{args.scale} workload, {t_cpu:.1f} s",
+               "cpu_model": model}
+        t_all, b_all = 0.0, 0.0
+        for c in range(calls):
+            i = c % len(DENSITIES)
+            k, _, _, key, xv = xs[i]
+            t0 = time.perf_counter()
+            _, ck, _ = oracle.spmspv_csc_host_f32_omp(n, cp, ri, cv, key, xv, cores)
+            t_all += time.perf_counter() - t0
+            b_all += alg_bytes(k, meta[i], ck.size)
+        cpu_all = {"value": round(b_all / t_all / 1e9, 3), "unit": "GB/s", "cores": cores,
+                   "kind": "the build's OpenMP port (not the reference, which is single-threaded)",
+                   "sample": f"the same {calls} calls, {t_all:.2f} s", "cpu_model": model}
 
     out = {
         "metric": "SpMSpV/BFS GB/s (% HBM peak) + GTEPS: learned selector vs best fixed kernel",
@@ -472,6 +783,13 @@ def main():
                         for i in range(len(DENSITIES))],
         "setup_s": round(setup_s, 2),
         "autotuned": tuned,
+        "ceilings": {k: round(v, 2) for k, v in ceilings.items()},
+        "errors_vs_fp64": {"bound": "|y - y64| <= 1e-5 * sum|a_ij x_j|", "per_density": errors,
+                           "max_rel_err": max((e["max_rel_err"] or 0.0) for e in errors),
+                           "all_index_sets_exact": all(e["index_sets_exact"] for e in errors)},
+        "speculation": {"timed_steps": {"hits": spec1[0] - spec0[0], "misses": spec1[1] - spec0[1]},
+                        "mixed_order_24_calls": mixed_res},
+        "cpu_baseline_allcore": cpu_all,
     }
     if fixed is not None:
         out["fixed_ms"] = {sa.KERNEL_NAMES[s]: [round(float(v), 5) for v in fixed[s]] for s in range(sa.NUM_SLOTS)}
@@ -481,6 +799,16 @@ def main():
         out["selector_regret_geomean"] = round(float(np.exp(np.mean(np.log1p(np.maximum(regret, 0))))) - 1, 4)
     if bfs is not None:
         out["bfs"] = bfs
+    if target is not None:
+        out["rmat22_target"] = target
+    if pagerank is not None:
+        out["pagerank"] = pagerank
+    if c1 is not None:
+        out["c1"] = c1
+    if not args.no_batched and not args.no_extra:
+        _BFS_GRAPH.clear()  # free the scale-22 graph before the scale-24 one
+        del M, A
+        out["batched"] = run_batched_bench(sa, args)
     print(json.dumps(out), flush=True)
 
 

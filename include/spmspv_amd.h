@@ -73,9 +73,13 @@ extern "C" {
 #define SPMSPV_K_ROW_LB 5           /* spmspv -> holaspmspv / csr5spmspv   spmspv.h:1812-1821 */
 #define SPMSPV_K_SPMV_NAIVE 6       /* naivespmv                          spmspv.h:1676-1683 */
 #define SPMSPV_K_SPMV_LB 7          /* spmv -> holaspmv / csr5spmv         spmspv.h:1789-1798 */
-#define SPMSPV_K_COL_BUCKET 8       /* CscBasedBucketSpmspv (dormant)      csc_bucket_spmspv.cuh:766-1086 */
+#define SPMSPV_K_COL_BUCKET 8       /* CscBasedBucketSpmspv                csc_bucket_spmspv.cuh:766-1086 */
+/* two-layer load balance (outside the selector's slots): CscBasedMyNoSortSpmspv[_keyval],
+ * CscBasedSortMySpmspv, spmspv.h:1823-1878, 2000-2029 */
+#define SPMSPV_K_COL_TWO_LAYER_ATOMIC 9
+#define SPMSPV_K_COL_TWO_LAYER_SORT 10
 #define SPMSPV_NUM_SLOTS 8          /* slots the selector chooses among */
-#define SPMSPV_NUM_KERNELS 9
+#define SPMSPV_NUM_KERNELS 11
 
 /* Vector types, set_vector_type (spmspv.h:977-982): 0 sparse, 1 dense,
  * 2 bit array. */
@@ -191,11 +195,24 @@ int spmspv_csc_nosort_merge_keyval(spmspv_handle_t h, int is_binlen_computed, fl
                                    int* ynnz, float* d_y, int* d_y_key, float* d_y_val);
 int spmspv_csc_sort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
                           float* d_y, int* d_y_key, float* d_y_val);
-/* CscBasedBucketSpmspv (csc_bucket_spmspv.cuh:766-1086): buckets = number
- * of row buckets, a power of two in [2, 256]; 0 keeps the handle's current
- * count (256, or the autotuner's pick). Other values: INVALID_ARGUMENT. */
+/* CscBasedBucketSpmspv (csc_bucket_spmspv.cuh:766-1086): products counted
+ * per row bin, written into their bins, each bin summed in an LDS window.
+ * buckets = the least number of row bins, a power of two in [2, 256] (bins
+ * hold at most 32,768 rows, so large m gets more); 0 keeps the handle's
+ * current count (256, or the autotuner's pick). Other values:
+ * INVALID_ARGUMENT; m above 2^26 rows: INVALID_ARGUMENT. */
 int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* d_y_key, float* d_y_val,
                       int buckets);
+/* CscBasedMyNoSortSpmspv[_keyval] / CscBasedSortMySpmspv (spmspv.h:1823-1878,
+ * 2000-2029): the two-layer load balance, tiles of exactly 1,792 products
+ * (the reference's write-back without the block offset,
+ * csc_two_layer_load_balance_fuse_reduction_modify.cuh:146-149, fixed). */
+int spmspv_csc_my_nosort(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz, float* d_y,
+                         int* d_y_key, float* d_y_val);
+int spmspv_csc_my_nosort_keyval(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
+                                float* d_y, int* d_y_key, float* d_y_val);
+int spmspv_csc_sort_my(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz, float* d_y,
+                       int* d_y_key, float* d_y_val);
 int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y);
 int spmspv_holaspmspv(spmspv_handle_t h, float alpha, float* d_y);
 int spmspv_csr5spmspv(spmspv_handle_t h, float alpha, float* d_y);
@@ -217,8 +234,26 @@ int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y);
  * merge-path tiles won)}. */
 int spmspv_autotune(spmspv_handle_t h, int reps, int* report);
 /* As spmspv_autotune; csr5 = 0 keeps the current K5/K7 format and only
- * tunes its parameters' merge-path side. */
+ * tunes its parameters' merge-path side. Both also pick the bucket slot's
+ * number of row buckets (16..256), the two size thresholds of the
+ * single-workgroup paths (the small-x single launch, up to 11,264 products;
+ * the one-workgroup sort + reduce of the sort slots, up to 3,072 items) and
+ * the bucket rule's bin_len range (spmspv_set_bucket_range), each timed on
+ * sub-vectors of x (evenly spaced keys) whose bin_len lies just under a
+ * candidate. */
 int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report);
+/* No counterpart in the reference: every tuned parameter, out[10] =
+ * {team_col, team_row, row_vt, spmv_type, sigma (0 unless CSR5), row
+ * buckets, small-x product threshold (0: path off), single-workgroup sort
+ * threshold, bucket rule's lowest and highest bin_len (0 and -1: rule off;
+ * INT_MAX: no upper end)}. */
+int spmspv_autotune_report(spmspv_handle_t h, int out[10]);
+/* No counterpart in the reference: the bucket rule behind the selector.
+ * In spmspv_run_auto and spmspv_run_batched, a choice of slot 2, 3, 5 or 7
+ * becomes the bucket slot (8) when lo <= bin_len <= hi (bin_len sampled for
+ * a large x); hi < lo turns it off (the default). spmspv_autotune sets the
+ * range where the bucket slot beat those four on sub-vectors of x. */
+int spmspv_set_bucket_range(spmspv_handle_t h, long long lo, long long hi);
 
 /* ---- learned selector (no reference counterpart in C++: the reference's
  * predictors are stubs returning 0, la/test/new_test_app.cu:121-131) ------ */
@@ -251,6 +286,21 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
  * spmspv_run_auto on (1) or off (0), and reports its right and wrong
  * guesses so far (either pointer may be null). */
 int spmspv_set_speculation(spmspv_handle_t h, int on);
+/* Switching-aware selection (the reference's "real" accounting charges the
+ * x format conversion whenever consecutive calls change kernel family,
+ * model/1-test_dtree.py:266-287): with penalty > 0, spmspv_run_auto keeps
+ * the previous call's family (column 0-3, row SpMSpV 4-5, SpMV 6-7) when
+ * that family's best slot has a logit within `penalty` of the network's top
+ * choice. The network is trained on soft targets exp(-(t/t_best - 1)/tau),
+ * so penalty = switching cost / (tau * t_best). 0 (the default) is off. */
+int spmspv_set_switch_penalty(spmspv_handle_t h, float penalty);
+/* No counterpart in the reference: on (1, the default) a column slot on an
+ * x of at most 4,096 keys whose products fit one workgroup (11,264) runs
+ * as one launch: scan, gather, LDS sort, reduce and the sorted output (the
+ * sort-based slots' result); spmspv_run_auto then decides such calls by
+ * this size rule before the network and reports slot 1. Off (0): every
+ * slot always runs its own launches. Results are the same either way. */
+int spmspv_set_small_x_path(spmspv_handle_t h, int on);
 int spmspv_speculation_stats(spmspv_handle_t h, long long* hits, long long* misses);
 
 /* Training on the GPU with the library's own forward/backward kernels.
@@ -277,6 +327,17 @@ int spmspv_mlp_forward(const float* params, const float* X, int B, float* logits
  * seconds, reached}. */
 int spmspv_bfs(spmspv_handle_t h, int src, int* d_levels, int policy, int use_graphs,
                double stats[4]);
+/* No counterpart in the reference: up to 64 BFSs advanced together over the
+ * same pattern as spmspv_bfs, one bit per source in a 64-bit word per
+ * vertex, so one pass over a level's edges serves every source (top-down
+ * push for small frontiers, bottom-up pull with early exit for large ones,
+ * the boolean semiring in integer words). srcs[nsrc] (host, 1 <= nsrc <= 64,
+ * repeats allowed); d_levels[n * nsrc] (device) receives the levels
+ * vertex-major: d_levels[v * nsrc + i] is the level of v from srcs[i]
+ * (-1 unreached), the same as spmspv_bfs from srcs[i]. stats (nullable) =
+ * {levels, undirected edges summed over the sources' components, seconds,
+ * reached (vertex, source) pairs}. */
+int spmspv_bfs_multi(spmspv_handle_t h, int nsrc, const int* srcs, int* d_levels, double stats[4]);
 /* No counterpart in the reference: spmspv_bfs that also records, for each of
  * the first max_levels levels, the level's device time in ms (level_ms,
  * host) and the 16 selector features of its frontier (level_feat, host,

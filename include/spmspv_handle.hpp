@@ -158,19 +158,18 @@ class SpmspvHandle {
                               TYPE_IT* d_y_key, TYPE_VT* d_y_val) {
     return spmspv_csc_sort_merge(h_, IsBinLenComputed, alpha, ynnz, d_y, d_y_key, d_y_val);
   }
-  // The two-layer load-balance variants (spmspv.h:1823-1878, 2000-2029)
-  // are served by the merge-path kernels (their write-back defect,
-  // csc_two_layer_load_balance_fuse_reduction_modify.cuh:146-149, is not
-  // reproduced).
+  // The two-layer load-balance variants (spmspv.h:1823-1878, 2000-2029),
+  // with their write-back defect
+  // (csc_two_layer_load_balance_fuse_reduction_modify.cuh:146-149) fixed.
   int CscBasedMyNoSortSpmspv(bool b, const TYPE_VT alpha, TYPE_IT* y_nnz, TYPE_VT* y, TYPE_IT* k, TYPE_VT* v) {
-    return spmspv_csc_nosort_merge(h_, b, alpha, y_nnz, y, k, v);
+    return spmspv_csc_my_nosort(h_, b, alpha, y_nnz, y, k, v);
   }
   int CscBasedMyNoSortSpmspv_keyval(bool b, const TYPE_VT alpha, TYPE_IT* ynnz, TYPE_VT* d_y, TYPE_IT* k,
                                     TYPE_VT* v) {
-    return spmspv_csc_nosort_merge_keyval(h_, b, alpha, ynnz, d_y, k, v);
+    return spmspv_csc_my_nosort_keyval(h_, b, alpha, ynnz, d_y, k, v);
   }
   int CscBasedSortMySpmspv(bool b, const TYPE_VT alpha, TYPE_IT* ynnz, TYPE_VT* d_y, TYPE_IT* k, TYPE_VT* v) {
-    return spmspv_csc_sort_merge(h_, b, alpha, ynnz, d_y, k, v);
+    return spmspv_csc_sort_my(h_, b, alpha, ynnz, d_y, k, v);
   }
   int CscBasedBucketSpmspv(const TYPE_VT alpha, TYPE_IT* y_nnz, TYPE_IT* k, TYPE_VT* v, int buckets) {
     return spmspv_csc_bucket(h_, alpha, y_nnz, k, v, buckets);

@@ -8,6 +8,7 @@
 #include <float.h>
 #include <limits.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -65,6 +66,56 @@ int oracle_spmspv_csc_host_f32(int m, const int* col_ptr, const int* row_idx,
     }
   }
   return idx;
+}
+
+int oracle_spmspv_csc_host_f32_omp(int m, const int* col_ptr, const int* row_idx,
+                                   const float* cval, int xnnz, const int* x_key,
+                                   const float* x_val, float* y_dense, int* y_key,
+                                   float* y_val, int threads) {
+  /* the loop of test_sort_spmspv.cu:336-351 with the rows split into one
+   * contiguous range per thread: every thread walks all of x's columns and
+   * adds only the products of its own rows (found by binary search, rows
+   * ascending in a column), in x's order, so each row's sum is the serial
+   * one. Then the |y| >= FLT_EPSILON filter of :354-372, in parallel. */
+  if (threads < 1) threads = 1;
+  int* cnt = (int*)calloc((size_t)threads + 1, sizeof(int));
+  int used = 1; /* the team size the runtime gave */
+#pragma omp parallel num_threads(threads)
+  {
+    const int t = omp_get_thread_num(), T = omp_get_num_threads();
+    const int r0 = (int)((long long)m * t / T), r1 = (int)((long long)m * (t + 1) / T);
+    for (int i = r0; i < r1; ++i) y_dense[i] = 0.f;
+    for (int i = 0; i < xnnz; ++i) {
+      const int c = x_key[i];
+      const float xv = x_val[i];
+      int lo = col_ptr[c], hi = col_ptr[c + 1];
+      while (lo < hi) { /* first entry with row >= r0 */
+        const int mid = lo + (hi - lo) / 2;
+        if (row_idx[mid] < r0) lo = mid + 1; else hi = mid;
+      }
+      for (int j = lo; j < col_ptr[c + 1] && row_idx[j] < r1; ++j) y_dense[row_idx[j]] += cval[j] * xv;
+    }
+    int k = 0;
+    for (int i = r0; i < r1; ++i) k += !(fabsf(y_dense[i]) < FLT_EPSILON);
+    cnt[t + 1] = k;
+#pragma omp barrier
+#pragma omp single
+    {
+      used = T;
+      for (int u = 0; u < T; ++u) cnt[u + 1] += cnt[u];
+    }
+    int o = cnt[t];
+    for (int i = r0; i < r1; ++i) {
+      if (!(fabsf(y_dense[i]) < FLT_EPSILON)) {
+        if (y_key) y_key[o] = i;
+        if (y_val) y_val[o] = y_dense[i];
+        ++o;
+      }
+    }
+  }
+  const int total = cnt[used];
+  free(cnt);
+  return total;
 }
 
 int oracle_spmspv_f64(int m, const int* col_ptr, const int* row_idx,

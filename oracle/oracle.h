@@ -46,6 +46,17 @@ int oracle_spmspv_csc_host_f32(int m, const int* col_ptr, const int* row_idx,
                                const float* x_val, float* y_dense,
                                int* y_key, float* y_val);
 
+/* The build's all-core CPU port, NOT the reference (whose CPU path is
+ * single-threaded): the SpmSpvCSCHost loop above with the rows split into
+ * one contiguous range per OpenMP thread (each thread walks every column of
+ * x's support and adds its own rows' products in x's order, so every row's
+ * FP32 sum equals the serial one bit for bit), then the same filter.
+ * Returns y_nnz. */
+int oracle_spmspv_csc_host_f32_omp(int m, const int* col_ptr, const int* row_idx,
+                                   const float* cval, int xnnz, const int* x_key,
+                                   const float* x_val, float* y_dense,
+                                   int* y_key, float* y_val, int threads);
+
 /* Judged FP64 SpMSpV, column-driven (the algorithm of SpmSpvCSCHost,
  * la/test/test_sort_spmspv.cu:342-351) with FP64 accumulation and the
  * structural support: a row is in the output iff some selected column has a

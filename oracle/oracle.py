@@ -51,6 +51,9 @@ def lib():
         L.oracle_scan_col_lengths.restype = ctypes.c_int64
         L.oracle_mlp_loss_grad.argtypes = [_c_int] * 5 + [_f64p, _f64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_mlp_loss_grad.restype = ctypes.c_double
+        L.oracle_spmspv_csc_host_f32_omp.argtypes = [_c_int, _i32p, _i32p, _f32p, _c_int, _i32p, _f32p, _f32p,
+                                                     _i32p, _f32p, _c_int]
+        L.oracle_spmspv_csc_host_f32_omp.restype = _c_int
         L.oracle_pagerank_f64.argtypes = [_c_int, _i32p, _i32p, _f32p, ctypes.c_double, ctypes.c_double, _c_int,
                                           _f64p, ctypes.c_void_p]
         L.oracle_pagerank_f64.restype = _c_int
@@ -101,6 +104,20 @@ def spmspv_csc_host_f32(m, col_ptr, row_idx, cval, x_key, x_val):
     xk = _i32(x_key) if len(x_key) else np.zeros(1, np.int32)
     xv = _f32(x_val) if len(x_val) else np.zeros(1, np.float32)
     k = lib().oracle_spmspv_csc_host_f32(m, _i32(col_ptr), ri, cv, len(x_key), xk, xv, yd, yk, yv)
+    return yd, yk[:k], yv[:k]
+
+
+def spmspv_csc_host_f32_omp(m, col_ptr, row_idx, cval, x_key, x_val, threads):
+    """The build's all-core port of SpmSpvCSCHost (row ranges per thread):
+    the same outputs as spmspv_csc_host_f32, bit for bit."""
+    yd = np.zeros(m, np.float32)
+    yk = np.zeros(max(m, 1), np.int32)
+    yv = np.zeros(max(m, 1), np.float32)
+    ri = _i32(row_idx) if len(row_idx) else np.zeros(1, np.int32)
+    cv = _f32(cval) if len(cval) else np.zeros(1, np.float32)
+    xk = _i32(x_key) if len(x_key) else np.zeros(1, np.int32)
+    xv = _f32(x_val) if len(x_val) else np.zeros(1, np.float32)
+    k = lib().oracle_spmspv_csc_host_f32_omp(m, _i32(col_ptr), ri, cv, len(x_key), xk, xv, yd, yk, yv, threads)
     return yd, yk[:k], yv[:k]
 
 

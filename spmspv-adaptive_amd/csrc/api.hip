@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -18,6 +19,16 @@ using namespace spmspv;
 
 namespace {
 thread_local std::string g_err;
+
+// k entries of x evenly spaced over its xn (sorted keys stay sorted).
+__global__ void k_spread_pick(const int* __restrict__ xk, const float* __restrict__ xv, int xn, int k,
+                              int* __restrict__ ok, float* __restrict__ ov) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const long long j = (long long)i * xn / k;
+  ok[i] = xk[j];
+  ov[i] = xv[j];
+}
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -75,6 +86,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
   int* ptr = c.take<int>(n + 1);
   int* xbeg = c.take<int>(n);
   int* split = c.take<int>(lb_tiles);
+  int* tls = c.take<int>(lb_tiles);
   int* tmm = c.take<int>(2 * prep_tiles);
   const size_t bucket_tiles = m / 8192 + 258;
   unsigned long long* ss =
@@ -109,8 +121,14 @@ static size_t carve(spmspv_handle_t h, char* base) {
   long long* ndev = c.take<long long>(4);
   float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
   float* wb = c.take<float>(SPMSPV_MLP_NWEIGHTS);
-  long long* selp = c.take<long long>(3 * kSelMaxGrid);
+  long long* selp = c.take<long long>(5 * kSelMaxGrid);
+  BinWs bws;
+  bws.hist = c.take<unsigned>(kBinMax);
+  bws.cursor = c.take<unsigned long long>(kBinMax);
+  bws.bstart = c.take<long long>(kBinMax + 1);
+  bws.cstart = c.take<int>(kBinMax + 1);
   if (base) {
+    h->bin_ws = bws;
     h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
     h->sort_words = dscan_tiles;
     h->red_words = red_tiles;
@@ -121,6 +139,7 @@ static size_t carve(spmspv_handle_t h, char* base) {
     h->ptr = ptr;
     h->xbeg = xbeg;
     h->split = split;
+    h->tl_starts = tls;
     h->tile_mm = tmm;
     h->scan_state = ss;
     h->acc = acc;
@@ -376,16 +395,64 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
   h->main_timed = false;  // set again below when this slot's tile kernel is timed
   h->count_sent = false;
   hipStream_t s = h->stream;
-  // signal word 3 for the final compaction, when the caller wants the count
-  // (marked pending just before the launch that writes it)
+  // signal word 3 for the result count, when the caller wants it: marked
+  // pending once, before the first launch of the call that may write it
+  bool sig3_marked = false;
+  auto mark_sig3 = [&]() {
+    if (!sig3_marked) __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
+    sig3_marked = true;
+  };
   auto count_sig = [&]() -> long long* {
     if (!h->count_to_host) return nullptr;
-    __atomic_store_n(h->sig + 3, kSigPending, __ATOMIC_RELEASE);
+    mark_sig3();
     h->count_sent = true;
     return h->sig_dev + 3;
   };
   const bool want_sparse = y_key || y_val || want_nnz;
   const int m = h->m;
+  // A fixed column slot (not behind a speculative gate) on a small x with a
+  // sparse result first tries the single-launch path. When the caller waits
+  // for the count anyway, the host reads the path's word first: a finished
+  // call returns at once, otherwise the slot's launches follow with bin_len
+  // known. Without a count, the slot's launches are queued behind it, gated
+  // on its not having finished the call.
+  struct GateRestore {
+    spmspv_handle_t h;
+    bool on = false;
+    ~GateRestore() {
+      if (on) h->gate = Gate{};
+    }
+  } gate_restore{h};
+  if (kid >= SPMSPV_K_COL_NAIVE_ATOMIC && kid <= SPMSPV_K_COL_LB_SORT && !h->gate.word && !y_dense &&
+      h->small_x_path && h->fuse_cap > 0 && h->vec_type != SPMSPV_VEC_DENSE && h->sparse_valid &&
+      h->xnnz <= kFuseKeys &&
+      h->binlen_host <= h->fuse_cap) {  // (binlen_host -1: not known yet)
+    if ((rc = need_csc(h))) return rc;
+    if (h->count_to_host) mark_sig3();
+    const bool wait = h->count_to_host;
+    if (wait) __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
+    const SelectorArgs fixed{nullptr, nullptr, 2, kid, wait ? h->sig_dev + 1 : nullptr, 0};
+    launch_col_fused(s, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, h->n, h->xnnz, m, h->ptr,
+                     h->xbeg, h->split, h->ctl, fixed, y_key ? y_key : h->yk_tmp, y_val ? y_val : h->yv_tmp,
+                     h->count_to_host ? h->sig_dev + 3 : nullptr, h->fuse_cap);
+    h->launches++;
+    if ((rc = check_launch(h, "small-x column path"))) return rc;
+    h->prep_valid = true;
+    if (wait) {
+      long long word = 0;
+      if ((rc = wait_signal(h, 1, &word))) return rc;
+      if (sel_word_done(word)) {
+        h->binlen_host = -1;
+        h->count_sent = true;  // the count is in signal word 3
+        return SPMSPV_SUCCESS;
+      }
+      h->binlen_host = sel_word_binlen(word);
+    } else {
+      h->binlen_host = -1;
+      h->gate = Gate{&h->ctl->kernel_choice, kid};
+      gate_restore.on = true;
+    }
+  }
   switch (kid) {
     case SPMSPV_K_COL_NAIVE_ATOMIC:
     case SPMSPV_K_COL_LB_ATOMIC: {
@@ -428,7 +495,7 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
         if ((rc = ensure_binlen_host(h))) return rc;
         hint = h->binlen_host;
       }
-      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, hint, h->nnz, next_ep(h, 5), h->gate);
+      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, hint, h->nnz, next_ep(h, 5), h->gate, h->small_sort_cap);
       h->launches += 6;
       if (y_dense) {
         launch_fill_f32(s, y_dense, m, 0.f, h->gate);
@@ -436,18 +503,49 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       }
       return check_launch(h, "column sort kernel");
     }
-    case SPMSPV_K_COL_BUCKET: {
+    case SPMSPV_K_COL_TWO_LAYER_ATOMIC:
+    case SPMSPV_K_COL_TWO_LAYER_SORT: {
       if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
       if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
-      launch_lb_gather(s, 1, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl,
-                       h->xnnz, nullptr, nullptr, nullptr, reinterpret_cast<int*>(h->sb.keyA),
-                       h->sb.valA, lb_tiles_hint(h));
+      const long long tiles = h->binlen_host >= 0 ? (h->binlen_host + kLbTile - 1) / kLbTile : -1;
+      if (kid == SPMSPV_K_COL_TWO_LAYER_ATOMIC) {
+        float* acc = y_dense ? y_dense : h->acc;
+        if (y_dense) launch_fill_f32(s, y_dense, m, 0.f);
+        launch_tl_gather(s, 0, h->ptr, h->xbeg, h->x_val, h->xnnz, h->tl_starts, h->csc_row, h->csc_val, h->ctl,
+                         acc, h->marks, nullptr, nullptr, tiles);
+        launch_compact(s, nullptr, h->marks, m, acc, y_dense ? 0 : 1, y_key, y_val, h->scan_state, h->ctl,
+                       next_ep(h, 1), nullptr, Gate{}, count_sig());
+        h->launches += 3;
+        return check_launch(h, "two-layer atomic kernel");
+      }
+      launch_tl_gather(s, 1, h->ptr, h->xbeg, h->x_val, h->xnnz, h->tl_starts, h->csc_row, h->csc_val, h->ctl,
+                       nullptr, nullptr, reinterpret_cast<int*>(h->sb.keyA), h->sb.valA, tiles);
       int* ok = y_key ? y_key : h->yk_tmp;
       float* ov = y_val ? y_val : h->yv_tmp;
       if ((rc = ensure_binlen_host(h))) return rc;
-      launch_bucket(s, h->sb, m, h->bucket_bits, h->ctl, h->acc, h->marks, h->binlen_host, h->nnz, next_ep(h, 1));
-      launch_compact(s, nullptr, h->marks, m, h->acc, 1, ok, ov, h->scan_state, h->ctl, next_ep(h, 1));
-      h->launches += 6;
+      launch_sort_reduce(s, h->sb, m, h->ctl, ok, ov, h->binlen_host, h->nnz, next_ep(h, 5), Gate{},
+                         h->small_sort_cap);
+      h->launches += 7;
+      if (y_dense) {
+        launch_fill_f32(s, y_dense, m, 0.f);
+        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
+      }
+      return check_launch(h, "two-layer sort kernel");
+    }
+    case SPMSPV_K_COL_BUCKET: {
+      if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+      if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
+      int* ok = y_key ? y_key : h->yk_tmp;
+      float* ov = y_val ? y_val : h->yv_tmp;
+      // grids from the host's bin_len when known, else from the bound nnz
+      if (!launch_bucket(s, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl, h->xnnz, m,
+                         h->bucket_bits, h->bin_ws, h->sb.keyB, h->sb.valB, h->acc, h->marks, lb_tiles_hint(h),
+                         h->binlen_host, h->nnz))
+        return set_error(h, SPMSPV_INVALID_ARGUMENT,
+                         "bucket slot: more than " + std::to_string(kBinMax << kBinRowBits) + " rows");
+      launch_compact(s, nullptr, h->marks, m, h->acc, 1, ok, ov, h->scan_state, h->ctl, next_ep(h, 1), nullptr,
+                     Gate{}, y_dense ? nullptr : count_sig());
+      h->launches += 5;
       if (y_dense) {
         launch_fill_f32(s, y_dense, m, 0.f);
         launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
@@ -605,6 +703,7 @@ int spmspv_destroy(spmspv_handle_t h) {
   (void)hipStreamSynchronize(h->stream);
   spmspv_bfs_release(h);
   pr_release(h);
+  ms_release(h);
   if (h->arena) (void)hipFree(h->arena);
   if (h->csc_alloc) (void)hipFree(h->csc_alloc);
   if (h->flush_buf) (void)hipFree(h->flush_buf);
@@ -711,7 +810,7 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
   int rc = ensure_arena(h);
   if (rc) return rc;
   // Everything a later call could allocate is reserved here: the CSR5 copy
-  // at its largest width, the BFS and PageRank buffers and a batch stash of
+  // at its largest width, the BFS, multi-source BFS and PageRank buffers and a batch stash of
   // kBatchReserve vectors.
   if (h->csr_row) {
     size_t most = 0;
@@ -726,7 +825,7 @@ int spmspv_alloc_pre_buffer(spmspv_handle_t h) {
       h->csr5_bytes = most;
     }
   }
-  if (h->m == h->n && ((rc = bfs_alloc(h)) || (rc = pr_alloc(h)))) return rc;
+  if (h->m == h->n && ((rc = bfs_alloc(h)) || (rc = pr_alloc(h)) || (rc = ms_alloc(h)))) return rc;
   if (h->batch_cap < spmspv_handle_s::kBatchReserve) {
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     if (h->batch_stash) HIP_TRY(h, hipFree(h->batch_stash));
@@ -1083,6 +1182,21 @@ int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* k, float* 
   }
   return spmspv_run(h, SPMSPV_K_COL_BUCKET, alpha, nullptr, k, v, ynnz);
 }
+int spmspv_csc_my_nosort(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz, float* d_y,
+                         int* k, float* v) {
+  (void)is_binlen_computed;
+  return spmspv_run(h, SPMSPV_K_COL_TWO_LAYER_ATOMIC, alpha, d_y, k, v, ynnz);
+}
+int spmspv_csc_my_nosort_keyval(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz, float* d_y,
+                                int* k, float* v) {
+  (void)is_binlen_computed;
+  return spmspv_run(h, SPMSPV_K_COL_TWO_LAYER_ATOMIC, alpha, d_y, k, v, ynnz);
+}
+int spmspv_csc_sort_my(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz, float* d_y, int* k,
+                       float* v) {
+  (void)is_binlen_computed;
+  return spmspv_run(h, SPMSPV_K_COL_TWO_LAYER_SORT, alpha, d_y, k, v, ynnz);
+}
 int spmspv_naivespmspv(spmspv_handle_t h, float alpha, float* d_y) {
   return spmspv_run(h, SPMSPV_K_ROW_NAIVE, alpha, d_y, nullptr, nullptr, nullptr);
 }
@@ -1211,14 +1325,29 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   if ((rc = ensure_arena(h)) || (rc = need_csc(h)) || (rc = ensure_mat_features(h)) ||
       (rc = need_sparse_x(h)))
     return rc;
+  // a large x's features-only launch may return the sampled estimate of
+  // bin_len (only row slots, which need none, skip the exact column scan)
   SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
-                   h->sig_dev + 1};
+                   h->sig_dev + 1, 1, h->prev_choice, h->switch_penalty, h->bucket_lo, h->bucket_hi};
   h->enc5_clear_pending = -1;
   __atomic_store_n(h->sig + 1, kSigPending, __ATOMIC_RELEASE);
   const int cls = size_class(h->xnnz);
   const int guess = h->spec_last[cls] - 1;
-  const bool spec = h->speculate && guess >= 0 && speculable(h, guess);
-  if (spec && !needs_scan(guess)) {
+  // a small x with a sparse result goes through the single-launch path,
+  // which also selects when the products do not fit; when the last call of
+  // this size class fitted, nothing is queued behind it
+  // (a selector pinned to a row slot keeps it: the path computes a column slot's result)
+  const bool fused = h->small_x_path && h->fuse_cap > 0 && !d_y_dense && h->xnnz <= kFuseKeys &&
+                     !(h->sel_mode == 2 && (h->sel_fixed < 0 || h->sel_fixed > SPMSPV_K_COL_LB_SORT));
+  const bool spec = h->speculate && guess >= 0 && speculable(h, guess) && !(fused && h->spec_fused[cls]);
+  if (fused) {
+    launch_col_fused(h->stream, h->csc_colptr, h->csc_row, h->csc_val, h->x_key, h->x_val, h->n, h->xnnz, h->m,
+                     h->ptr, h->xbeg, h->split, h->ctl, sel, d_y_key ? d_y_key : h->yk_tmp,
+                     d_y_val ? d_y_val : h->yv_tmp, nullptr, h->fuse_cap);
+    h->launches++;
+    if ((rc = check_launch(h, "small-x column path"))) return rc;
+    h->prep_valid = true;
+  } else if (spec && !needs_scan(guess)) {
     // the guessed slot needs none of the column scan: features only, with
     // x's support encoding (which the row slots read) scattered on the way
     const bool row = guess >= SPMSPV_K_ROW_NAIVE && guess <= SPMSPV_K_SPMV_LB;
@@ -1263,10 +1392,19 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   const int clear_pending = h->enc5_clear_pending;
   h->enc5_clear_pending = -1;
   if ((rc = wait_signal(h, 1, &word))) return rc;
-  h->binlen_host = sel_word_binlen(word);
   const int kid = sel_word_choice(word);
-  if (kid < 0 || kid >= SPMSPV_NUM_SLOTS)
+  if (kid < 0 || kid > SPMSPV_K_COL_BUCKET)  // the eight classes or the bucket rule's slot
     return set_error(h, SPMSPV_DEVICE_ERROR, "selector returned no slot");
+  if (fused) h->spec_fused[cls] = sel_word_done(word);
+  h->prev_choice = kid;
+  if (sel_word_done(word)) {
+    // the single launch finished the call: its word carries y_nnz
+    h->binlen_host = -1;
+    if (kernel_used) *kernel_used = kid;
+    if (y_nnz) *y_nnz = (int)sel_word_binlen(word);
+    return SPMSPV_SUCCESS;
+  }
+  h->binlen_host = sel_word_estimated(word) ? -1 : sel_word_binlen(word);
   h->spec_last[cls] = (signed char)(kid + 1);
   if (kernel_used) *kernel_used = kid;
   if (spec && kid == guess) {
@@ -1282,6 +1420,42 @@ int spmspv_run_auto(spmspv_handle_t h, float alpha, float* d_y_dense, int* d_y_k
   }
   if (spec) h->spec_misses++;
   return spmspv_run(h, kid, alpha, d_y_dense, d_y_key, d_y_val, y_nnz);
+}
+
+int spmspv_autotune_report(spmspv_handle_t h, int out[10]) {
+  if (!h || !out) return set_error(h, SPMSPV_INVALID_ARGUMENT, "null argument");
+  out[0] = h->team_col;
+  out[1] = h->team_row;
+  out[2] = h->row_vt;
+  out[3] = h->spmv_type;
+  out[4] = h->spmv_type == 1 ? csr5_sigma_of(h->sigma, h->m, h->nnz) : 0;
+  out[5] = 1 << h->bucket_bits;
+  out[6] = h->small_x_path ? h->fuse_cap : 0;
+  out[7] = h->small_sort_cap;
+  // the bucket rule's bin_len range (0, -1: off; INT_MAX as the upper end: none)
+  out[8] = (int)std::min<long long>(h->bucket_lo, INT_MAX);
+  out[9] = h->bucket_hi < h->bucket_lo ? -1 : (int)std::min<long long>(h->bucket_hi, INT_MAX);
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_bucket_range(spmspv_handle_t h, long long lo, long long hi) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->bucket_lo = lo;
+  h->bucket_hi = hi;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_switch_penalty(spmspv_handle_t h, float penalty) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (!(penalty >= 0.f)) return set_error(h, SPMSPV_INVALID_ARGUMENT, "penalty must be >= 0");
+  h->switch_penalty = penalty;
+  return SPMSPV_SUCCESS;
+}
+
+int spmspv_set_small_x_path(spmspv_handle_t h, int on) {
+  if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  h->small_x_path = on != 0;
+  return SPMSPV_SUCCESS;
 }
 
 int spmspv_set_speculation(spmspv_handle_t h, int on) {
@@ -1326,7 +1500,8 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
     std::vector<float> ms;
     for (int r = 0; r <= reps; ++r) {
       HIP_TRY(h, hipEventRecord(h->ev0, h->stream));
-      int e = run_kernel(h, kid, nullptr, h->yk_tmp, h->yv_tmp, true);
+      int cnt = 0;  // as a caller runs it: the count comes back to the host
+      int e = spmspv_run(h, kid, 1.f, nullptr, h->yk_tmp, h->yv_tmp, &cnt);
       if (e) return e;
       HIP_TRY(h, hipEventRecord(h->ev1, h->stream));
       HIP_TRY(h, hipEventSynchronize(h->ev1));
@@ -1345,6 +1520,9 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   const int teams[] = {2, 4, 8, 16, 32, 64};
   float best = 1e30f, t = 0.f;
   int pick = h->team_col;
+  // the team and tile candidates are timed on their own launches
+  const bool saved_small = h->small_x_path;
+  h->small_x_path = false;
   for (int tc : teams) {
     h->team_col = tc;
     if ((rc = time_slot(SPMSPV_K_COL_NAIVE_ATOMIC, &t))) return rc;
@@ -1401,9 +1579,138 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
     h->spmv_type = type;
     h->sigma = sigma;
     if (type == 1 && (rc = ensure_csr5(h))) return rc;
-    // captured BFS levels hold the old tiling
-    spmspv_bfs_release(h);
   }
+  h->small_x_path = saved_small;
+  // the bucket slot's number of row buckets
+  {
+    float bbest = 1e30f;
+    int bpick = h->bucket_bits;
+    for (int bb = 4; bb <= 8; ++bb) {
+      h->bucket_bits = bb;
+      if ((rc = time_slot(SPMSPV_K_COL_BUCKET, &t))) return rc;
+      if (better(t, bbest)) {
+        bbest = t;
+        bpick = bb;
+      }
+    }
+    h->bucket_bits = bpick;
+  }
+  // Thresholds of the single-workgroup paths, timed on sub-vectors of x
+  // whose bin_len lies just under each candidate: the largest candidate at
+  // which the single workgroup still beats the multi-launch path is kept.
+  // A sub-vector takes k keys evenly spaced over x (written to the handle's
+  // internal x buffers), so its columns are spread like those of a sparse x
+  // of that size; the first k keys of a dense x would be k adjacent columns,
+  // whose contiguous reads flatter the single workgroup. When x itself lives
+  // in the internal buffers, prefixes of x are used.
+  int* xk = h->x_key;
+  float* xv = h->x_val;
+  const int xn = h->xnnz;
+  const bool spread = xk != h->xk_int && xv != h->xv_int;
+  auto subset = [&](int k) -> int {
+    if (!spread) return spmspv_set_sparse_x(h, xk, xv, k);
+    if (k > 0)
+      hipLaunchKernelGGL(k_spread_pick, dim3((k + 255) / 256), dim3(256), 0, h->stream, xk, xv, xn, k, h->xk_int,
+                         h->xv_int);
+    int r = check_launch(h, "autotune sub-vector");
+    return r ? r : spmspv_set_sparse_x(h, h->xk_int, h->xv_int, k);
+  };
+  auto prefix = [&](long long target, int kmax, int* kout, long long* blout) -> int {
+    int lo = 0, hi = kmax, r;  // bin_len(sub-vector of lo keys) <= target
+    long long bl = 0;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if ((r = subset(mid)) || (r = spmspv_vec_features(h, &bl, nullptr, nullptr))) return r;
+      if (bl <= target) lo = mid;
+      else hi = mid - 1;
+    }
+    if ((r = subset(lo)) || (r = spmspv_vec_features(h, &bl, nullptr, nullptr))) return r;
+    *kout = lo;
+    *blout = bl;
+    return SPMSPV_SUCCESS;
+  };
+  {
+    int cap = 0;
+    for (int c : {2048, 4096, 8192, kFuseCap}) {
+      int k;
+      long long bl;
+      if ((rc = prefix(c, std::min(xn, kFuseKeys), &k, &bl))) return rc;
+      if (bl < c / 2) break;  // x holds no prefix near this size
+      float tf = 0.f, t2 = 0.f, t3 = 0.f;
+      h->small_x_path = true;
+      h->fuse_cap = kFuseCap;
+      if ((rc = time_slot(SPMSPV_K_COL_LB_SORT, &tf))) return rc;
+      h->small_x_path = false;
+      if ((rc = time_slot(SPMSPV_K_COL_LB_ATOMIC, &t2)) || (rc = time_slot(SPMSPV_K_COL_LB_SORT, &t3))) return rc;
+      if (!(tf < std::min(t2, t3))) break;
+      cap = c;
+    }
+    h->small_x_path = true;
+    h->fuse_cap = cap > 0 ? cap : h->fuse_cap;
+  }
+  {
+    const bool saved_path = h->small_x_path;
+    h->small_x_path = false;
+    int cap = 0;
+    for (int c : {1024, 2048, kSmallSortCap}) {
+      int k;
+      long long bl;
+      if ((rc = prefix(c, xn, &k, &bl))) return rc;
+      if (bl < c / 2) break;
+      float t1 = 0.f, tm = 0.f;
+      h->small_sort_cap = kSmallSortCap;
+      if ((rc = time_slot(SPMSPV_K_COL_LB_SORT, &t1))) return rc;
+      h->small_sort_cap = 0;
+      if ((rc = time_slot(SPMSPV_K_COL_LB_SORT, &tm))) return rc;
+      if (!(t1 < tm)) break;
+      cap = c;
+    }
+    h->small_x_path = saved_path;
+    h->small_sort_cap = cap > 0 ? cap : kSmallSortCap;
+  }
+  // The bin_len range of the bucket rule: on sub-vectors of x whose bin_len
+  // lies just under nnz / 4^k (k = 5..0), the bucket slot against the
+  // load-balanced column and row slots (2, 3, 5, 7), which the rule
+  // replaces; the range spans the sizes where it is more than 2 % faster
+  // than all four, its ends half-way (geometrically) to the nearest sizes
+  // where it is not.
+  {
+    h->bucket_lo = 0;
+    h->bucket_hi = -1;
+    long long prev_lose = -1, lo = -1, hi = -1;
+    bool done = false;
+    for (int k = 5; k >= 0 && !done; --k) {
+      const long long c = std::max<long long>(1, (long long)h->nnz >> (2 * k));
+      int kk;
+      long long bl;
+      if ((rc = prefix(c, xn, &kk, &bl))) return rc;
+      if (bl < c / 2 || bl <= 0) continue;  // x holds no sub-vector near this size
+      float tb = 0.f, best = 1e30f;
+      if ((rc = time_slot(SPMSPV_K_COL_BUCKET, &tb))) return rc;
+      for (int sl : {SPMSPV_K_COL_LB_ATOMIC, SPMSPV_K_COL_LB_SORT, SPMSPV_K_ROW_LB, SPMSPV_K_SPMV_LB}) {
+        float t = 0.f;
+        if ((rc = time_slot(sl, &t))) return rc;
+        best = std::min(best, t);
+      }
+      if (better(tb, best)) {
+        if (lo < 0) lo = prev_lose < 0 ? 1 : (long long)std::sqrt((double)prev_lose * (double)bl);
+        hi = bl;
+      } else {
+        if (lo >= 0) {
+          hi = (long long)std::sqrt((double)hi * (double)bl);
+          done = true;  // the range ends here
+        }
+        prev_lose = bl;
+      }
+    }
+    if (lo >= 0) {
+      h->bucket_lo = lo;
+      h->bucket_hi = done ? hi : LLONG_MAX;  // still ahead at the largest size: no upper end
+    }
+  }
+  if ((rc = spmspv_set_sparse_x(h, xk, xv, xn))) return rc;
+  // captured BFS levels hold the old tiling and thresholds
+  bfs_drop_graphs(h);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   if (report) {
     report[0] = h->team_col;

@@ -65,7 +65,7 @@ int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int
                                   const_cast<float*>(d_vval) + vptr[v], xn)))
       return rc;
     SelectorArgs sel{h->have_weights ? h->weights_dev : nullptr, h->mat, h->sel_mode, h->sel_fixed,
-                     h->batch_stash + v};
+                     h->batch_stash + v, 0, -1, 0.f, h->bucket_lo, h->bucket_hi};
     // features only: the chosen pipeline runs the column scan itself when
     // its slot needs one (the scan buffers hold one vector at a time)
     launch_select_feat(h->stream, h->csc_colptr, h->x_key, h->x_val, h->n, h->xnnz, h->sel_part, h->ctl,
@@ -84,7 +84,7 @@ int spmspv_run_batched(spmspv_handle_t h, int nvec, const int* d_vptr, const int
                        "run_batched: output capacity of vector " + std::to_string(v) + " is " +
                            std::to_string(yoff[v + 1] - yoff[v]) + ", needs " + std::to_string(need));
     const int kid = sel_word_choice(st[v]);
-    if (kid < 0 || kid >= SPMSPV_NUM_SLOTS)
+    if (kid < 0 || kid > SPMSPV_K_COL_BUCKET)
       return set_error(h, SPMSPV_DEVICE_ERROR, "run_batched: selector returned no slot");
   }
   // 2. queued pipelines

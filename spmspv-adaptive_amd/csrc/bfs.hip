@@ -3,16 +3,18 @@
 // replays frontier files recorded elsewhere (la/test/test_application.cu:
 // 207-261, frontier format readsvmdata.hpp:308-348), so this driver is new.
 //
-// A level runs the boolean form of the slot chosen for it:
+// A level runs the boolean form of the slot chosen for it (row and column
+// indices only: no matrix value is read and no FP32 arithmetic is done):
 //   K0/K2  top-down: frontier vertices scan their columns and claim
 //          unvisited rows (naive team per vertex / merge-path balanced);
-//   K1/K3  top-down gather of all neighbours, in-house radix sort + unique,
-//          then claim (the next frontier comes out sorted);
+//   K1/K3  top-down gather of all neighbours' row keys, in-house radix sort
+//          of the keys alone, then each distinct row claimed;
 //   K4     bottom-up: each unvisited row (one lane) scans its neighbours
 //          and stops at the first frontier member;
-//   K5/K7  merge-path row tiles testing the frontier bit array, then claim
-//          (on booleans the masked and unmasked row kernels coincide);
-//   K6     naive row scan without early exit, then claim.
+//   K5/K7  merge-path row tiles OR-ing the frontier bits of each row's
+//          columns into a hit bit, then claim (on booleans the masked and
+//          unmasked row kernels coincide);
+//   K6     naive row scan without early exit, hit bits, then claim.
 // Levels are int32 and unique, so any order of claims gives the same
 // levels. The selector for level L+1 runs in the same stream right after
 // level L (features of the new frontier + MLP, one launch), so a level
@@ -37,8 +39,6 @@ struct BfsState {
   int* cnt = nullptr;                       // [0],[1] sizes, [2] level, [3] bottom-up queue size
   int* queue = nullptr;                     // m: rows the bottom-up head left undecided
   unsigned long long* reached = nullptr;    // [0] vertices, [1] degree sum
-  float* ybuf = nullptr;                    // m, scratch output of row kernels
-  float* ones = nullptr;                    // n ones: x values of a frontier
   void* alloc = nullptr;
   hipStream_t own_stream = nullptr;
   int* levels_bound = nullptr;              // levels array the graphs were captured with
@@ -218,9 +218,11 @@ __global__ __launch_bounds__(NT) void k_td_lb(const int* __restrict__ ptr,
 }
 
 // After the sort slots: claims the unique neighbour keys.
-__global__ void k_claim_list(const int* __restrict__ keys, const Ctl* ctl, int* cnt, int cur,
-                             unsigned* visited, int* levels, int* nf, unsigned* nbits) {
-  const int k = ctl->ynnz;
+// Claims the distinct rows of a sorted key list of *n_dev items (repeats
+// are skipped without a test-and-set).
+__global__ void k_claim_sorted(const unsigned* __restrict__ keys, const long long* __restrict__ n_dev,
+                               int* cnt, int cur, unsigned* visited, int* levels, int* nf, unsigned* nbits) {
+  const long long k = *n_dev;
   const int level = cnt[2];
   int* ncnt = cnt + (cur ^ 1);
   for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < k; i0 += (long long)gridDim.x * blockDim.x) {
@@ -228,8 +230,8 @@ __global__ void k_claim_list(const int* __restrict__ keys, const Ctl* ctl, int* 
     bool cl = false;
     int v = 0;
     if (i < k) {
-      v = keys[i];
-      cl = try_claim(visited, v);
+      v = (int)keys[i];
+      if (i == 0 || keys[i - 1] != (unsigned)v) cl = try_claim(visited, v);
     }
     record(v, cl, level, levels, nf, ncnt, nbits);
   }
@@ -406,16 +408,21 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
       break;
     case SPMSPV_K_COL_NAIVE_SORT:
     case SPMSPV_K_COL_LB_SORT: {
+      // boolean: the neighbours' row keys alone, sorted, then each distinct
+      // row claimed
       int* bk = reinterpret_cast<int*>(h->sb.keyA);
       if (slot == SPMSPV_K_COL_NAIVE_SORT)
-        launch_naive_col(s, 1, h->csc_colptr, h->csc_row, h->csc_val, f, b->ones, h->ptr, h->n, 0,
-                         b->cnt + cur, h->team_col, nullptr, nullptr, bk, h->sb.valA);
+        launch_naive_col(s, 2, h->csc_colptr, h->csc_row, nullptr, f, nullptr, h->ptr, h->n, 0,
+                         b->cnt + cur, h->team_col, nullptr, nullptr, bk, nullptr);
       else
-        launch_lb_gather(s, 1, h->ptr, h->xbeg, b->ones, h->split, h->csc_row, h->csc_val, h->ctl, 0,
-                         b->cnt + cur, nullptr, nullptr, bk, h->sb.valA, -1);
-      launch_sort_reduce(s, h->sb, h->m, h->ctl, h->yk_tmp, h->yv_tmp, -1, h->nnz, Ep{0, 1});
-      hipLaunchKernelGGL(k_claim_list, dim3(kBfsGrid), dim3(256), 0, s, h->yk_tmp, h->ctl, b->cnt, cur,
-                         b->visited, levels, nf, nbits);
+        launch_lb_gather(s, 2, h->ptr, h->xbeg, nullptr, h->split, h->csc_row, nullptr, h->ctl, 0,
+                         b->cnt + cur, nullptr, nullptr, bk, nullptr, -1);
+      SortBufs kb = h->sb;
+      kb.valA = kb.valB = nullptr;
+      const int fb = launch_radix_sort(s, kb, h->m, &h->ctl->bin_len, h->ctl, -1, h->nnz, false, Ep{0, 1},
+                                       Gate{}, h->small_sort_cap);
+      hipLaunchKernelGGL(k_claim_sorted, dim3(kBfsGrid), dim3(256), 0, s, fb ? kb.keyB : kb.keyA,
+                         &h->ctl->bin_len, b->cnt, cur, b->visited, levels, nf, nbits);
       break;
     }
     case SPMSPV_K_ROW_NAIVE:
@@ -426,13 +433,13 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
                            h->csr_col, b->fbits[cur], b->cnt, cur, b->visited, levels, nf, nbits, b->queue);
       }
       break;
-    default: {  // 5, 6, 7: row kernels with hit tracking, then claim
+    default: {  // 5, 6, 7: boolean row kernels (hit bits only), then claim
       if (slot == SPMSPV_K_SPMV_NAIVE)
-        launch_row_naive(s, true, true, h->csr_row, h->csr_col, h->csr_val, b->ones, b->fbits[cur],
-                         h->m, h->team_row, b->ybuf, h->bitmap);
+        launch_row_naive(s, true, true, h->csr_row, h->csr_col, nullptr, nullptr, b->fbits[cur],
+                         h->m, h->team_row, nullptr, h->bitmap);
       else
-        launch_row_merge(s, true, true, h->csr_row, h->csr_col, h->csr_val, nullptr, b->fbits[cur],
-                         h->m, h->nnz, h->row_split, h->n_row_tiles, h->row_vt, b->ybuf, h->bitmap,
+        launch_row_merge(s, true, true, h->csr_row, h->csr_col, nullptr, nullptr, b->fbits[cur],
+                         h->m, h->nnz, h->row_split, h->n_row_tiles, h->row_vt, nullptr, h->bitmap,
                          h->row_carry_val, h->row_carry_row, h->row_carry_hit);
       hipLaunchKernelGGL(k_claim_bits, dim3(std::min(kBfsGrid, (words + 255) / 256)), dim3(256), 0, s,
                          h->bitmap, h->m, b->cnt, cur, b->visited, levels, nf, nbits);
@@ -463,7 +470,7 @@ int bfs_alloc(spmspv_handle_t h) {
   };
   const size_t o_vis = take(words * 4), o_fb0 = take(words * 4), o_fb1 = take(words * 4);
   const size_t o_f0 = take(n * 4 + 4), o_f1 = take(n * 4 + 4), o_cnt = take(64), o_r = take(64);
-  const size_t o_y = take(m * 4 + 4), o_ones = take(n * 4 + 4), o_q = take(m * 4 + 4);
+  const size_t o_q = take(m * 4 + 4);
   if (hipMalloc(&b->alloc, off) != hipSuccess) {
     delete b;
     return set_error(h, SPMSPV_OUT_OF_MEMORY, "bfs buffers");
@@ -476,8 +483,6 @@ int bfs_alloc(spmspv_handle_t h) {
   b->front[1] = reinterpret_cast<int*>(p + o_f1);
   b->cnt = reinterpret_cast<int*>(p + o_cnt);
   b->reached = reinterpret_cast<unsigned long long*>(p + o_r);
-  b->ybuf = reinterpret_cast<float*>(p + o_y);
-  b->ones = reinterpret_cast<float*>(p + o_ones);
   b->queue = reinterpret_cast<int*>(p + o_q);
   if (!h->stream) {
     if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -486,10 +491,6 @@ int bfs_alloc(spmspv_handle_t h) {
       return set_error(h, SPMSPV_DEVICE_ERROR, "bfs stream");
     }
   }
-  hipStream_t s = h->stream ? h->stream : b->own_stream;
-  launch_fill_f32(s, b->ones, (long long)n, 1.0f);
-  launch_fill_f32(s, b->ybuf, (long long)m, 0.0f);
-  if (hipStreamSynchronize(s) != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, "bfs init");
   h->bfs = b;
   return SPMSPV_SUCCESS;
 }

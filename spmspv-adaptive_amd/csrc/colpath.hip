@@ -23,6 +23,37 @@
 
 namespace spmspv {
 
+// Selector inputs of a large x from kSampleKeys evenly spaced keys (the
+// definition in kernels.hpp), by the whole workgroup; every thread gets
+// the results.
+template <int NT>
+__device__ void sample_x(const int* __restrict__ colptr, const int* __restrict__ xkey, int ncols, int xnnz,
+                         long long* s_l, int* s_i, long long* bl, int* mx) {
+  static_assert(kSampleKeys % NT == 0, "whole samples per thread");
+  constexpr int PER = kSampleKeys / NT;
+  int c[PER], b[PER], e[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) c[j] = xkey[(long long)(threadIdx.x + j * NT) * xnnz / kSampleKeys];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int ok = (unsigned)c[j] < (unsigned)ncols ? 1 : 0;
+    const int ci = c[j] & -ok;
+    b[j] = colptr[ci];
+    e[j] = colptr[ci + ok];
+  }
+  long long sum = 0;
+  int m = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    sum += e[j] - b[j];
+    m = max(m, e[j] - b[j]);
+  }
+  sum = block_sum<NT, long long>(sum, s_l);
+  m = block_max<NT, int>(m, s_i);
+  *bl = (long long)llrint((double)sum * (double)xnnz / (double)kSampleKeys);
+  *mx = m;
+}
+
 // ------------------------------------------------------------------ prep
 // Ordered tiles (tile = blockIdx.x). Each tile publishes its max/min column
 // length before its look-back aggregate; the last tile, whose look-back has
@@ -34,7 +65,7 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
                                              const int* __restrict__ xnnz_dev, int* __restrict__ ptr,
                                              int* __restrict__ xbeg, int* __restrict__ split,
                                              unsigned long long* state, int* tile_mm, Ctl* ctl,
-                                             Ep ep, SelectorArgs sel) {
+                                             Ep ep, SelectorArgs sel, long long* samp) {
   __shared__ unsigned long long s_pre;
   __shared__ long long s_l[NT / kWave];
   __shared__ int s_i[NT / kWave];
@@ -101,6 +132,19 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   }
   if (t >= ntiles) return;
   const unsigned epoch = ep_of(ctl, ep);
+  // a large x (ntiles > kSampT) samples in its first kSampT tiles, one key
+  // per thread, beside the scan (the positions sample_x takes)
+  constexpr int kSampT = kSampleKeys / NT;
+  const bool sampled = sel.mat != nullptr && xnnz > kSampleMin;
+  const bool my_samp = sampled && samp && t < (unsigned)kSampT;
+  int slen = 0;
+  if (my_samp) {
+    const long long sidx = (long long)t * NT + threadIdx.x;
+    const int sc = xkey[sidx * xnnz / kSampleKeys];
+    const int sok = (unsigned)sc < (unsigned)ncols ? 1 : 0;
+    const int sci = sc & -sok;
+    slen = colptr[sci + sok] - colptr[sci];
+  }
   const long long base = (long long)t * (NT * IPT) + (long long)threadIdx.x * IPT;
   int len[IPT], beg[IPT];
   long long tsum = 0;
@@ -140,6 +184,12 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
   if (selects) stage();
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
+  long long ssum = 0;
+  int smx = 0;
+  if (my_samp) {
+    ssum = block_sum<NT, long long>((long long)slen, s_l);
+    smx = block_max<NT, int>(slen, s_i);
+  }
   // a single tile (small x) needs neither the look-back nor the tile max/min
   const bool single = ntiles == 1;
   if (!single && threadIdx.x == 0) {
@@ -148,6 +198,10 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     // the look-back has seen every earlier tile (no cache write-back fence)
     __hip_atomic_store(&tile_mm[2 * t], mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&tile_mm[2 * t + 1], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (my_samp) {
+      __hip_atomic_store(&samp[2 * t], ssum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&samp[2 * t + 1], (long long)smx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   }
   long long btot;
@@ -199,7 +253,28 @@ __global__ __launch_bounds__(NT) void k_prep(const int* __restrict__ colptr,
     }
   }
   __syncthreads();
-  run_selector(xnnz, (long long)pre + btot, gmx);
+  // a large x's selector inputs are sampled (the same definition in every
+  // launch that selects); the selector word keeps the exact bin_len
+  long long fbl = (long long)pre + btot;
+  int fmx = gmx;
+  if (selects && sampled) {
+    if (samp && ntiles > (unsigned)kSampT) {
+      // the first tiles' sample partials, in place since the look-back
+      // has seen those tiles publish
+      long long os = 0;
+      int om = 0;
+      for (int u = threadIdx.x; u < kSampT; u += NT) {
+        os += __hip_atomic_load(&samp[2 * u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        om = max(om, (int)__hip_atomic_load(&samp[2 * u + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      }
+      const long long st = block_sum<NT, long long>(os, s_l);
+      fmx = block_max<NT, int>(om, s_i);
+      fbl = (long long)llrint((double)st * (double)xnnz / (double)kSampleKeys);
+    } else {
+      sample_x<NT>(colptr, xkey, ncols, xnnz, s_l, s_i, &fbl, &fmx);
+    }
+  }
+  run_selector(xnnz, fbl, fmx);
   // the selector's choice was written by thread 0 (lane 0 of wave 0)
   if (sel.stash && threadIdx.x == 0)
     *sel.stash = sel_word((long long)pre + btot, sel.mat ? ctl->kernel_choice : -1);
@@ -241,6 +316,22 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
     if (threadIdx.x < kMF) mfr = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
   }
   const int w = threadIdx.x / kWave;
+  // for a large x, this thread's share of the sampled keys (the positions
+  // sample_x takes), whose loads go out with the scan's
+  const bool sampled = selects && xnnz > kSampleMin;
+  long long ssum = 0;
+  int smx = 0;
+  if (sampled) {
+    for (long long sidx = (long long)blockIdx.x * NT + threadIdx.x; sidx < kSampleKeys;
+         sidx += (long long)gridDim.x * NT) {
+      const int c = xkey[sidx * xnnz / kSampleKeys];
+      const int ok = (unsigned)c < (unsigned)ncols ? 1 : 0;
+      const int ci = c & -ok;
+      const int len = colptr[ci + ok] - colptr[ci];
+      ssum += len;
+      smx = max(smx, len);
+    }
+  }
   long long sum = 0;
   int mx = 0, mn = INT_MAX;
   // the support encoding of a round is scattered at the start of the next
@@ -309,12 +400,21 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
   sum = block_sum<NT, long long>(sum, s_l);
   mx = block_max<NT, int>(mx, s_i);
   mn = block_min<NT, int>(mn, s_i);
+  if (sampled) {
+    ssum = block_sum<NT, long long>(ssum, s_l);
+    smx = block_max<NT, int>(smx, s_i);
+  }
   if (threadIdx.x == 0) {
     __hip_atomic_store(&part[blockIdx.x], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&part[kSelMaxGrid + blockIdx.x], (long long)mx, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&part[2 * kSelMaxGrid + blockIdx.x], (long long)mn, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
+    if (sampled) {
+      __hip_atomic_store(&part[3 * kSelMaxGrid + blockIdx.x], ssum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part[4 * kSelMaxGrid + blockIdx.x], (long long)smx, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // partials in place before the count
     const unsigned before =
         __hip_atomic_fetch_add(&ctl->done[kSlotSelect], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -355,14 +455,165 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
   SelectorArgs sl = sel;
   if (sel.weights) sl.weights = s_w;
   sl.mat = &s_mf;
-  select_in_wave(ctl, sl, (double)xnnz, (double)sum, (double)mx);
+  long long fbl = sum;
+  int fmx = mx;
+  if (sampled) {
+    // the sampled inputs: the other workgroups' partials and this one's
+    long long os = 0;
+    int om = 0;
+    for (unsigned u = threadIdx.x; u < gridDim.x; u += NT) {
+      if (u == blockIdx.x) continue;
+      os += __hip_atomic_load(&part[3 * kSelMaxGrid + u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      om = max(om, (int)__hip_atomic_load(&part[4 * kSelMaxGrid + u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT));
+    }
+    const long long st = ssum + block_sum<NT, long long>(os, s_l);
+    fmx = max(smx, block_max<NT, int>(om, s_i));
+    fbl = (long long)llrint((double)st * (double)xnnz / (double)kSampleKeys);
+  }
+  select_in_wave(ctl, sl, (double)xnnz, (double)fbl, (double)fmx);
   if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(sum, ctl->kernel_choice);
   if (pending) scatter();
+}
+
+// Features-only selector for a large x (xnnz > kSampleMin) when the caller
+// takes an estimated bin_len: the first kSampleKeys / NT workgroups take one
+// sampled key per thread, publish their partial sum and maximum and count
+// themselves done, and the last of them runs the selector; only then does
+// any workgroup scatter its share of x's support encoding, so the network
+// runs while the scatter's stores are in flight (no workgroup waits for
+// another). Sampling inside one workgroup made that workgroup's 12,000
+// cache-line fills the kernel's critical path, and sampling after the
+// scatter put the network behind it. The selector word carries the
+// estimate, flagged.
+template <int NT, int KPT>
+__global__ __launch_bounds__(NT) void k_select_sampled(const int* __restrict__ colptr,
+                                                       const int* __restrict__ xkey,
+                                                       const float* __restrict__ xval, int ncols, int xnnz,
+                                                       long long* part, Ctl* ctl, SelectorArgs sel,
+                                                       float* __restrict__ enc, const int* __restrict__ enc_perm) {
+  constexpr int kSampT = kSampleKeys / NT;  // the grid has at least this many workgroups
+  __shared__ __align__(16) float s_w[kNWeights];
+  __shared__ MatFeat s_mf;
+  __shared__ long long s_l[NT / kWave];
+  __shared__ int s_i[NT / kWave];
+  __shared__ unsigned s_last;
+  if (blockIdx.x < kSampT) {
+    constexpr int kWPT = (kNWeights + NT - 1) / NT;
+    constexpr int kMF = sizeof(MatFeat) / sizeof(double);
+    static_assert(kMF <= NT, "MatFeat is copied as doubles");
+    // any sampling workgroup may be the last: each holds the selector's inputs
+    float wr[kWPT];
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        wr[i] = idx < kNWeights ? sel.weights[idx] : 0.f;
+      }
+    }
+    const double mfr = threadIdx.x < kMF ? reinterpret_cast<const double*>(sel.mat)[threadIdx.x] : 0.0;
+    // this thread's sample (the positions sample_x takes)
+    const long long sidx = (long long)blockIdx.x * NT + threadIdx.x;
+    const int sc = xkey[sidx * xnnz / kSampleKeys];
+    const int sok = (unsigned)sc < (unsigned)ncols ? 1 : 0;
+    const int sci = sc & -sok;
+    const int len = colptr[sci + sok] - colptr[sci];
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        if (idx < kNWeights) s_w[wlds_index(idx)] = wr[i];
+      }
+    }
+    if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
+    long long sum = block_sum<NT, long long>((long long)len, s_l);
+    int mx = block_max<NT, int>(len, s_i);
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&part[blockIdx.x], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part[kSelMaxGrid + blockIdx.x], (long long)mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // partials in place before the count
+      const unsigned before =
+          __hip_atomic_fetch_add(&ctl->done[kSlotSelect], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_last = before == kSampT - 1 ? 1u : 0u;
+      // the last one resets the count for the next launch (stream-ordered)
+      if (before == kSampT - 1)
+        __hip_atomic_store(&ctl->done[kSlotSelect], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (s_last) {
+      long long osum = 0;
+      int omx = 0;
+      for (int u = threadIdx.x; u < kSampT; u += NT) {
+        if (u == (int)blockIdx.x) continue;
+        osum += __hip_atomic_load(&part[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        omx = max(omx, (int)__hip_atomic_load(&part[kSelMaxGrid + u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      }
+      sum += block_sum<NT, long long>(osum, s_l);
+      mx = max(mx, block_max<NT, int>(omx, s_i));
+      const long long bl = (long long)llrint((double)sum * (double)xnnz / (double)kSampleKeys);
+      if (threadIdx.x == 0) {
+        ctl->bin_len = bl;
+        ctl->max_len = mx;
+        ctl->min_len = 0;
+        ctl->xnnz = xnnz;
+      }
+      SelectorArgs sl = sel;
+      if (sel.weights) sl.weights = s_w;
+      sl.mat = &s_mf;
+      select_in_wave(ctl, sl, (double)xnnz, (double)bl, (double)mx);
+      if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(bl, ctl->kernel_choice | kSelEstimated);
+    }
+  }
+  // this workgroup's share of the support encoding: the workgroups after
+  // the sampling ones scatter (all of them when there are no others)
+  if (!enc) return;
+  const bool others = gridDim.x > (unsigned)kSampT;
+  if (others && blockIdx.x < kSampT) return;
+  const long long sb = others ? blockIdx.x - kSampT : blockIdx.x;
+  const long long sg = others ? gridDim.x - kSampT : gridDim.x;
+  const int w = threadIdx.x / kWave;
+  for (long long t0 = sb * NT * KPT; t0 < xnnz; t0 += sg * NT * KPT) {
+    const long long b0 = t0 + (long long)w * kWave * KPT + lane();
+    int c[KPT], pc[KPT];
+    float xv[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const long long i = b0 + (long long)j * kWave;
+      const long long ic = i < xnnz ? i : (long long)xnnz - 1;
+      c[j] = xkey[ic];
+      xv[j] = xval[ic];
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const int ok = (b0 + (long long)j * kWave < xnnz && (unsigned)c[j] < (unsigned)ncols) ? 1 : 0;
+      const int ci = c[j] & -ok;
+      pc[j] = enc_perm ? enc_perm[ci] : ci;
+      c[j] = ok;
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) asm volatile("" ::"v"(pc[j]), "v"(xv[j]));
+#pragma unroll
+    for (int j = 0; j < KPT; ++j)
+      if (c[j]) enc[pc[j]] = xv[j] == 0.f ? -0.f : xv[j];
+  }
 }
 
 void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
                         int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
                         float* enc, const int* enc_perm) {
+  if (sel.estimate && sel.mat && xnnz > kSampleMin) {
+    constexpr int kPer = 256 * 16;
+    // the sampling workgroups (one key per thread), then with the encoding
+    // one round of 16 entries per thread for the others, kSelMaxGrid
+    // workgroups at most
+    const int sgrid = kSampleKeys / 256;
+    const int grid = enc ? sgrid + (int)std::max<long long>(1, std::min<long long>(kSelMaxGrid - sgrid,
+                                                                                    ((long long)xnnz + kPer - 1) / kPer))
+                         : sgrid;
+    hipLaunchKernelGGL((k_select_sampled<256, 16>), dim3(grid), dim3(256), 0, s, colptr, xkey, xval, ncols,
+                       xnnz, part, ctl, sel, enc, enc_perm);
+    return;
+  }
   // 16 keys per thread fill at most 512 workgroups up to 2 M keys; below
   // 256 K keys, 8 per thread give twice the workgroups and half the chain
   // per thread (3 µs less at 100 K keys, 4 µs more at 1 M)
@@ -382,18 +633,19 @@ void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const
 
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
                  const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
-                 int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep) {
+                 int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep, long long* samp) {
   // exact grid when the host knows xnnz, else the bound from ncols
   const long long cnt = xnnz_dev ? ncols : xnnz;
   const int grid = (int)std::max<long long>(1, (cnt + kPrepTile - 1) / kPrepTile);
   SelectorArgs a{};
   if (sel) a = *sel;
   hipLaunchKernelGGL((k_prep<kPrepNT, kPrepIPT>), dim3(grid), dim3(kPrepNT), 0, s, colptr, xkey,
-                     ncols, xnnz, xnnz_dev, ptr, xbeg, split, state, tile_mm, ctl, ep, a);
+                     ncols, xnnz, xnnz_dev, ptr, xbeg, split, state, tile_mm, ctl, ep, a, samp);
 }
 
 // ------------------------------------------------------------- naive col
-// mode 0: atomics into acc + hit bitmap; mode 1: bins at ptr[e] + offset.
+// mode 0: atomics into acc + hit bitmap; mode 1: bins at ptr[e] + offset;
+// mode 2: row keys only at ptr[e] + offset (boolean: no value is read).
 template <int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr,
                                                   const int* __restrict__ crow,
@@ -414,10 +666,13 @@ __global__ __launch_bounds__(NT) void k_naive_col(const int* __restrict__ colptr
   for (long long e = (long long)blockIdx.x * tpb + tm; e < xnnz; e += (long long)gridDim.x * tpb) {
     const int c = xkey[e];
     if ((unsigned)c >= (unsigned)ncols) continue;
-    const float xv = xval[e];
+    const float xv = MODE == 2 ? 0.f : xval[e];
     const int b = colptr[c];
     const int en = colptr[c + 1];
-    if (MODE == 0) {
+    if (MODE == 2) {
+      const int o = ptr[e] - b;
+      for (int j = b + tid_t; j < en; j += team) bkey[o + j] = crow[j];
+    } else if (MODE == 0) {
       for (int j = b + tid_t; j < en; j += team) {
         const int r = crow[j];
         atomic_add_f32(acc + r, cval[j] * xv);
@@ -445,6 +700,9 @@ void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* cro
   if (grid < 1) grid = 1;
   if (mode == 0)
     hipLaunchKernelGGL((k_naive_col<256, 0>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
+                       xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval, g);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_naive_col<256, 2>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
                        xval, ptr, ncols, xnnz, xnnz_dev, lg, acc, marks, bkey, bval, g);
   else
     hipLaunchKernelGGL((k_naive_col<256, 1>), dim3(grid), dim3(256), 0, s, colptr, crow, cval, xkey,
@@ -496,7 +754,7 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
         const int ic = i < xnnz ? i : (xnnz > 0 ? xnnz - 1 : 0);
         pv[u] = ptr[i];
         bv[u] = xbeg[ic];
-        xv[u] = xval[ic];
+        xv[u] = MODE == 2 ? 0.f : xval[ic];
       }
 #pragma unroll
       for (int u = 0; u < SEGPT; ++u) {
@@ -551,14 +809,16 @@ __global__ __launch_bounds__(NT) void k_lb_gather(const int* __restrict__ ptr,
 #pragma unroll
       for (int k = 0; k < VT; ++k) {
         r[k] = crow[idx[k]];
-        cv[k] = cval[idx[k]];
+        cv[k] = MODE == 2 ? 0.f : cval[idx[k]];
       }
 #pragma unroll
       for (int k = 0; k < VT; ++k) {
         const int q = threadIdx.x + k * NT;
         if (q < np) {
           const float v = cv[k] * xs[k];
-          if (MODE == 0) {
+          if (MODE == 2) {
+            bkey[b0 + q] = r[k];
+          } else if (MODE == 0) {
             atomic_add_f32(acc + r[k], v);
             marks[r[k]] = 1;
           } else {
@@ -581,9 +841,128 @@ void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, 
   if (mode == 0)
     hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 0>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
                        xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval, g);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 2>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
+                       xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval, g);
   else
     hipLaunchKernelGGL((k_lb_gather<kLbNT, kLbVT, 1>), dim3((unsigned)grid), dim3(kLbNT), 0, s, ptr,
                        xbeg, xval, split, crow, cval, ctl, xnnz, xnnz_dev, acc, marks, bkey, bval, g);
+}
+
+// ----------------------------------------------------- two-layer gather
+// The reference's two-layer load balance (CscBasedMyNoSortSpmspv /
+// CscBasedSortMySpmspv, spmspv.h:1823-1878, 2000-2029;
+// csc_two_layer_load_balance_fuse_reduction_modify.cuh:31-394): tiles of
+// exactly T products (the merge path counts x entries as items too).
+// Layer 1, k_tl_starts: per tile, the x entry holding its first product
+// (the last i with ptr[i] <= t*T). Layer 2, k_tl_gather: the tile's x
+// entries are staged in LDS when they fit (empty columns can make a tile
+// span many; then every product searches ptr in memory) and each product
+// finds its entry there by binary search. Products go to the accumulator
+// (atomics + row marks) or to their bin at the global product index (the
+// reference's write-back dropped the block offset, :146-149).
+__global__ void k_tl_starts(const int* __restrict__ ptr, const Ctl* __restrict__ ctl, int xnnz, int tile,
+                            int* __restrict__ starts) {
+  const long long bl = ctl->bin_len;
+  const long long ntiles = (bl + tile - 1) / tile;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles;
+       t += (long long)gridDim.x * blockDim.x) {
+    if (t == ntiles) {
+      starts[t] = xnnz;
+      continue;
+    }
+    const long long p = t * tile;
+    int lo = 0, hi = xnnz;  // ptr[lo] <= p < ptr[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ptr[mid] <= p) lo = mid;
+      else hi = mid;
+    }
+    starts[t] = lo;
+  }
+}
+
+template <int NT, int VT, int MODE>
+__global__ __launch_bounds__(NT) void k_tl_gather(const int* __restrict__ ptr, const int* __restrict__ xbeg,
+                                                  const float* __restrict__ xval,
+                                                  const int* __restrict__ starts,
+                                                  const int* __restrict__ crow,
+                                                  const float* __restrict__ cval,
+                                                  const Ctl* __restrict__ ctl, float* acc,
+                                                  unsigned char* marks, int* __restrict__ bkey,
+                                                  float* __restrict__ bval, Gate g) {
+  if (gate_shut(g)) return;
+  constexpr int T = NT * VT;
+  __shared__ int s_ptr[T + 1];
+  __shared__ int s_beg[T + 1];
+  __shared__ float s_xv[T + 1];
+  const long long bl = ctl->bin_len;
+  const long long ntiles = (bl + T - 1) / T;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long p0 = t * T;
+    const long long p1 = min(p0 + (long long)T, bl);
+    const int e0 = starts[t];
+    // the last product's entry is at most the next tile's first one
+    const int e1 = t + 1 < ntiles ? starts[t + 1] : starts[ntiles] - 1;
+    const int span = e1 - e0 + 1;
+    const bool fits = span <= T + 1;
+    if (fits) {
+      for (int k = threadIdx.x; k < span; k += NT) {
+        s_ptr[k] = ptr[e0 + k];
+        s_beg[k] = xbeg[e0 + k];
+        s_xv[k] = xval[e0 + k];
+      }
+    }
+    __syncthreads();
+    int idx[VT], ent[VT];
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const long long p = p0 + (long long)j * NT + threadIdx.x;
+      int lo = 0, hi = span;  // (ptr of entry e0 + lo) <= p < (ptr of e0 + hi)
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        const int pm = fits ? s_ptr[mid] : ptr[e0 + mid];
+        if (pm <= p) lo = mid;
+        else hi = mid;
+      }
+      ent[j] = lo;
+      const int base = fits ? s_ptr[lo] : ptr[e0 + lo];
+      const int beg = fits ? s_beg[lo] : xbeg[e0 + lo];
+      idx[j] = p < p1 ? beg + (int)(p - base) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      if (idx[j] < 0) continue;
+      const long long p = p0 + (long long)j * NT + threadIdx.x;
+      const float xv = fits ? s_xv[ent[j]] : xval[e0 + ent[j]];
+      const int r = crow[idx[j]];
+      const float v = cval[idx[j]] * xv;
+      if (MODE == 0) {
+        atomic_add_f32(acc + r, v);
+        marks[r] = 1;
+      } else {
+        bkey[p] = r;
+        bval[p] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_tl_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval, int xnnz,
+                      int* starts, const int* crow, const float* cval, const Ctl* ctl, float* acc,
+                      unsigned char* marks, int* bkey, float* bval, long long tiles_hint, Gate g) {
+  constexpr int NT = 256, VT = 7, T = NT * VT;
+  const long long tiles = tiles_hint >= 0 ? tiles_hint : 8192;
+  const int sgrid = (int)std::max<long long>(1, std::min<long long>(1024, (tiles + 1 + 255) / 256));
+  hipLaunchKernelGGL(k_tl_starts, dim3(sgrid), dim3(256), 0, s, ptr, ctl, xnnz, T, starts);
+  const int grid = (int)std::max<long long>(1, std::min<long long>(4096, tiles));
+  if (mode == 0)
+    hipLaunchKernelGGL((k_tl_gather<NT, VT, 0>), dim3(grid), dim3(NT), 0, s, ptr, xbeg, xval, starts, crow, cval,
+                       ctl, acc, marks, bkey, bval, g);
+  else
+    hipLaunchKernelGGL((k_tl_gather<NT, VT, 1>), dim3(grid), dim3(NT), 0, s, ptr, xbeg, xval, starts, crow, cval,
+                       ctl, acc, marks, bkey, bval, g);
 }
 
 // -------------------------------------------------------------- compact

@@ -3,9 +3,10 @@
 // csr5_spmspv.h:449-610 with CSR5Preprocess, spmspv.h:1425-1582, built for
 // 32-lane warps). The format is re-derived for wave64:
 //   * a tile is 64 lanes x S items (S = sigma in {8, 16, 24, 32}); lane l
-//     owns the tile's items l*S .. l*S+S-1 in CSR order, stored transposed
-//     (item i of lane l at tile*64*S + i*64 + l) so that every item step is
-//     one coalesced load per wave;
+//     owns the tile's items l*S .. l*S+S-1 in CSR order, stored in groups
+//     of four (item i of lane l at tile*64*S + (i/4)*256 + l*4 + i%4) so
+//     that every four item steps are one 16-byte load per lane, 1 KB per
+//     wave side by side;
 //   * the descriptor is one 32-bit word per lane: bit i marks a row start
 //     at item i (bit 0 of lane 0 always marks the tile start). Marker
 //     offsets (the reference's y_offset) come from a DPP prefix sum of the
@@ -146,9 +147,13 @@ __global__ void k_csr5_transpose(const int* __restrict__ col, const float* __res
                                  float* __restrict__ cval) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (long long)gridDim.x * blockDim.x) {
+    // position o of a tile holds item i of lane l with o = (i / 4) * 256 +
+    // l * 4 + i % 4: each lane's items come in 16-byte groups of four, one
+    // vector load per group, a wave's load 1 KB side by side
     const long long t = q / T;
     const int o = (int)(q - t * T);
-    const int i = o / kWave, l = o % kWave;
+    const int g = o >> 8, r = o & 255;
+    const int l = r >> 2, i = g * 4 + (r & 3);
     const long long src = t * T + (long long)l * S + i;
     const bool ok = src < nnz;
     // with an item order, position src of the copy holds CSR item item[src]
@@ -222,13 +227,25 @@ __global__ __launch_bounds__(csr5_nt(S, HOT)) void k_csr5(const int* __restrict_
   unsigned hG = 0u, bG = 0u;
   unsigned fG;  // the tile's marker word
 
+  static_assert(S % 4 == 0, "items come in groups of four per lane");
+  typedef int i4v __attribute__((ext_vector_type(4)));
+  typedef float f4v __attribute__((ext_vector_type(4)));
   auto load_cv = [&](int tt) {
-    const int* __restrict__ ct = ccol + (size_t)tt * T + lanes;
-    const float* __restrict__ vt = cval + (size_t)tt * T + lanes;
+    // group g of this lane: items 4g .. 4g+3, one 16-byte load each
+    const i4v* __restrict__ ct = reinterpret_cast<const i4v*>(ccol + (size_t)tt * T) + lanes;
+    const f4v* __restrict__ vt = reinterpret_cast<const f4v*>(cval + (size_t)tt * T) + lanes;
 #pragma unroll
-    for (int i = 0; i < S; ++i) {
-      cN[i] = __builtin_nontemporal_load(ct + i * kWave);
-      vN[i] = __builtin_nontemporal_load(vt + i * kWave);
+    for (int g4 = 0; g4 < S / 4; ++g4) {
+      const i4v c = __builtin_nontemporal_load(ct + g4 * kWave);
+      const f4v v = __builtin_nontemporal_load(vt + g4 * kWave);
+      cN[4 * g4] = c.x;
+      cN[4 * g4 + 1] = c.y;
+      cN[4 * g4 + 2] = c.z;
+      cN[4 * g4 + 3] = c.w;
+      vN[4 * g4] = v.x;
+      vN[4 * g4 + 1] = v.y;
+      vN[4 * g4 + 2] = v.z;
+      vN[4 * g4 + 3] = v.w;
     }
   };
   auto gather = [&](int tt) {

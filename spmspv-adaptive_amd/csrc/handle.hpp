@@ -13,6 +13,7 @@
 
 struct BfsState;
 struct PrState;
+struct MsState;
 
 struct spmspv_handle_s {
   int m = 0, n = 0, nnz = 0;
@@ -59,6 +60,7 @@ struct spmspv_handle_s {
   int* ptr = nullptr;        // n+1 (x scan); doubles as the degree histogram
   int* xbeg = nullptr;       // n
   int* split = nullptr;      // LB tiles + 2
+  int* tl_starts = nullptr;  // LB tiles + 2: first x entry of each two-layer tile
   int* tile_mm = nullptr;    // 2 per prep tile
   unsigned long long* scan_state = nullptr;
   size_t scan_words = 0, sort_words = 0, red_words = 0;  // look-back state sizes
@@ -66,6 +68,7 @@ struct spmspv_handle_s {
   unsigned* bitmap = nullptr;  // ceil(m/32), all-zero between calls
   unsigned char* marks = nullptr;  // one byte per row (padded to 32), all-zero between calls
   spmspv::SortBufs sb{};
+  spmspv::BinWs bin_ws{};  // bucket slot's per-bin counts, starts, cursors and chunks
   int* yk_tmp = nullptr;     // m
   float* yv_tmp = nullptr;   // m
   float* x_dense_int = nullptr;  // n
@@ -102,6 +105,9 @@ struct spmspv_handle_s {
   int team_row = 8;
   int row_vt = 7;  // row-merge items per thread (autotuned)
   int bucket_bits = 8;  // bucket slot: 2^bucket_bits row buckets (autotuned, or set by csc_bucket)
+  // bin_len range in which auto mode runs the bucket slot in place of a
+  // load-balanced column or row slot (autotuned; off while hi < lo)
+  long long bucket_lo = 0, bucket_hi = -1;
 
   // host mirrors
   int* h_pin = nullptr;       // small pinned readback area (64 ints)
@@ -126,6 +132,7 @@ struct spmspv_handle_s {
 
   BfsState* bfs = nullptr;
   PrState* pr = nullptr;  // PageRank driver buffers
+  MsState* ms = nullptr;  // multi-source BFS buffers
   void* csr5_mem = nullptr;          // CSR5 copy of the CSR (spmv_type 1)
   size_t csr5_bytes = 0;
   bool csr5_ready = false;
@@ -137,6 +144,12 @@ struct spmspv_handle_s {
   spmspv::Gate gate{};
   bool speculate = true;
   signed char spec_last[33] = {};
+  int prev_choice = -1;        // slot of the last selector-dispatched call
+  float switch_penalty = 0.f;  // switching-aware choice (0: off)
+  bool spec_fused[33] = {};  // the last call of the size class ended in the single-launch path
+  bool small_x_path = true;  // column slots on a small x try the single-launch path first
+  int fuse_cap = spmspv::kFuseCap;             // largest bin_len of that path (autotuned)
+  int small_sort_cap = spmspv::kSmallSortCap;  // largest single-workgroup sort (autotuned)
   long long spec_hits = 0, spec_misses = 0;
   int batch_cap = 0;
   // batch stash reserved by alloc_pre_buffer; a larger batch grows it
@@ -172,6 +185,9 @@ int bfs_alloc(spmspv_handle_t h);
 // Allocates / frees the PageRank driver's buffers (square matrices).
 int pr_alloc(spmspv_handle_t h);
 void pr_release(spmspv_handle_t h);
+// Allocates / frees the multi-source BFS buffers (square matrices).
+int ms_alloc(spmspv_handle_t h);
+void ms_release(spmspv_handle_t h);
 // Drops the captured BFS level graphs (they hold matrix and arena pointers).
 void bfs_drop_graphs(spmspv_handle_t h);
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel);

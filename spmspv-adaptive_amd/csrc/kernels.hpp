@@ -58,35 +58,80 @@ struct SelectorArgs {
   int mode;              // 0 mlp, 1 tree, 2 fixed
   int fixed;
   long long* stash;      // when set: the selector word of this x (sel_word) is stored here
+  // features-only launch for a large x: the selector word may carry the
+  // sampled estimate of bin_len (flagged kSelEstimated) instead of the sum
+  int estimate;
+  // switching-aware choice: with penalty > 0 and prev >= 0, a slot of the
+  // previous call's family (column 0-3, row SpMSpV 4-5, SpMV 6-7) whose
+  // logit is within penalty of the top one is kept instead of a switch
+  int prev;
+  float penalty;
+  // tuned size rule behind the network: a load-balanced column or row slot
+  // (2, 3, 5, 7) becomes the bucket slot (8) when bin_lo <= bin_len <= bin_hi
+  // (off by default: bin_hi < bin_lo)
+  long long bin_lo = 0, bin_hi = -1;
 };
+__host__ __device__ inline int slot_family(int k) { return k < 4 ? 0 : (k < 6 ? 1 : 2); }
+// The selector's x inputs for a large x (xnnz > kSampleMin) come from
+// kSampleKeys evenly spaced keys (positions floor(i * xnnz / kSampleKeys)):
+// bin_len is their column lengths' sum scaled by xnnz / kSampleKeys, and
+// max_len their longest column. Every launch that runs the selector uses
+// this definition, so training and dispatch see the same features.
+constexpr int kSampleMin = 1 << 16, kSampleKeys = 4096;
 // {bin_len, kernel choice} in one 64-bit word, so that a single store
 // publishes both (no ordering fence); never negative, choice 0xff = none.
+// Flag kSelEstimated in the choice byte: bin_len is the sampled estimate.
+constexpr int kSelEstimated = 0x40;
+// Flag kSelDone: the launch that selected also computed the result (the
+// single-launch small-x column path); the bin_len field holds y_nnz. Set
+// in ctl->kernel_choice too, where it shuts every gated launch.
+constexpr int kSelDone = 0x80;
 __host__ __device__ inline long long sel_word(long long bin_len, int choice) {
   return (bin_len << 8) | (long long)(choice & 0xff);
 }
 __host__ __device__ inline long long sel_word_binlen(long long w) { return w >> 8; }
 __host__ __device__ inline int sel_word_choice(long long w) {
   const int c = (int)(w & 0xff);
-  return c == 0xff ? -1 : c;
+  return c == 0xff ? -1 : (c & 0x3f);
 }
+__host__ __device__ inline bool sel_word_done(long long w) {
+  const int c = (int)(w & 0xff);
+  return c != 0xff && (c & kSelDone) != 0;
+}
+__host__ __device__ inline bool sel_word_estimated(long long w) {
+  const int c = (int)(w & 0xff);
+  return c != 0xff && (c & kSelEstimated) != 0;
+}
+// samp (2 * kSampleKeys / kPrepNT long longs, or null): the first tiles'
+// partials of a large x's sampled selector inputs, so that the sampling
+// overlaps the scan instead of following the last tile's look-back.
 void launch_prep(hipStream_t s, const int* colptr, const int* xkey, int ncols, int xnnz,
                  const int* xnnz_dev, int* ptr, int* xbeg, int* split, unsigned long long* state,
-                 int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep);
+                 int* tile_mm, Ctl* ctl, const SelectorArgs* sel, Ep ep, long long* samp = nullptr);
 // Features-only selector: ctl->{bin_len, max_len, min_len, xnnz} of x and,
 // with sel.mat set, the selector's choice (and the selector word in
-// sel.stash). part: 3 * kSelMaxGrid partials. No column scan. With enc set
+// sel.stash). part: 5 * kSelMaxGrid partials. No column scan. With enc set
 // (already all +0.0), x's support encoding is scattered into it as well.
 void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const float* xval,
                         int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
                         float* enc, const int* enc_perm = nullptr);
+// mode 0: FP32 atomics into acc + marks; mode 1: (row, a*x) at the product
+// index; mode 2: the row keys alone (boolean, cval and xval unread).
 void launch_naive_col(hipStream_t s, int mode, const int* colptr, const int* crow, const float* cval,
                       const int* xkey, const float* xval, const int* ptr, int ncols, int xnnz,
                       const int* xnnz_dev, int team, float* acc, unsigned char* marks, int* bkey,
                       float* bval, Gate g = {});
+// modes as launch_naive_col.
 void launch_lb_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval,
                       const int* split, const int* crow, const float* cval, const Ctl* ctl,
                       int xnnz, const int* xnnz_dev, float* acc, unsigned char* marks, int* bkey,
                       float* bval, long long tiles_hint, Gate g = {});
+// Two-layer load balance: tiles of exactly kLbTile products; starts holds
+// ceil(bin_len / kLbTile) + 1 ints. mode 0: atomics into acc + marks,
+// mode 1: bins at the product index.
+void launch_tl_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, const float* xval, int xnnz,
+                      int* starts, const int* crow, const float* cval, const Ctl* ctl, float* acc,
+                      unsigned char* marks, int* bkey, float* bval, long long tiles_hint, Gate g = {});
 // hits (bit array, or mark bytes when marks != null) -> sorted (key, val);
 // clears the hits (and acc entries when reset_acc) so the accumulator
 // returns to all-zero. With keymap, hit k is reported as key keymap[k]
@@ -115,6 +160,23 @@ void launch_set_bits(hipStream_t s, const int* key, int nnz, const int* nnz_dev,
                      unsigned* bits);
 void launch_clear_keys(hipStream_t s, const int* key, int nnz, float* dense, unsigned* bits);
 
+// Single-launch small-x column path: x of at most kFuseKeys keys whose
+// products (bin_len) fit kFuseCap run scan, gather, LDS sort, reduce and
+// the sorted output in one workgroup (k_col_fused, sort.hip). It also
+// writes the column scan (ptr, xbeg, split, ctl) like launch_prep, so a
+// call that does not fit goes on with the usual launches (gated on
+// ctl->kernel_choice, which gets kSelDone when the launch finished the
+// call). In auto mode (sel.mode != 2) a call that fits is reported as
+// kFusedSlot, the sort-based slot whose result it computes. Returns false
+// (nothing launched) when xnnz > kFuseKeys.
+constexpr int kFuseNT = 1024, kFuseKeys = 4096, kFuseCap = kFuseNT * 11;
+constexpr int kFusedSlot = 1;  // SPMSPV_K_COL_NAIVE_SORT
+// cap (<= kFuseCap): largest bin_len that takes the single launch (the
+// autotuner may lower it).
+bool launch_col_fused(hipStream_t s, const int* colptr, const int* crow, const float* cval, const int* xkey,
+                      const float* xval, int ncols, int xnnz, int m, int* ptr, int* xbeg, int* split, Ctl* ctl,
+                      const SelectorArgs& sel, int* ykey, float* yval, long long* count_sig, int cap);
+
 // --- sort path -----------------------------------------------------------
 // Sorts (bkey, bval)[bin_len] by key (< m) with an LSD radix sort
 // (onesweep), then reduces equal keys into (ykey, yval) and ctl->ynnz.
@@ -132,19 +194,36 @@ struct SortBufs {
 };
 // n_hint: item count when the host knows it (exact grids), else -1 and the
 // grids cover n_cap. ep: first of the 5 epochs the sort + reduce use.
+// small_cap (<= kSmallSortCap): largest item count that takes the
+// single-workgroup path (the autotuner may lower it).
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
-                        long long binlen_hint, long long n_cap, Ep ep, Gate g = {});
+                        long long binlen_hint, long long n_cap, Ep ep, Gate g = {},
+                        int small_cap = kSmallSortCap);
 // Sorts (keyA, valA)[*n_dev] by key < range; returns 0 if the result is in
-// A, 1 if in B.
+// A, 1 if in B. With valA and valB null it sorts the keys alone.
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g = {});
+                      long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g = {},
+                      int small_cap = kSmallSortCap);
 
-// Bucket slot: partition the gathered bins by the top bucket_bits (1..8)
-// row bits, then fixed chunks of the bucketed products sum per row in LDS
-// windows and add each row's sum once into acc (marks set); the caller's
-// compaction gives the sorted result and clears acc and marks.
-void launch_bucket(hipStream_t s, const SortBufs& b, int m, int bucket_bits, Ctl* ctl, float* acc,
-                   unsigned char* marks, long long binlen_hint, long long n_cap, Ep ep);
+// Bucket slot: row bins of at most 2^kBinRowBits rows (at least
+// 2^bucket_bits bins, at most kBinMax); products counted per bin, scattered
+// into their bins (bkey/bval, bin_len capacity) and summed per bin in LDS
+// windows into acc (marks set); the caller's compaction gives the sorted
+// result and clears acc and marks. Needs k_prep's scan and splits. The
+// workspace's counts are zero between calls (the plan kernel clears them).
+// Returns false (nothing launched) when m needs more than kBinMax bins.
+constexpr int kBinRowBits = 15, kBinMax = 2048, kBinChunk = 16384;
+struct BinWs {
+  unsigned* hist;              // kBinMax counts
+  unsigned long long* cursor;  // kBinMax write positions
+  long long* bstart;           // kBinMax + 1 bin starts
+  int* cstart;                 // kBinMax + 1 first accumulate chunk per bin
+};
+int bin_geometry(int m, int bucket_bits, int* rbits);
+bool launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* xval, const int* split,
+                   const int* crow, const float* cval, const Ctl* ctl, int xnnz, int m, int bucket_bits,
+                   const BinWs& ws, unsigned* bkey, float* bval, float* acc, unsigned char* marks,
+                   long long tiles_hint, long long binlen_hint, long long binlen_cap);
 
 // --- row path ------------------------------------------------------------
 // --- CSR5 for wave64 (spmv_type 1) ----------------------------------------
@@ -216,8 +295,9 @@ int resident_blocks(const void* kernel, int block);
 void launch_row_split(hipStream_t s, const int* row_ptr, int m, int nnz, int* split, int ntiles,
                       int tile);
 // vt in {5, 7, 11}: merged items per thread (tile = 256 * vt).
-// x == nullptr: x is given by its bit array xbits alone (x = 1 on it; the
-// kernel runs masked with hits). Otherwise, with masked or hits, x must be
+// x == nullptr: the boolean form, x given by its bit array xbits alone:
+// hit bits into ybits, no values read (val may be null), y not written (may
+// be null). Otherwise, with masked or hits, x must be
 // the support-encoded vector (launch_encode_x / launch_scatter with
 // encode); without either it is plain dense x.
 void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
@@ -225,6 +305,8 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                       const int* split, int ntiles, int vt, float* y, unsigned* ybits,
                       float* carry_val, int* carry_row, int* carry_hit,
                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, Gate g = {});
+// val == nullptr and x == nullptr: the boolean form (hit bits into ybits
+// only).
 void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int team,
                       float* y, unsigned* ybits);

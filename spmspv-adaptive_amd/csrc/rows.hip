@@ -126,8 +126,9 @@ __device__ __forceinline__ RowTile row_tile(const int* __restrict__ split, int t
 //   kXEnc  : the support-encoded x (+0.0 outside the support, a nonzero bit
 //            pattern inside it, an explicit zero stored as -0.0), so one
 //            gather yields both the value and the support bit,
-//   kXBits : only x's bit array, x = 1 on the support (boolean semiring of
-//            the BFS levels).
+//   kXBits : only x's bit array (boolean semiring of the BFS levels): no
+//            values are read (val may be null), a row's result is its hit
+//            bit alone and y is not written (may be null).
 // Loads use a per-tile base pointer with 32-bit offsets and are
 // unconditional (clamped) so that a stage's loads issue back to back.
 // Rows that cross tiles leave their tail in carry_*[t] (row i1), which
@@ -174,13 +175,12 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
     const int jb = g.j0 < nnz ? g.j0 : nnz - 1;
     const unsigned lim = g.nj > 0 ? (unsigned)(g.nj - 1) : 0u;
     const int* __restrict__ ct = col + jb;
-    const float* __restrict__ vt = val + jb;
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
       unsigned k = tid + i * NT;
       k = k < lim ? k : lim;
       cN[i] = __builtin_nontemporal_load(ct + k);
-      vN[i] = __builtin_nontemporal_load(vt + k);
+      if (XIN != kXBits) vN[i] = __builtin_nontemporal_load(val + jb + k);
     }
   };
   auto gather = [&](int tt) {
@@ -204,8 +204,8 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
     // copy of a register whose load is in flight (a copy waits for it)
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
-      asm volatile("v_mov_b32 %0, %1" : "=v"(vG[i]) : "v"(vN[i]));
-      if (XIN == kXBits) asm volatile("v_mov_b32 %0, %1" : "=v"(cG[i]) : "v"(cN[i]));
+      if (XIN != kXBits) asm volatile("v_mov_b32 %0, %1" : "=v"(vG[i]) : "v"(vN[i]));
+      else asm volatile("v_mov_b32 %0, %1" : "=v"(cG[i]) : "v"(cN[i]));
     }
   };
 
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
       float p;
       if (XIN == kXBits) {
         in = (__builtin_bit_cast(unsigned, xG[i]) >> (cG[i] & 31)) & 1u;
-        p = in ? vG[i] : 0.f;
+        p = 0.f;
       } else {
         in = __builtin_bit_cast(unsigned, xG[i]) != 0u;
         p = vG[i] * xG[i];
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
           if (re[i] > rs[i] && re[i] > g.j0) {
             s_end[re[i] - 1 - g.j0] = g.i0 + (int)k;
           } else {
-            yt[k] = 0.f;
+            if (XIN != kXBits) yt[k] = 0.f;
             if (HITS) s_yh[k] = 0;
           }
         }
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
             first_hit = hit;
             had_end = true;
           } else {
-            y[(unsigned)r] = sum;
+            if (XIN != kXBits) y[(unsigned)r] = sum;
             if (HITS) s_yh[r - g.i0] = (unsigned char)hit;
           }
           sum = 0.f;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(NT) void k_row_merge(const int* __restrict__ row_pt
     int hin, tail_h;
     seg_scan_row<NT>(had_end, sum, hit, &cin, &hin, &tail_v, &tail_h, s_f, s_sv, s_sh);
     if (had_end) {
-      y[(unsigned)first_row] = cin + first_sum;
+      if (XIN != kXBits) y[(unsigned)first_row] = cin + first_sum;
       if (HITS) s_yh[first_row - g.i0] = (unsigned char)(hin | first_hit);
     }
     if (tid == 0) {
@@ -345,7 +345,7 @@ __global__ void k_row_fixup(const float* __restrict__ carry_val, const int* __re
     float s;
     if (!carry_run<int>(carry_val, carry_row, HITS ? carry_hit : nullptr, t, ntiles, r, s, h))
       continue;
-    y[r] += s;
+    if (y) y[r] += s;  // null: hit bits only (boolean rows)
     if (HITS && h) atomicOr(ybits + (r >> 5), 1u << (r & 31));
   }
 }
@@ -375,7 +375,7 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                       hipEvent_t ev_end, Gate g) {
   if (m <= 0 || ntiles <= 0) return;
   if (nnz == 0) {  // every row is empty; no hits
-    launch_fill_f32(s, y, m, 0.f, g);
+    if (y) launch_fill_f32(s, y, m, 0.f, g);
     return;
   }
   // one resident wave of workgroups, each walking many tiles
@@ -413,7 +413,9 @@ void launch_row_merge(hipStream_t s, bool masked, bool hits, const int* row_ptr,
                        carry_hit, ntiles, y, ybits, g);
 }
 
-template <int NT, bool MASKED, bool HITS>
+// BOOL (val and x null): the boolean semiring of the BFS levels; a row's
+// result is its hit bit alone (no values read, y not written).
+template <int NT, bool MASKED, bool HITS, bool BOOL>
 __global__ __launch_bounds__(NT) void k_row_naive(const int* __restrict__ row_ptr,
                                                   const int* __restrict__ col,
                                                   const float* __restrict__ val,
@@ -438,11 +440,13 @@ __global__ __launch_bounds__(NT) void k_row_naive(const int* __restrict__ row_pt
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         c[u] = __builtin_nontemporal_load(col + j + u * team);
-        v[u] = __builtin_nontemporal_load(val + j + u * team);
+        if (!BOOL) v[u] = __builtin_nontemporal_load(val + j + u * team);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (MASKED) {
+        if (BOOL) {
+          h |= test_bit(xbits, c[u]) ? 1 : 0;
+        } else if (MASKED) {
           if (test_bit(xbits, c[u])) {
             s += v[u] * x[c[u]];
             h = 1;
@@ -455,7 +459,9 @@ __global__ __launch_bounds__(NT) void k_row_naive(const int* __restrict__ row_pt
     }
     for (; j < e; j += team) {
       const int c = col[j];
-      if (MASKED) {
+      if (BOOL) {
+        h |= test_bit(xbits, c) ? 1 : 0;
+      } else if (MASKED) {
         if (test_bit(xbits, c)) {
           s += val[j] * x[c];
           h = 1;
@@ -466,16 +472,18 @@ __global__ __launch_bounds__(NT) void k_row_naive(const int* __restrict__ row_pt
       }
     }
     for (int o = team >> 1; o > 0; o >>= 1) {
-      s += __shfl_xor(s, o, kWave);
-      if (HITS) h |= __shfl_xor(h, o, kWave);
+      if (!BOOL) s += __shfl_xor(s, o, kWave);
+      if (HITS || BOOL) h |= __shfl_xor(h, o, kWave);
     }
     if (tl == 0) {
-      y[r] = s;
-      if (HITS && h) atomicOr(ybits + (r >> 5), 1u << (r & 31));
+      if (!BOOL) y[r] = s;
+      if ((HITS || BOOL) && h) atomicOr(ybits + (r >> 5), 1u << (r & 31));
     }
   }
 }
 
+// val == nullptr and x == nullptr: the boolean form (hit bits of the rows
+// that touch xbits' support, into ybits; y unused).
 void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr, const int* col,
                       const float* val, const float* x, const unsigned* xbits, int m, int team,
                       float* y, unsigned* ybits) {
@@ -484,13 +492,14 @@ void launch_row_naive(hipStream_t s, bool masked, bool hits, const int* row_ptr,
   while ((1 << lg) < team && lg < 6) ++lg;
   const int tpb = 256 >> lg;
   const int grid = (int)std::min<long long>(16384, ((long long)m + tpb - 1) / tpb);
-#define SPMSPV_ROW_NAIVE(MK, HT)                                                                   \
-  hipLaunchKernelGGL((k_row_naive<256, MK, HT>), dim3(grid), dim3(256), 0, s, row_ptr, col, val, x, \
+#define SPMSPV_ROW_NAIVE(MK, HT, BO)                                                                   \
+  hipLaunchKernelGGL((k_row_naive<256, MK, HT, BO>), dim3(grid), dim3(256), 0, s, row_ptr, col, val, x, \
                      xbits, m, lg, y, ybits)
-  if (masked && hits) SPMSPV_ROW_NAIVE(true, true);
-  else if (masked) SPMSPV_ROW_NAIVE(true, false);
-  else if (hits) SPMSPV_ROW_NAIVE(false, true);
-  else SPMSPV_ROW_NAIVE(false, false);
+  if (!val && !x) SPMSPV_ROW_NAIVE(true, true, true);
+  else if (masked && hits) SPMSPV_ROW_NAIVE(true, true, false);
+  else if (masked) SPMSPV_ROW_NAIVE(true, false, false);
+  else if (hits) SPMSPV_ROW_NAIVE(false, true, false);
+  else SPMSPV_ROW_NAIVE(false, false, false);
 #undef SPMSPV_ROW_NAIVE
 }
 

@@ -108,6 +108,15 @@ __device__ __forceinline__ float lane_bcast(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
+// The tuned size rule behind the selector: a load-balanced column or row
+// slot (2, 3, 5, 7) becomes the bucket slot inside [bin_lo, bin_hi].
+__device__ inline int bucket_rule(int best, double bin_len, const SelectorArgs& sel) {
+  if ((best == 2 || best == 3 || best == 5 || best == 7) && bin_len >= (double)sel.bin_lo &&
+      bin_len <= (double)sel.bin_hi)
+    return 8;
+  return best;
+}
+
 // Runs in wave 0 of the calling workgroup (the other waves return at
 // once; no barrier inside). sel.weights, when set, points at the weights in
 // LDS in the wlds_index layout. Writes ctl->feat and ctl->kernel_choice
@@ -126,7 +135,8 @@ __device__ void select_in_wave(Ctl* ctl, const SelectorArgs& sel, double xnnz, d
   }
   if (sel.mode == 1 || !sel.weights) {
     if (l == 0)
-      ctl->kernel_choice = tree_select(mf->n, mf->nnz, mf->avg_row, mf->std_row, xnnz, bin_len, max_len);
+      ctl->kernel_choice =
+          bucket_rule(tree_select(mf->n, mf->nnz, mf->avg_row, mf->std_row, xnnz, bin_len, max_len), bin_len, sel);
     return;
   }
   const float* w = sel.weights;
@@ -182,7 +192,23 @@ __device__ void select_in_wave(Ctl* ctl, const SelectorArgs& sel, double xnnz, d
       best = k;
     }
   }
-  if (l == 0) ctl->kernel_choice = best;
+  // switching-aware: stay in the previous call's family when its best slot
+  // is within the penalty (the soft targets make a logit gap a relative
+  // time gap over tau, so the penalty is the switching cost in those units)
+  if (sel.penalty > 0.f && sel.prev >= 0 && slot_family(best) != slot_family(sel.prev)) {
+    int keep = -1;
+    float kz = 0.f;
+#pragma unroll
+    for (int k = 0; k < kC; ++k) {
+      const float zk = lane_bcast(z, k);
+      if (slot_family(k) == slot_family(sel.prev) && zk >= bz - sel.penalty && (keep < 0 || zk > kz)) {
+        keep = k;
+        kz = zk;
+      }
+    }
+    if (keep >= 0) best = keep;
+  }
+  if (l == 0) ctl->kernel_choice = bucket_rule(best, bin_len, sel);
 }
 
 }  // namespace spmspv

@@ -15,6 +15,7 @@
 // The reduce is deterministic: fixed-order segmented scans inside a tile
 // and a fixed-order carry fix-up across tiles.
 #include "kernels.hpp"
+#include "selector.hpp"
 
 namespace spmspv {
 
@@ -307,10 +308,15 @@ __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned* __restrict_
     int rank[IPT];
     unsigned dig[IPT];
     load_tile_keys<NT, IPT>(kin, n, base, key, valid);
-  #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const long long gi = base + (long long)w * kWave * IPT + j * kWave + lane();
-      val[j] = vin[valid[j] ? gi : n - 1];
+      if (vin) {  // null: keys only
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const long long gi = base + (long long)w * kWave * IPT + j * kWave + lane();
+        val[j] = vin[valid[j] ? gi : n - 1];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) val[j] = 0.f;
     }
     wave_rank<IPT>(key, valid, shift, s_cnt + w * 256, rank, dig);
     __syncthreads();
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(NT) void k_sort_scatter(const unsigned* __restrict_
       const unsigned dd = (k >> shift) & 255u;
       const unsigned g = s_gbase[dd] + (i - s_dstart[dd]);
       kout[g] = k;
-      vout[g] = s_v[i];
+      if (vout) vout[g] = s_v[i];
     }
     __syncthreads();
   }
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
                                                    const float* __restrict__ vin,
                                                    const long long* __restrict__ n_dev, int passes,
                                                    int mode, Ctl* ctl, int* ykey, float* yval,
-                                                   unsigned* kout, float* vout, Gate g) {
+                                                   unsigned* kout, float* vout, int cap, Gate g) {
   if (gate_shut(g)) return;
   constexpr int TILE = NT * IPT;
   constexpr int NW = NT / kWave;
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
   __shared__ int s_f[NW];
   __shared__ float s_fv[NW];
   const long long n = *n_dev;
-  if (n > TILE) return;
+  if (n > TILE || n > cap) return;
   const int w = threadIdx.x / kWave;
   unsigned key[IPT];
   float val[IPT];
@@ -373,7 +379,7 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
     const int gi = w * kWave * IPT + j * kWave + lane();
     valid[j] = gi < n;
     key[j] = valid[j] ? kin[gi] : 0u;
-    val[j] = valid[j] ? vin[gi] : 0.f;
+    val[j] = valid[j] && vin ? vin[gi] : 0.f;  // vin null: keys only
   }
   for (int p = 0; p < passes; ++p) {
     wave_rank<IPT>(key, valid, 8 * p, s_cnt + w * 256, rank, dig);
@@ -408,7 +414,7 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
   if (mode == 1) {
     for (int i = threadIdx.x; i < cnt; i += NT) {
       kout[i] = s_k[i];
-      vout[i] = s_v[i];
+      if (vout) vout[i] = s_v[i];
     }
     return;
   }
@@ -531,15 +537,15 @@ static void launch_digit_pass(hipStream_t s, const SortBufs& b, const unsigned* 
 // in A, 1 when in B. ep.dev >= 0 (captured graphs) uses device epochs
 // ep.dev + pass; otherwise ep.v + pass.
 int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long long* n_dev, Ctl* ctl,
-                      long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g) {
+                      long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g, int small_cap) {
   const int passes = key_passes(range);
   const int final_b = passes & 1;
-  const bool small_known = n_hint >= 0 && n_hint <= kSmallSortCap;
-  const bool big_known = n_hint > kSmallSortCap;
+  const bool small_known = n_hint >= 0 && n_hint <= small_cap;
+  const bool big_known = n_hint > small_cap;
   if (!big_known && !skip_small) {
     hipLaunchKernelGGL((k_sort_small<kSortNT, kSortIPT>), dim3(1), dim3(kSortNT), 0, s, b.keyA,
                        b.valA, n_dev, passes, 1, ctl, nullptr, nullptr, final_b ? b.keyB : b.keyA,
-                       final_b ? b.valB : b.valA, g);
+                       final_b ? b.valB : b.valA, small_cap, g);
   }
   if (small_known) return final_b;
   for (int p = 0; p < passes; ++p) {
@@ -548,7 +554,7 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
     unsigned* kout = (p & 1) ? b.keyA : b.keyB;
     float* vout = (p & 1) ? b.valA : b.valB;
     const Ep e{ep.v + (unsigned)p, ep.dev < 0 ? -1 : ep.dev + p};
-    launch_digit_pass(s, b, kin, vin, kout, vout, 8 * p, n_dev, ctl, n_hint, n_cap, kSmallSortCap, e, g);
+    launch_digit_pass(s, b, kin, vin, kout, vout, 8 * p, n_dev, ctl, n_hint, n_cap, small_cap, e, g);
   }
   return final_b;
 }
@@ -562,26 +568,271 @@ void launch_partition_pass(hipStream_t s, const SortBufs& b, int shift, const lo
 }
 
 void launch_sort_reduce(hipStream_t s, const SortBufs& b, int m, Ctl* ctl, int* ykey, float* yval,
-                        long long binlen_hint, long long n_cap, Ep ep, Gate g) {
+                        long long binlen_hint, long long n_cap, Ep ep, Gate g, int small_cap) {
   const long long* n_dev = &ctl->bin_len;
   const int passes = key_passes(m);
-  const bool small_known = binlen_hint >= 0 && binlen_hint <= kSmallSortCap;
-  const bool big_known = binlen_hint > kSmallSortCap;
+  const bool small_known = binlen_hint >= 0 && binlen_hint <= small_cap;
+  const bool big_known = binlen_hint > small_cap;
   if (!big_known) {
     hipLaunchKernelGGL((k_sort_small<kSortNT, kSortIPT>), dim3(1), dim3(kSortNT), 0, s, b.keyA,
-                       b.valA, n_dev, passes, 0, ctl, ykey, yval, nullptr, nullptr, g);
+                       b.valA, n_dev, passes, 0, ctl, ykey, yval, nullptr, nullptr, small_cap, g);
   }
   if (small_known) return;
-  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, n_cap, true, ep, g);
+  const int fb = launch_radix_sort(s, b, m, n_dev, ctl, binlen_hint, n_cap, true, ep, g, small_cap);
   const unsigned* kf = fb ? b.keyB : b.keyA;
   const float* vf = fb ? b.valB : b.valA;
   const Ep er{ep.v + 4u, ep.dev < 0 ? -1 : ep.dev + 4};
   hipLaunchKernelGGL((k_reduce<kRedNT, kRedIPT>), dim3(tiles_grid(binlen_hint, n_cap, kRedTile)),
                      dim3(kRedNT), 0, s, kf, vf, n_dev, ctl, ykey, yval, b.carry_val, b.carry_idx,
-                     b.red_state, er, kSmallSortCap, g);
+                     b.red_state, er, small_cap, g);
   const int fgrid = std::min(2048, std::max(1, tiles_grid(binlen_hint, n_cap, kRedTile) / 256 + 1));
   hipLaunchKernelGGL(k_carry_fixup, dim3(fgrid), dim3(256), 0, s, b.carry_val, b.carry_idx, n_dev,
-                     kRedTile, kSmallSortCap, yval, g);
+                     kRedTile, small_cap, yval, g);
+}
+
+// ------------------------------------------------ single-launch small x
+// A column SpMSpV whose x has at most kFuseKeys keys and whose products fit
+// one workgroup's LDS runs in one launch of one workgroup: the column
+// scan of x (as k_prep's single tile, including the merge-path splits, so a
+// call that does not fit continues with the usual launches), then, when
+// bin_len <= kFuseCap and sparse outputs are given, the gather of every
+// product (row, a*x) in x order, a stable LDS radix sort by row, the
+// in-order reduce and the sorted output with its count. The result is the
+// sort-based slots' (the same products, the same order). In auto mode a
+// call that fits skips the network (a size rule in front of it: no slot
+// reads less than the products of x); one that does not fit runs the
+// selector as k_prep does. When it finishes the call it sets the done flag
+// in ctl->kernel_choice (which shuts every gated launch queued behind it),
+// writes the count to ctl->ynnz, then to *count_sig and the selector word
+// (flagged kSelDone, count in the bin_len field) once the outputs are
+// complete.
+// One workgroup of 1,024 threads (four waves per SIMD, so that each wave's
+// LDS and memory waits overlap three others'); KPT keys and IPT products per
+// thread.
+template <int NT, int KPT, int IPT>
+__global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr, const int* __restrict__ crow,
+                                                  const float* __restrict__ cval, const int* __restrict__ xkey,
+                                                  const float* __restrict__ xval, int ncols, int xnnz, int passes,
+                                                  int* __restrict__ ptr, int* __restrict__ xbeg,
+                                                  int* __restrict__ split, Ctl* ctl, SelectorArgs sel,
+                                                  int* __restrict__ ykey, float* __restrict__ yval,
+                                                  long long* count_sig, int cap) {
+  static_assert(NT >= 256 && NT % 256 == 0, "a thread per radix digit");
+  constexpr int NW = NT / kWave, CAP = NT * IPT, KEYS = NT * KPT;
+  constexpr int kRawBytes = CAP * 8 > kNWeights * 4 ? CAP * 8 : kNWeights * 4;
+  // the sort's keys and values, or the selector's weights (never both);
+  // before the sort, the keys' space holds the product -> x entry map
+  __shared__ __align__(16) unsigned char s_raw[kRawBytes];
+  __shared__ int s_ptr[KEYS + 1];
+  __shared__ int s_beg[KEYS];
+  __shared__ unsigned s_cnt[NW * 256];
+  __shared__ unsigned s_dstart[256];
+  __shared__ unsigned s_t[NW];
+  __shared__ long long s_l[NW];
+  __shared__ int s_i[NW];
+  __shared__ int s_f[NW];
+  __shared__ float s_fv[NW];
+  __shared__ MatFeat s_mf;
+  unsigned* s_k = reinterpret_cast<unsigned*>(s_raw);
+  float* s_v = reinterpret_cast<float*>(s_raw + CAP * 4);
+  float* s_w = reinterpret_cast<float*>(s_raw);
+  int* s_seg = reinterpret_cast<int*>(s_raw);
+  // ---- column scan of x
+  const int base = threadIdx.x * KPT;
+  int c[KPT], beg[KPT], len[KPT];
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) c[j] = xnnz > 0 ? xkey[base + j < xnnz ? base + j : xnnz - 1] : 0;
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    const int ok = (base + j < xnnz && (unsigned)c[j] < (unsigned)ncols) ? 1 : 0;
+    const int ci = c[j] & -ok;
+    beg[j] = colptr[ci];
+    len[j] = colptr[ci + ok];
+    c[j] = ok;
+  }
+  long long tsum = 0;
+  int mx = 0, mn = INT_MAX;
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    len[j] = c[j] ? len[j] - beg[j] : 0;
+    beg[j] = c[j] ? beg[j] : 0;
+    if (c[j]) {
+      mx = max(mx, len[j]);
+      mn = min(mn, len[j]);
+    }
+    tsum += len[j];
+  }
+  mx = block_max<NT, int>(mx, s_i);
+  mn = block_min<NT, int>(mn, s_i);
+  long long bl;
+  long long run = block_excl_sum<NT, long long>(tsum, s_l, &bl);
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    const int i = base + j;
+    if (i < xnnz) {
+      ptr[i] = (int)run;
+      xbeg[i] = beg[j];
+      s_ptr[i] = (int)run;
+      s_beg[i] = beg[j];
+      if (split) {
+        const long long Mi = i + run;
+        const long long Mn = Mi + len[j] + 1;
+        for (long long d = (Mi / kLbTile + 1) * kLbTile; d <= Mn; d += kLbTile) split[d / kLbTile] = i + 1;
+      }
+      run += len[j];
+    }
+  }
+  if (threadIdx.x == 0) {
+    ptr[xnnz] = (int)bl;
+    s_ptr[xnnz] = (int)bl;
+    ctl->bin_len = bl;
+    ctl->max_len = xnnz > 0 ? mx : 0;
+    ctl->min_len = mn;
+    ctl->xnnz = xnnz;
+    const long long nt = ((long long)xnnz + bl + kLbTile - 1) / kLbTile;
+    ctl->n_tiles = nt;
+    if (split) {
+      split[0] = 0;
+      split[nt] = xnnz;
+    }
+  }
+  const bool fuse = bl <= CAP && bl <= cap && ykey != nullptr && yval != nullptr;
+  if (!fuse) {
+    if (sel.mode == 2 || !sel.mat) {
+      // a fixed slot (or no selector): the gated launches behind run
+      if (threadIdx.x == 0) {
+        if (sel.mode == 2) ctl->kernel_choice = sel.fixed;
+        if (sel.stash) *sel.stash = sel_word(bl, sel.mode == 2 ? sel.fixed : -1);
+      }
+      return;
+    }
+    // the selector, as the last tile of k_prep runs it
+    {
+      constexpr int kWPT = (kNWeights + NT - 1) / NT;
+      constexpr int kMF = sizeof(MatFeat) / sizeof(double);
+      if (sel.weights) {
+#pragma unroll
+        for (int i = 0; i < kWPT; ++i) {
+          const int idx = threadIdx.x + i * NT;
+          if (idx < kNWeights) s_w[wlds_index(idx)] = sel.weights[idx];
+        }
+      }
+      if (threadIdx.x < kMF)
+        reinterpret_cast<double*>(&s_mf)[threadIdx.x] = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
+      __syncthreads();
+      SelectorArgs sl = sel;
+      if (sel.weights) sl.weights = s_w;
+      sl.mat = &s_mf;
+      select_in_wave(ctl, sl, (double)xnnz, (double)bl, (double)(xnnz > 0 ? mx : 0));
+    }
+    if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(bl, ctl->kernel_choice);
+    return;
+  }
+  __syncthreads();  // s_ptr / s_beg complete
+  // ---- product -> x entry map, merge-path style: thread t fills products
+  // t*IPT .. t*IPT+IPT-1 from one search and a forward walk (x entries with
+  // empty columns are stepped over)
+  {
+    const int p0 = threadIdx.x * IPT;
+    if (p0 < bl) {
+      int lo = 0, hi = xnnz;  // s_ptr[lo] <= p0 < s_ptr[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_ptr[mid] <= p0) lo = mid;
+        else hi = mid;
+      }
+      int e = lo, nxt = s_ptr[lo + 1];
+#pragma unroll
+      for (int k = 0; k < IPT; ++k) {
+        const int p = p0 + k;
+        if (p < bl) {
+          while (nxt <= p) nxt = s_ptr[++e + 1];
+          s_seg[p] = e;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- products into registers, in x order (wave-major items as wave_rank takes them)
+  const int w = threadIdx.x / kWave;
+  unsigned key[IPT];
+  float val[IPT];
+  bool valid[IPT];
+  int rank[IPT];
+  unsigned dig[IPT];
+  int pos[IPT];
+  float xv[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int p = w * kWave * IPT + j * kWave + lane();
+    valid[j] = p < bl;
+    const int e = valid[j] ? s_seg[p] : 0;
+    pos[j] = valid[j] ? s_beg[e] + (p - s_ptr[e]) : 0;
+    xv[j] = valid[j] ? xval[e] : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    key[j] = valid[j] ? (unsigned)crow[pos[j]] : 0u;
+    val[j] = valid[j] ? cval[pos[j]] * xv[j] : 0.f;
+  }
+  __syncthreads();  // every map entry read before the sort reuses the space
+  // ---- LDS radix sort (stable), as k_sort_small; the digit offsets by
+  // threads 0..255 (one per digit)
+  for (int pss = 0; pss < passes; ++pss) {
+    wave_rank<IPT>(key, valid, 8 * pss, s_cnt + w * 256, rank, dig);
+    __syncthreads();
+    unsigned tot = 0;
+    if (threadIdx.x < 256) tot = digit_wave_offsets<NW>(s_cnt);
+    unsigned dtot;
+    const unsigned dex = block_excl_sum<NT, unsigned>(tot, s_t, &dtot);
+    if (threadIdx.x < 256) s_dstart[threadIdx.x] = dex;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (valid[j]) {
+        const unsigned q = s_dstart[dig[j]] + s_cnt[w * 256 + dig[j]] + rank[j];
+        s_k[q] = key[j];
+        s_v[q] = val[j];
+      }
+    }
+    __syncthreads();
+    if (pss + 1 < passes) {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int gi = w * kWave * IPT + j * kWave + lane();
+        if (valid[j]) {
+          key[j] = s_k[gi];
+          val[j] = s_v[gi];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  auto no_prefix = [](int) -> long long { return 0; };
+  const long long tot = tile_reduce<NT, IPT>(s_k, s_v, (int)bl, 0u, true, no_prefix, ykey, yval, nullptr,
+                                             nullptr, 0, s_i, s_f, s_fv);
+  // ---- publish: outputs complete, then the count and the done flag
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int choice = sel.mode == 2 ? sel.fixed : kFusedSlot;
+    ctl->ynnz = (int)tot;
+    ctl->sort_small = 1;
+    ctl->kernel_choice = choice | kSelDone;
+    __threadfence_system();
+    if (count_sig) *count_sig = tot;
+    if (sel.stash) *sel.stash = sel_word(tot, choice | kSelDone);
+  }
+}
+
+bool launch_col_fused(hipStream_t s, const int* colptr, const int* crow, const float* cval, const int* xkey,
+                      const float* xval, int ncols, int xnnz, int m, int* ptr, int* xbeg, int* split, Ctl* ctl,
+                      const SelectorArgs& sel, int* ykey, float* yval, long long* count_sig, int cap) {
+  if (xnnz > kFuseKeys) return false;
+  hipLaunchKernelGGL((k_col_fused<kFuseNT, kFuseKeys / kFuseNT, kFuseCap / kFuseNT>), dim3(1), dim3(kFuseNT), 0, s,
+                     colptr, crow, cval, xkey, xval, ncols, xnnz, key_passes(m), ptr, xbeg, split, ctl, sel, ykey,
+                     yval, count_sig, cap);
+  return true;
 }
 
 }  // namespace spmspv

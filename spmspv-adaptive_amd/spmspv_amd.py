@@ -30,11 +30,14 @@ K_ROW_LB = 5
 K_SPMV_NAIVE = 6
 K_SPMV_LB = 7
 K_COL_BUCKET = 8
+K_COL_TWO_LAYER_ATOMIC = 9
+K_COL_TWO_LAYER_SORT = 10
 NUM_SLOTS = 8
 # vector types (set_vector_type)
 SPMSPV_VEC_SPARSE, SPMSPV_VEC_DENSE, SPMSPV_VEC_BITARRAY = 0, 1, 2
 KERNEL_NAMES = ["col-naive-atomic", "col-naive-sort", "col-lb-atomic", "col-lb-sort",
-                "row-naive", "row-lb", "spmv-naive", "spmv-lb", "col-bucket"]
+                "row-naive", "row-lb", "spmv-naive", "spmv-lb", "col-bucket", "col-two-layer-atomic",
+                "col-two-layer-sort"]
 MLP_F, MLP_H1, MLP_H2, MLP_C = 16, 64, 64, 8
 MLP_NPARAM = MLP_H1 * MLP_F + MLP_H1 + MLP_H2 * MLP_H1 + MLP_H2 + MLP_C * MLP_H2 + MLP_C
 MLP_NWEIGHTS = MLP_NPARAM + 2 * MLP_F
@@ -99,6 +102,9 @@ _SIGS = {
     "spmspv_csc_nosort_merge_keyval": [_vp, _i, _f, ctypes.POINTER(_i), _vp, _vp, _vp],
     "spmspv_csc_sort_merge": [_vp, _i, _f, ctypes.POINTER(_i), _vp, _vp, _vp],
     "spmspv_csc_bucket": [_vp, _f, ctypes.POINTER(_i), _vp, _vp, _i],
+    "spmspv_csc_my_nosort": [_vp, _i, _f, ctypes.POINTER(_i), _vp, _vp, _vp],
+    "spmspv_csc_my_nosort_keyval": [_vp, _i, _f, ctypes.POINTER(_i), _vp, _vp, _vp],
+    "spmspv_csc_sort_my": [_vp, _i, _f, ctypes.POINTER(_i), _vp, _vp, _vp],
     "spmspv_naivespmspv": [_vp, _f, _vp],
     "spmspv_holaspmspv": [_vp, _f, _vp],
     "spmspv_csr5spmspv": [_vp, _f, _vp],
@@ -110,11 +116,15 @@ _SIGS = {
     "spmspv_autotune2": [_vp, _i, _i, _vp],
     "spmspv_time_main_kernel": [_vp, _i],
     "spmspv_set_speculation": [_vp, _i],
+    "spmspv_set_small_x_path": [_vp, _i],
+    "spmspv_set_switch_penalty": [_vp, _f],
+    "spmspv_set_bucket_range": [_vp, _ll, _ll],
     "spmspv_bfs_trace": [_vp, _i, _vp, _i, _i, _vp, _i, _vp, _vp],
     "spmspv_bfs_selector_set_weights": [_vp, _vp, _i],
     "spmspv_speculation_stats": [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "spmspv_main_kernel_ms": [_vp, _vp],
     "spmspv_autotune": [_vp, _i, _vp],
+    "spmspv_autotune_report": [_vp, _vp],
     "spmspv_selector_set_weights": [_vp, _vp, _i],
     "spmspv_selector_mode": [_vp, _i, _i],
     "spmspv_select": [_vp, ctypes.POINTER(_i)],
@@ -123,6 +133,7 @@ _SIGS = {
     "spmspv_mlp_loss_grad": [_vp, _vp, _vp, _i, ctypes.POINTER(_f), _vp],
     "spmspv_mlp_forward": [_vp, _vp, _i, _vp],
     "spmspv_bfs": [_vp, _i, _vp, _i, _i, _vp],
+    "spmspv_bfs_multi": [_vp, _i, _vp, _vp, _vp],
     "spmspv_run_batched": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "spmspv_pagerank": [_vp, _f, _f, _i, _i, _vp, _vp, _vp],
     "spmspv_csc_spmspv_noalloc": [_i, _i, _ll, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
@@ -222,6 +233,14 @@ class DeviceArray:
         out = np.empty(count, self.dtype)
         if count:
             _check(lib().spmspv_memcpy_d2h(_np(out), self.ptr, count * self.dtype.itemsize))
+        return out
+
+    def to_host_range(self, offset, count):
+        """Elements [offset, offset + count) to the host."""
+        out = np.empty(int(count), self.dtype)
+        if count:
+            src = _vp(self.ptr.value + int(offset) * self.dtype.itemsize)
+            _check(lib().spmspv_memcpy_d2h(_np(out), src, int(count) * self.dtype.itemsize))
         return out
 
     def upload(self, a):
@@ -419,8 +438,20 @@ class SpmspvHandle:
     def autotune(self, reps=3, csr5=True):
         rep = np.zeros(5, np.int32)
         self._c(lib().spmspv_autotune2(self.h, reps, 1 if csr5 else 0, _np(rep)))
-        return {"team_col": int(rep[0]), "team_row": int(rep[1]), "row_vt": int(rep[2]),
-                "spmv_type": int(rep[3]), "sigma": int(rep[4])}
+        return self.tuned()
+
+    def tuned(self):
+        """Every tuned parameter (spmspv_autotune_report)."""
+        r = np.zeros(10, np.int32)
+        self._c(lib().spmspv_autotune_report(self.h, _np(r)))
+        return {"team_col": int(r[0]), "team_row": int(r[1]), "row_vt": int(r[2]), "spmv_type": int(r[3]),
+                "sigma": int(r[4]), "buckets": int(r[5]), "small_x_products": int(r[6]),
+                "one_workgroup_sort": int(r[7]), "bucket_rule": [int(r[8]), int(r[9])]}
+
+    def set_bucket_range(self, lo, hi):
+        """bin_len range in which auto mode runs the bucket slot in place of
+        slot 2, 3, 5 or 7 (hi < lo: off)."""
+        return self._c(lib().spmspv_set_bucket_range(self.h, int(lo), int(hi)))
 
     def select(self):
         k = _i(-1)
@@ -437,6 +468,14 @@ class SpmspvHandle:
     def bfs(self, src, levels, policy=-1, use_graphs=0):
         st = np.zeros(4, np.float64)
         self._c(lib().spmspv_bfs(self.h, src, levels.ptr, policy, use_graphs, _np(st)))
+        return st
+
+    def bfs_multi(self, srcs, levels):
+        """Up to 64 BFSs at once; levels (device, n * len(srcs) ints) gets
+        levels[v * nsrc + i]. Returns the stats."""
+        srcs = np.ascontiguousarray(srcs, np.int32)
+        st = np.zeros(4, np.float64)
+        self._c(lib().spmspv_bfs_multi(self.h, int(srcs.size), _np(srcs), levels.ptr, _np(st)))
         return st
 
     def bfs_trace(self, src, levels, policy=-1, use_graphs=0, max_levels=128):
@@ -476,6 +515,15 @@ class SpmspvHandle:
     def set_speculation(self, on=True):
         """Speculative queueing of run_auto (on by default)."""
         return self._c(lib().spmspv_set_speculation(self.h, 1 if on else 0))
+
+    def set_switch_penalty(self, penalty):
+        """Switching-aware selection: keep the previous call's kernel family
+        within `penalty` logits of the top choice (0: off)."""
+        return self._c(lib().spmspv_set_switch_penalty(self.h, penalty))
+
+    def set_small_x_path(self, on=True):
+        """Single-launch path of the column slots for a small x (on by default)."""
+        return self._c(lib().spmspv_set_small_x_path(self.h, 1 if on else 0))
 
     def speculation_stats(self):
         """(right guesses, wrong guesses) of run_auto's speculation so far."""

@@ -7,9 +7,11 @@ The reference labels each (matrix, x) pair with the fastest of its kernels
 (t_pred - t_best) / t_best (model/1-test_dtree.py:300-317). Here:
 
   corpus  builds synthetic matrices on the host, autotunes each one (as a
-          user would once per matrix), times all eight slots on the GPU
-          (cold cache, median of reps) and records the 16 device features
-          and the rule fallback's choice for each (matrix, x):
+          user would once per matrix), times all eight slots on the GPU as
+          the selector dispatches them (spmspv_run_auto pinned to the slot:
+          the selector launch and speculative queueing included; cold cache,
+          median of reps) and records the 16 device features and the rule
+          fallback's choice for each (matrix, x):
             uniform random (n = 2^14..2^21 log-uniform, mean nnz/col 2..64),
             R-MAT (a in [0.45, 0.65], d = 0.05, b = c),
             banded (bandwidth 3..201), block-diagonal (dense 32x32 blocks);
@@ -84,13 +86,27 @@ def build_matrix(spec, seed):
     return n, rp, col, val
 
 
-def time_slot(A, slot, yk, yv, reps, flush):
+def time_slot(A, slot, yk, yv, reps, flush, dispatched=True, reset=None):
+    """Median cold time of one slot. dispatched: the call as the selector
+    dispatches it (spmspv_run_auto with the selector pinned to the slot, so
+    the label carries the selector launch and the speculative queueing the
+    slot gets in use), else a plain spmspv_run of the slot. reset (when
+    given) sets x again before every call, so that each call converts x
+    afresh (the time c of the reference's accounting); without it the x
+    formats of the previous call are reused (its c_single)."""
     ts = []
+    if dispatched:
+        A.selector_mode(2, slot)
     for r in range(reps + 1):
+        if reset:
+            reset()
         if flush:
             A.flush_cache(flush)
         A.timer_start()
-        A.run(slot, y_key=yk, y_val=yv)
+        if dispatched:
+            A.run_auto(y_key=yk, y_val=yv)
+        else:
+            A.run(slot, y_key=yk, y_val=yv)
         ms = A.timer_stop()
         if r:
             ts.append(ms)
@@ -98,13 +114,14 @@ def time_slot(A, slot, yk, yv, reps, flush):
 
 
 def cmd_corpus(args):
-    feats, times, rule, meta = [], [], [], []
+    feats, times, single, rule, meta = [], [], [], [], []
     t0 = time.time()
     flush = args.flush_mb << 20
 
     def save():
         np.savez(args.out, feats=np.array(feats, np.float32).reshape(-1, sa.MLP_F),
                  times=np.array(times, np.float64).reshape(-1, sa.NUM_SLOTS),
+                 times_single=np.array(single, np.float64).reshape(-1, sa.NUM_SLOTS),
                  rule=np.array(rule, np.int32), meta=np.array(meta, np.int64).reshape(-1, 4))
 
     for seed in range(args.start, args.start + args.count):
@@ -138,7 +155,13 @@ def cmd_corpus(args):
             A.selector_mode(1)
             rule.append(A.select())
             feats.append(A.features())
-            times.append([time_slot(A, s, yk, yv, args.reps, flush) for s in range(sa.NUM_SLOTS)])
+            def reset():
+                A.set_sparse_x(dk, dv, k)
+            times.append([time_slot(A, s, yk, yv, args.reps, flush, args.dispatched, reset)
+                          for s in range(sa.NUM_SLOTS)])
+            single.append([time_slot(A, s, yk, yv, args.reps, flush, args.dispatched)
+                           for s in range(sa.NUM_SLOTS)] if args.single else times[-1])
+            A.selector_mode(1)
             meta.append([seed, n, int(rp[-1]), k])
         del M, A
         if (seed - args.start) % 25 == 0:
@@ -255,7 +278,7 @@ def soft_targets(times, tau):
     return (z / z.sum(axis=1, keepdims=True)).astype(np.float32)
 
 
-def predict(w, X):
+def logits(w, X):
     F = sa.MLP_F
     mean, istd = w[sa.MLP_NPARAM:sa.MLP_NPARAM + F], w[sa.MLP_NPARAM + F:]
     Xn = np.ascontiguousarray((X - mean) * istd, np.float32)
@@ -264,7 +287,57 @@ def predict(w, X):
                                      X.shape[0], z.ctypes.data_as(ctypes.c_void_p))
     if rc:
         raise sa.SpmspvError(rc, "mlp forward")
-    return z.argmax(axis=1)
+    return z
+
+
+def predict(w, X):
+    return logits(w, X).argmax(axis=1)
+
+
+FAMILY = np.array([0, 0, 0, 0, 1, 1, 2, 2])  # column, row SpMSpV, SpMV (csrc/kernels.hpp slot_family)
+
+
+def switching_choices(z, seq, penalty):
+    """The device's switching-aware choice (csrc/selector.hpp) replayed on
+    logits z in call order: keep the previous call's family when its best
+    slot is within `penalty` of the top logit; seq marks the sequences."""
+    out = np.zeros(len(z), np.int64)
+    prev = -1
+    for i in range(len(z)):
+        if i and seq[i] != seq[i - 1]:
+            prev = -1
+        best = int(np.argmax(z[i]))
+        if penalty > 0 and prev >= 0 and FAMILY[best] != FAMILY[prev]:
+            same = np.nonzero(FAMILY == FAMILY[prev])[0]
+            k = int(same[np.argmax(z[i][same])])
+            if z[i][k] >= z[i][best] - penalty:
+                best = k
+        out[i] = best
+        prev = best
+    return out
+
+
+def real_loss_ratio(times, single, seq, choice):
+    """The reference's switching-aware accounting (model/1-test_dtree.py:
+    266-287, 312-313): a call costs its plain time c when its kernel family
+    differs from the previous call's in the sequence and c_single (x formats
+    kept from the previous call) otherwise; per sequence (pred_real -
+    best_real) / best with best the per-call oracle-best plain time, the
+    oracle itself charged the same way; averaged over the sequences."""
+    ratios = []
+    for sq in np.unique(seq):
+        idx = np.nonzero(seq == sq)[0]
+        best_k = times[idx].argmin(axis=1)
+        best_plain = float(times[idx, best_k].sum())
+        pr, br = 0.0, 0.0
+        for j, i in enumerate(idx):
+            ck, bk = int(choice[i]), int(best_k[j])
+            same_p = j > 0 and FAMILY[ck] == FAMILY[int(choice[idx[j - 1]])]
+            same_b = j > 0 and FAMILY[bk] == FAMILY[int(best_k[j - 1])]
+            pr += single[i, ck] if (j == 0 or same_p) else times[i, ck]
+            br += single[i, bk] if (j == 0 or same_b) else times[i, bk]
+        ratios.append((pr - br) / best_plain)
+    return float(np.mean(ratios))
 
 
 def geo_regret(times, choice):
@@ -276,6 +349,7 @@ def cmd_train(args):
     parts = [np.load(p, allow_pickle=False) for p in args.corpus]
     feats = np.concatenate([p["feats"] for p in parts])
     times = np.concatenate([p["times"] for p in parts])
+    single = np.concatenate([p["times_single"] if "times_single" in p.files else p["times"] for p in parts])
     rule = np.concatenate([p["rule"] for p in parts])
     meta = np.concatenate([p["meta"] for p in parts])
     train = meta[:, 0] < args.holdout_from
@@ -307,6 +381,14 @@ def cmd_train(args):
             "best_fixed_regret": fixed[best_fixed],
             "fixed_regrets": {sa.KERNEL_NAMES[s]: fixed[s] for s in range(sa.NUM_SLOTS)},
             "mlp_accuracy": float((pred == tm.argmin(axis=1)).mean()),
+            "real_loss_ratio": {
+                "mlp": real_loss_ratio(tm, single[mask], meta[mask, 0], pred),
+                "best_fixed": real_loss_ratio(tm, single[mask], meta[mask, 0],
+                                              np.full(mask.sum(), best_fixed)),
+                "mlp_switching_aware": {
+                    str(p): real_loss_ratio(tm, single[mask], meta[mask, 0],
+                                            switching_choices(logits(w, feats[mask]), meta[mask, 0], p))
+                    for p in (0.5, 1.0, 2.0, 4.0)}},
             "mlp_total_ms": float(tm[np.arange(len(pred)), pred].sum()),
             "best_fixed_total_ms": float(tm[:, best_fixed].sum()),
             "oracle_total_ms": float(tm.min(axis=1).sum()),
@@ -327,6 +409,10 @@ def main():
     c.add_argument("--reps", type=int, default=3)
     c.add_argument("--flush-mb", type=int, default=512)
     c.add_argument("--out", required=True)
+    c.add_argument("--no-single", dest="single", action="store_false",
+                   help="skip the second timing with x's formats kept from the previous call (c_single)")
+    c.add_argument("--plain-calls", dest="dispatched", action="store_false",
+                   help="label with plain spmspv_run times instead of the selector-dispatched call's")
     c.add_argument("--no-autotune", dest="autotune", action="store_false",
                    help="time the slots with the default widths instead of the autotuned ones")
     c.add_argument("--budget-s", type=float, default=0.0,

@@ -81,3 +81,50 @@ def test_bfs_rmat22_full_size(gpu):
             st = M.A.bfs(int(src), lev, policy=policy, use_graphs=graphs)
             np.testing.assert_array_equal(lev.to_host(), ref, err_msg=f"src {src} policy {policy}")
             assert int(st[0]) == nl
+
+
+@pytest.mark.parametrize("nsrc", [1, 7, 64])
+def test_bfs_multi_levels_exact(graph12, nsrc):
+    """Multi-source BFS: the levels of every source equal the CPU BFS's
+    (vertex-major output), sources including an isolated vertex and a
+    repeated one."""
+    M, cp, ri, srcs4, isolated = graph12
+    deg = np.diff(cp)
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.choice(np.nonzero(deg > 0)[0], size=nsrc, replace=True).astype(np.int32)
+    if nsrc > 2:
+        srcs[1] = srcs[0]  # repeated source
+        if isolated is not None:
+            srcs[2] = isolated
+    lev = sa.DeviceArray(M.n * nsrc, np.int32)
+    print(f"bfs_multi nsrc {nsrc}", flush=True)
+    st = M.A.bfs_multi(srcs, lev)
+    got = lev.to_host().reshape(M.n, nsrc)
+    edges, reached, maxl = 0.0, 0, 0
+    for i, s_ in enumerate(srcs):
+        ref, nl = oracle.bfs(M.n, cp, ri, int(s_))
+        np.testing.assert_array_equal(got[:, i], ref, err_msg=f"source {i} ({s_})")
+        edges += deg[ref >= 0].sum() / 2
+        reached += int((ref >= 0).sum())
+        maxl = max(maxl, nl)
+    assert int(st[3]) == reached
+    assert st[1] == edges
+    assert int(st[0]) == maxl
+
+
+def test_bfs_multi_path_and_components(gpu):
+    import scipy.sparse as sp
+    n = 300
+    rows = list(range(199)) + list(range(1, 200)) + [210, 211, 212, 211, 212, 210]
+    cols = list(range(1, 200)) + list(range(199)) + [211, 212, 210, 210, 211, 212]
+    A = sp.csr_matrix((np.ones(len(rows), np.float32), (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    M = sa.Matrix(n, n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float32))
+    cp, ri, _ = oracle.csr_to_csc(n, n, A.indptr, A.indices, A.data)
+    srcs = np.array([0, 199, 100, 211, 250, 5], np.int32)
+    lev = sa.DeviceArray(n * srcs.size, np.int32)
+    M.A.bfs_multi(srcs, lev)
+    got = lev.to_host().reshape(n, srcs.size)
+    for i, s_ in enumerate(srcs):
+        ref, _ = oracle.bfs(n, cp, ri, int(s_))
+        np.testing.assert_array_equal(got[:, i], ref, err_msg=f"source {s_}")

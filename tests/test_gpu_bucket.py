@@ -1,0 +1,165 @@
+"""The bucket slot (row bins summed in LDS windows) and the size rule that
+lets auto mode run it, against the FP64 oracle through the C-ABI.
+
+Index sets bit-exact; FP32 values within |y - y64| <= 1e-5 * sum_j |a_ij x_j|
+(the bound of every value check here)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import spmspv_amd as sa
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check(m, cp, ri, cv, key, val, yk, yv, tag):
+    ek, ev, eb = oracle.spmspv_f64(m, cp, ri, cv, key, val)
+    assert yk.size == ek.size, f"{tag}: ynnz {yk.size} != {ek.size}"
+    np.testing.assert_array_equal(yk, ek, err_msg=tag)
+    err = np.abs(yv.astype(np.float64) - ev)
+    bad = np.nonzero(err > TOL * eb + 1e-30)[0]
+    assert bad.size == 0, f"{tag}: {bad.size} values out of bound, first rows {ek[bad[:5]]}"
+
+
+def bucket(M, key, val, buckets=0):
+    dk = sa.DeviceArray.from_host(key if key.size else np.zeros(1, np.int32), np.int32)
+    dv = sa.DeviceArray.from_host(val if val.size else np.zeros(1, np.float32), np.float32)
+    M.A.set_sparse_x(dk, dv, int(key.size))
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    c = ctypes.c_int(-1)
+    rc = sa.lib().spmspv_csc_bucket(M.A.h, 1.0, ctypes.byref(c), yk.ptr, yv.ptr, buckets)
+    assert rc == 0, f"spmspv_csc_bucket returned {rc}"
+    return yk.to_host(c.value), yv.to_host(c.value)
+
+
+@pytest.fixture(scope="module")
+def rmat16(gpu):
+    """R-MAT scale 16 (65,536 rows): with 256 bins of 256 rows the hub bins
+    hold many 16,384-product chunks and the tail bins one chunk each."""
+    rp, col, val = sa.gen_rmat_csr(16, 16, seed=31)
+    n = 1 << 16
+    M = sa.Matrix(n, n, rp, col, val)
+    return M, oracle.csr_to_csc(n, n, rp, col, val)
+
+
+@pytest.mark.parametrize("buckets", [2, 16, 256])
+@pytest.mark.parametrize("density", [1e-3, 0.05, 1.0])
+def test_bins_and_chunks(rmat16, buckets, density):
+    M, (cp, ri, cv) = rmat16
+    k = max(1, int(round(density * M.n)))
+    key, val = sa.gen_sparse_vector(M.n, k, 900 + k)
+    say(f"bucket slot, rmat16, {buckets} bins, xnnz {k}")
+    yk, yv = bucket(M, key, val, buckets)
+    check(M.m, cp, ri, cv, key, val, yk, yv, f"{buckets} bins density {density}")
+
+
+def test_many_rows_per_bin_limit(gpu):
+    """2^20 rows: at most 2^15 rows per bin gives 32 bins at least, whatever
+    the requested count (2 here); one hub row collects 60,000 products."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(12)
+    m = n = 1 << 20
+    rows = np.concatenate([rng.integers(0, m, 400000), np.full(60000, 777)])
+    cols = np.concatenate([rng.integers(0, n, 400000), rng.choice(n, 60000, replace=False)])
+    vals = rng.uniform(-1, 1, rows.size).astype(np.float32)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    M = sa.Matrix(m, n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float32))
+    cp, ri, cv = oracle.csr_to_csc(m, n, A.indptr.astype(np.int32), A.indices.astype(np.int32),
+                                   A.data.astype(np.float32))
+    for k in (1000, 100000, n):
+        key, val = sa.gen_sparse_vector(n, k, 40 + k)
+        for b in (2, 256):
+            say(f"bucket slot, 2^20 rows, {b} bins requested, xnnz {k}")
+            yk, yv = bucket(M, key, val, b)
+            check(m, cp, ri, cv, key, val, yk, yv, f"2^20 rows xnnz {k} bins {b}")
+
+
+def test_repeat_calls_leave_counts_clear(rmat16):
+    """The per-bin counts, the accumulator and the marks return to zero
+    after every call: alternating sizes give the oracle's result each time."""
+    M, (cp, ri, cv) = rmat16
+    for it, k in enumerate([50, 40000, 3, 65536, 700]):
+        key, val = sa.gen_sparse_vector(M.n, k, 5000 + it)
+        yk, yv = bucket(M, key, val)
+        check(M.m, cp, ri, cv, key, val, yk, yv, f"repeat {it}")
+
+
+def test_rule_routes_auto_calls(rmat16):
+    """With the rule's range over every size, an auto call whose network
+    choice is 2, 3, 5 or 7 runs the bucket slot (and gives the oracle's
+    result); with the rule off the bucket slot never runs."""
+    M, (cp, ri, cv) = rmat16
+    A = M.A
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    try:
+        # the rule fallback's choice (mode 1) is what the rule sees; a pinned
+        # slot (mode 2) bypasses the rule, and so does the single-launch
+        # small-x path, which is off here
+        A.set_small_x_path(False)
+        A.selector_mode(1)
+        A.set_bucket_range(0, 1 << 62)
+        for k in (700, 6000, 30000, 65536):
+            key, val = sa.gen_sparse_vector(M.n, k, 6100 + k)
+            dk = sa.DeviceArray.from_host(key, np.int32)
+            dv = sa.DeviceArray.from_host(val, np.float32)
+            A.set_sparse_x(dk, dv, k)
+            A.selector_mode(1)
+            plain = A.select()
+            c, used = A.run_auto(y_key=yk, y_val=yv)
+            say(f"rule on: xnnz {k} rule fallback picks {plain}, ran {used}")
+            expect = sa.K_COL_BUCKET if plain in (2, 3, 5, 7) else plain
+            assert used == expect
+            check(M.m, cp, ri, cv, key, val, yk.to_host(c), yv.to_host(c), f"rule on xnnz {k}")
+        A.set_bucket_range(0, -1)
+        for k in (6000, 65536):
+            key, val = sa.gen_sparse_vector(M.n, k, 6100 + k)
+            dk = sa.DeviceArray.from_host(key, np.int32)
+            dv = sa.DeviceArray.from_host(val, np.float32)
+            A.set_sparse_x(dk, dv, k)
+            c, used = A.run_auto(y_key=yk, y_val=yv)
+            assert used != sa.K_COL_BUCKET
+            check(M.m, cp, ri, cv, key, val, yk.to_host(c), yv.to_host(c), f"rule off xnnz {k}")
+    finally:
+        A.set_bucket_range(0, -1)
+        A.set_small_x_path(True)
+
+
+def test_autotuned_rule_then_parity(rmat16):
+    """The autotuner's range is off or ordered; auto calls across densities
+    match the oracle whichever slot the rule sends them to."""
+    M, (cp, ri, cv) = rmat16
+    A = M.A
+    key = np.arange(M.n, dtype=np.int32)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(np.ones(M.n, np.float32), np.float32)
+    A.set_sparse_x(dk, dv, M.n)
+    tuned = A.autotune(reps=2)
+    lo, hi = tuned["bucket_rule"]
+    say(f"autotuned bucket rule {lo}..{hi}")
+    assert (lo, hi) == (0, -1) or 0 < lo <= hi
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    A.selector_mode(1)
+    for d in (1e-3, 1e-2, 0.1, 0.5, 1.0):
+        k = max(1, int(round(d * M.n)))
+        key, val = sa.gen_sparse_vector(M.n, k, 7100 + k)
+        dk = sa.DeviceArray.from_host(key, np.int32)
+        dv = sa.DeviceArray.from_host(val, np.float32)
+        A.set_sparse_x(dk, dv, k)
+        c, used = A.run_auto(y_key=yk, y_val=yv)
+        say(f"autotuned rule: density {d} ran {used}")
+        check(M.m, cp, ri, cv, key, val, yk.to_host(c), yv.to_host(c), f"autotuned density {d}")
+    A.set_bucket_range(0, -1)

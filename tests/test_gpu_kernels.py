@@ -47,7 +47,7 @@ def run_slot(M, k, key, val, want_dense=False):
     return out
 
 
-SLOTS = list(range(9))  # the eight selector slots + the bucket kernel
+SLOTS = list(range(11))  # the eight selector slots, the bucket kernel and the two-layer pair
 
 
 @pytest.fixture(scope="module")

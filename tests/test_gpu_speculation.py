@@ -50,6 +50,9 @@ def test_forced_slot_sequence(rmat12_csr5):
     A.set_sparse_x(dk, dv, key.size)
     A.run(7, y_key=yk, y_val=yv)
     A.set_speculation(True)
+    # this x is small enough for the single-launch column path, which would
+    # finish the column calls without a guess; the guesses are the subject here
+    A.set_small_x_path(False)
     h0, m0 = A.speculation_stats()
     seq = [2, 2, 7, 7, 0, 5, 5, 2, 3, 3, 1, 1, 2]
     try:
@@ -72,6 +75,7 @@ def test_forced_slot_sequence(rmat12_csr5):
                 assert np.all(np.abs(dense - ref) <= TOL * bound + 1e-30), f"call {i}: dense output"
     finally:
         A.selector_mode(1)
+        A.set_small_x_path(True)
     h1, m1 = A.speculation_stats()
     # (previous choice -> forced slot): none->2 no guess, 2->2 right, 2->7
     # wrong, 7->7 right, 7->0 wrong, 0->5 wrong, 5->5 right, 5->2 wrong,

@@ -353,3 +353,17 @@ def test_pagerank_cycle_is_uniform_and_eps_bound_holds():
         got, sizes = oracle.pagerank_f64(200, A.indptr, A.indices, A.data, 0.85, eps, 5000)
         assert np.abs(got - want).sum() <= eps * 0.85
         assert sizes.size > 1 and sizes[0] == 200
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_allcore_port_equals_serial_bit_for_bit(threads):
+    m, n = 500, 400
+    A = rand_csr(m, n, 0.05, 21 + threads).tocsc()
+    A.sort_indices()
+    key, val = rand_x(n, 150, 5)
+    key = key[np.random.default_rng(1).permutation(key.size)]  # any key order
+    yd, yk, yv = oracle.spmspv_csc_host_f32(m, A.indptr, A.indices, A.data, key, val)
+    pd, pk, pv = oracle.spmspv_csc_host_f32_omp(m, A.indptr, A.indices, A.data, key, val, threads)
+    np.testing.assert_array_equal(pd.view(np.uint32), yd.view(np.uint32))
+    np.testing.assert_array_equal(pk, yk)
+    np.testing.assert_array_equal(pv.view(np.uint32), yv.view(np.uint32))
