@@ -122,12 +122,14 @@ static size_t carve(spmspv_handle_t h, char* base) {
   float* w = c.take<float>(SPMSPV_MLP_NWEIGHTS);
   float* wb = c.take<float>(SPMSPV_MLP_NWEIGHTS);
   long long* selp = c.take<long long>(5 * kSelMaxGrid);
+  long long* psamp = c.take<long long>(2 * (kSampleKeys / kPrepNT));
   BinWs bws;
   bws.hist = c.take<unsigned>(kBinMax);
   bws.cursor = c.take<unsigned long long>(kBinMax);
   bws.bstart = c.take<long long>(kBinMax + 1);
   bws.cstart = c.take<int>(kBinMax + 1);
   if (base) {
+    h->prep_samp = psamp;
     h->bin_ws = bws;
     h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
     h->sort_words = dscan_tiles;
@@ -373,7 +375,8 @@ int ensure_binlen_host(spmspv_handle_t h) {
 
 int run_prep(spmspv_handle_t h, bool with_split, const SelectorArgs* sel) {
   launch_prep(h->stream, h->csc_colptr, h->x_key, h->n, h->xnnz, nullptr, h->ptr, h->xbeg,
-              with_split ? h->split : nullptr, h->scan_state, h->tile_mm, h->ctl, sel, next_ep(h, 1));
+              with_split ? h->split : nullptr, h->scan_state, h->tile_mm, h->ctl, sel, next_ep(h, 1),
+              h->prep_samp);
   h->launches++;
   int rc = check_launch(h, "prep");
   if (rc) return rc;
@@ -680,7 +683,7 @@ int spmspv_create(spmspv_handle_t* out, int m, int n, int nnz) {
   h->team_row = std::min(64, std::max(2, pow2_team((double)nnz / m)));
   hipError_t e = hipHostMalloc((void**)&h->h_pin, 64 * sizeof(int), hipHostMallocDefault);
   if (e == hipSuccess)
-    e = hipHostMalloc((void**)&h->sig, 16 * sizeof(long long), hipHostMallocCoherent | hipHostMallocMapped);
+    e = hipHostMalloc((void**)&h->sig, kSigWords * sizeof(long long), hipHostMallocCoherent | hipHostMallocMapped);
   if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&h->sig_dev, h->sig, 0);
   if (e != hipSuccess) {
     if (h->h_pin) (void)hipHostFree(h->h_pin);

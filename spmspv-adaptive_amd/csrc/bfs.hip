@@ -17,8 +17,10 @@
 //   K6     naive row scan without early exit, hit bits, then claim.
 // Levels are int32 and unique, so any order of claims gives the same
 // levels. The selector for level L+1 runs in the same stream right after
-// level L (features of the new frontier + MLP, one launch), so a level
-// costs one host synchronisation; with graphs on, each (slot, parity) level
+// level L (features of the new frontier + MLP, one launch), and a last
+// one-wave kernel writes the next frontier's size and choice, tagged with
+// the level counter, into one host-coherent word that the host polls (no
+// stream wait, no copies); with graphs on, each (slot, parity) level
 // sequence is captured once as a hipGraph and replayed. Every kernel reads
 // the frontier size from device memory, which is what makes the capture
 // valid for any level.
@@ -52,7 +54,6 @@ namespace {
 constexpr int kBfsGrid = 2048;
 constexpr int kBuBatch = 8;  // neighbours each row tests in the bottom-up head pass
 constexpr int kBuTeam = 8;   // lanes per queued row in the bottom-up tail pass
-constexpr int kFeatPin = 32;  // h_pin words 32..47: the next level's selector features
 
 __device__ __forceinline__ int wave_append(int* counter, bool pred) {
   const uint64_t b = __ballot(pred);
@@ -108,6 +109,21 @@ __global__ void k_bfs_init(int* levels, int n, unsigned* visited, unsigned* fb0,
 // Opens a level: empties the next frontier, advances the level number and
 // the device epoch that the level's look-back kernels use (their launch
 // arguments are frozen inside a captured graph).
+// The next level's frontier size and selector choice, tagged with the level
+// counter, into one host-coherent word (sig[0]), its selector features into
+// sig[1..8] first: the host polls one word per level instead of waiting for
+// the stream and three copies.
+__global__ void k_bfs_signal(const int* __restrict__ cnt, int nxt, const Ctl* __restrict__ ctl, long long* sig) {
+  if (threadIdx.x < 16) reinterpret_cast<float*>(sig + 1)[threadIdx.x] = ctl->feat[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long long w = ((long long)(cnt[2] & 0xffffff) << 40) | ((long long)(ctl->kernel_choice & 0xff) << 32) |
+                        (long long)(unsigned)cnt[nxt];
+    __threadfence_system();
+    __hip_atomic_store(sig, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 __global__ void k_bfs_begin(int* cnt, int nxt, unsigned* nbits, int words, Ctl* ctl) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[nxt] = 0;
@@ -447,10 +463,8 @@ void enqueue_level(spmspv_handle_t h, hipStream_t s, int slot, int cur, int* lev
     }
   }
   launch_prep(s, h->csc_colptr, nf, h->n, 0, b->cnt + nxt, h->ptr, h->xbeg, h->split, h->scan_state,
-              h->tile_mm, h->ctl, &sel, Ep{0, 0});
-  (void)hipMemcpyAsync(h->h_pin + 16, b->cnt + nxt, sizeof(int), hipMemcpyDeviceToHost, s);
-  (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
-  (void)hipMemcpyAsync(h->h_pin + kFeatPin, h->ctl->feat, sizeof(float) * 16, hipMemcpyDeviceToHost, s);
+              h->tile_mm, h->ctl, &sel, Ep{0, 0}, h->prep_samp);
+  hipLaunchKernelGGL(k_bfs_signal, dim3(1), dim3(64), 0, s, b->cnt, nxt, h->ctl, h->sig_dev + kSigBfs);
 }
 
 }  // namespace
@@ -559,23 +573,41 @@ static int bfs_run(spmspv_handle_t h, int src, int* d_levels, int policy, int us
   hipLaunchKernelGGL(k_bfs_init, dim3(std::min(4096, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,
                      h->n, b->visited, b->fbits[0], b->fbits[1], b->front[0], b->cnt, b->reached, src);
   launch_prep(s, h->csc_colptr, b->front[0], h->n, 0, b->cnt, h->ptr, h->xbeg, h->split,
-              h->scan_state, h->tile_mm, h->ctl, &sel, next_ep(h, 1));
-  (void)hipMemcpyAsync(h->h_pin + 16, b->cnt, sizeof(int), hipMemcpyDeviceToHost, s);
-  (void)hipMemcpyAsync(h->h_pin + 17, &h->ctl->kernel_choice, sizeof(int), hipMemcpyDeviceToHost, s);
-  (void)hipMemcpyAsync(h->h_pin + kFeatPin, h->ctl->feat, sizeof(float) * 16, hipMemcpyDeviceToHost, s);
-  hipError_t e = hipStreamSynchronize(s);
+              h->scan_state, h->tile_mm, h->ctl, &sel, next_ep(h, 1), h->prep_samp);
+  // the level words: frontier size, choice and the level counter as tag
+  long long* sw = h->sig + kSigBfs;
+  __atomic_store_n(sw, kSigPending, __ATOMIC_RELEASE);
+  hipLaunchKernelGGL(k_bfs_signal, dim3(1), dim3(64), 0, s, b->cnt, 0, h->ctl, h->sig_dev + kSigBfs);
+  hipError_t e = hipGetLastError();
   const bool trace = max_levels > 0 && level_ms && level_feat;
+  // waits for the word of level `tag` (polling; the trace's events need the
+  // stream drained instead)
+  long long word = 0;
+  bool out_of_sequence = false;
+  auto level_word = [&](int tag) -> hipError_t {
+    if (trace) {
+      hipError_t r = hipStreamSynchronize(s);
+      if (r != hipSuccess) return r;
+      word = __atomic_load_n(sw, __ATOMIC_ACQUIRE);
+    } else if (wait_signal(h, kSigBfs, &word) != SPMSPV_SUCCESS) {
+      return hipErrorUnknown;
+    }
+    if ((word >> 40) != tag) out_of_sequence = true;
+    return out_of_sequence ? hipErrorUnknown : hipSuccess;
+  };
+  if (e == hipSuccess) e = level_word(0);
   int cur = 0, nlev = 1;
   int slot_hist[8] = {0};
-  while (e == hipSuccess && h->h_pin[16] > 0) {
-    int slot = policy >= 0 ? policy : h->h_pin[17];
+  while (e == hipSuccess && (int)(word & 0xffffffff) > 0) {
+    int slot = policy >= 0 ? policy : (int)((word >> 32) & 0xff);
     if (slot < 0 || slot >= SPMSPV_NUM_SLOTS) slot = SPMSPV_K_COL_LB_ATOMIC;
     slot_hist[slot]++;
     const int lev = nlev - 1;  // index of the level about to run
     if (trace && lev < max_levels) {
-      std::memcpy(level_feat + 16 * lev, h->h_pin + kFeatPin, sizeof(float) * 16);
+      std::memcpy(level_feat + 16 * lev, sw + 1, sizeof(float) * 16);
       (void)hipEventRecord(h->ev0, s);
     }
+    __atomic_store_n(sw, kSigPending, __ATOMIC_RELEASE);
     if (use_graphs) {
       hipGraphExec_t& ge = b->graph[slot][cur];
       if (!ge) {
@@ -598,7 +630,7 @@ static int bfs_run(spmspv_handle_t h, int src, int* d_levels, int policy, int us
       e = hipGetLastError();
     }
     if (trace && lev < max_levels && e == hipSuccess) e = hipEventRecord(h->ev1, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = level_word(nlev);
     if (trace && lev < max_levels && e == hipSuccess) {
       float ms = 0.f;
       e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
@@ -619,6 +651,7 @@ static int bfs_run(spmspv_handle_t h, int src, int* d_levels, int policy, int us
     h->dev_levels = 0;
   }
   h->stream = saved;
+  if (out_of_sequence) return set_error(h, SPMSPV_DEVICE_ERROR, "bfs: a level's word arrived out of sequence");
   if (e != hipSuccess) return set_error(h, SPMSPV_DEVICE_ERROR, std::string("bfs: ") + hipGetErrorString(e));
   hipLaunchKernelGGL(k_reached, dim3(std::min(1024, (h->n + 255) / 256)), dim3(256), 0, s, d_levels,
                      h->csc_colptr, h->n, b->reached);

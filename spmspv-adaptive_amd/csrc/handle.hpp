@@ -62,6 +62,7 @@ struct spmspv_handle_s {
   int* split = nullptr;      // LB tiles + 2
   int* tl_starts = nullptr;  // LB tiles + 2: first x entry of each two-layer tile
   int* tile_mm = nullptr;    // 2 per prep tile
+  long long* prep_samp = nullptr;  // the first prep tiles' sampled selector partials
   unsigned long long* scan_state = nullptr;
   size_t scan_words = 0, sort_words = 0, red_words = 0;  // look-back state sizes
   float* acc = nullptr;      // m, all-zero between calls
@@ -169,6 +170,9 @@ struct spmspv_handle_s {
   } while (0)
 
 constexpr long long kSigPending = -0x5A5A5A5A5A5ALL;
+// host-coherent signal words: 1 selector word, 3 result count, 16.. a BFS
+// level's frontier size and choice (16) and its selector features (17..24)
+constexpr int kSigWords = 32, kSigBfs = 16;
 
 namespace spmspv {
 int set_error(spmspv_handle_t h, int code, const std::string& msg);
