@@ -25,6 +25,7 @@ PageRank driver, the config-1 uniform matrix, and the reference's CPU path
 build's all-core port.
 """
 import argparse
+import glob
 import json
 import os
 import sys
@@ -689,6 +690,12 @@ def main():
     for i in dom_is:
         tpath = os.path.join(ROOT, "profiles",
                              f"{sa.KERNEL_NAMES[dom_slot]}{fmt}_rmat{args.scale}_d{DENSITIES[i]}.json")
+        if not os.path.exists(tpath) and fmt.startswith("-csr5-s"):
+            # the autotuner's CSR5 width can differ between runs where the
+            # widths are within 2-3 %: the profile of another width
+            alts = sorted(glob.glob(os.path.join(ROOT, "profiles",
+                                                 f"{sa.KERNEL_NAMES[dom_slot]}-csr5-s*_rmat{args.scale}_d{DENSITIES[i]}.json")))
+            tpath = alts[0] if alts else tpath
         if not os.path.exists(tpath):
             traffic_src = None
             break
