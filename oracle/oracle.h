@@ -57,7 +57,7 @@ int oracle_spmspv_csc_host_f32_omp(int m, const int* col_ptr, const int* row_idx
                                    const float* x_val, float* y_dense,
                                    int* y_key, float* y_val, int threads);
 
-/* Judged FP64 SpMSpV, column-driven (the algorithm of SpmSpvCSCHost,
+/* FP64 SpMSpV the results are checked against, column-driven (the algorithm of SpmSpvCSCHost,
  * la/test/test_sort_spmspv.cu:342-351) with FP64 accumulation and the
  * structural support: a row is in the output iff some selected column has a
  * stored entry in it, whatever the value (also x values of 0).
@@ -68,7 +68,7 @@ int oracle_spmspv_f64(int m, const int* col_ptr, const int* row_idx,
                       const float* x_val, double* y64, double* bound,
                       int* y_key, double* y_val, double* y_bound);
 
-/* Judged FP64 SpMV with dense x (row-driven, the loop of serialspmv,
+/* FP64 SpMV with dense x the results are checked against (row-driven, the loop of serialspmv,
  * la/test/test_application.cu:107-116, in FP64). Writes y[m], bound[m]. */
 void oracle_spmv_f64(int m, const int* row_ptr, const int* col,
                      const float* val, const float* x_dense,

@@ -122,7 +122,7 @@ def spmspv_csc_host_f32_omp(m, col_ptr, row_idx, cval, x_key, x_val, threads):
 
 
 def spmspv_f64(m, col_ptr, row_idx, cval, x_key, x_val):
-    """Judged FP64 SpMSpV with structural support. Returns (keys, vals, bounds)."""
+    """FP64 SpMSpV with structural support, the results are checked against. Returns (keys, vals, bounds)."""
     y64 = np.zeros(max(m, 1), np.float64)
     bnd = np.zeros(max(m, 1), np.float64)
     yk = np.zeros(max(m, 1), np.int32)
@@ -137,7 +137,7 @@ def spmspv_f64(m, col_ptr, row_idx, cval, x_key, x_val):
 
 
 def spmv_f64(m, row_ptr, col, val, x_dense):
-    """Judged FP64 dense-x SpMV. Returns (y, bound)."""
+    """FP64 dense-x SpMV the results are checked against. Returns (y, bound)."""
     y = np.zeros(max(m, 1), np.float64)
     b = np.zeros(max(m, 1), np.float64)
     col = _i32(col) if len(col) else np.zeros(1, np.int32)

@@ -477,32 +477,29 @@ __global__ __launch_bounds__(NT) void k_select_feat(const int* __restrict__ colp
 }
 
 // Features-only selector for a large x (xnnz > kSampleMin) when the caller
-// takes an estimated bin_len: the first kSampleKeys / NT workgroups take one
-// sampled key per thread, publish their partial sum and maximum and count
-// themselves done, and the last of them runs the selector; only then does
-// any workgroup scatter its share of x's support encoding, so the network
-// runs while the scatter's stores are in flight (no workgroup waits for
-// another). Sampling inside one workgroup made that workgroup's 12,000
-// cache-line fills the kernel's critical path, and sampling after the
-// scatter put the network behind it. The selector word carries the
-// estimate, flagged.
+// takes an estimated bin_len: workgroup 0 (NT = kSampleKeys threads) loads
+// one sampled key per thread with the selector's weights, then the keys'
+// column bounds, reduces them and runs the selector, with no step that waits
+// for another workgroup; workgroups 1.. scatter x's support encoding
+// meanwhile. (Spreading 4,096 samples over 16 workgroups needed their
+// partials to meet through memory, two more dependent round trips; 4,096
+// samples in one workgroup were 12,000 cache-line fills through one L1.)
+// The selector word carries the estimate, flagged.
 template <int NT, int KPT>
 __global__ __launch_bounds__(NT) void k_select_sampled(const int* __restrict__ colptr,
                                                        const int* __restrict__ xkey,
                                                        const float* __restrict__ xval, int ncols, int xnnz,
-                                                       long long* part, Ctl* ctl, SelectorArgs sel,
-                                                       float* __restrict__ enc, const int* __restrict__ enc_perm) {
-  constexpr int kSampT = kSampleKeys / NT;  // the grid has at least this many workgroups
+                                                       Ctl* ctl, SelectorArgs sel, float* __restrict__ enc,
+                                                       const int* __restrict__ enc_perm) {
+  static_assert(NT == kSampleKeys, "one sampled key per thread of workgroup 0");
   __shared__ __align__(16) float s_w[kNWeights];
   __shared__ MatFeat s_mf;
   __shared__ long long s_l[NT / kWave];
   __shared__ int s_i[NT / kWave];
-  __shared__ unsigned s_last;
-  if (blockIdx.x < kSampT) {
+  if (blockIdx.x == 0) {
     constexpr int kWPT = (kNWeights + NT - 1) / NT;
     constexpr int kMF = sizeof(MatFeat) / sizeof(double);
     static_assert(kMF <= NT, "MatFeat is copied as doubles");
-    // any sampling workgroup may be the last: each holds the selector's inputs
     float wr[kWPT];
     if (sel.weights) {
 #pragma unroll
@@ -513,8 +510,7 @@ __global__ __launch_bounds__(NT) void k_select_sampled(const int* __restrict__ c
     }
     const double mfr = threadIdx.x < kMF ? reinterpret_cast<const double*>(sel.mat)[threadIdx.x] : 0.0;
     // this thread's sample (the positions sample_x takes)
-    const long long sidx = (long long)blockIdx.x * NT + threadIdx.x;
-    const int sc = xkey[sidx * xnnz / kSampleKeys];
+    const int sc = xkey[(long long)threadIdx.x * xnnz / kSampleKeys];
     const int sok = (unsigned)sc < (unsigned)ncols ? 1 : 0;
     const int sci = sc & -sok;
     const int len = colptr[sci + sok] - colptr[sci];
@@ -526,52 +522,26 @@ __global__ __launch_bounds__(NT) void k_select_sampled(const int* __restrict__ c
       }
     }
     if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
-    long long sum = block_sum<NT, long long>((long long)len, s_l);
-    int mx = block_max<NT, int>(len, s_i);
+    const long long sum = block_sum<NT, long long>((long long)len, s_l);  // its barriers publish s_w, s_mf
+    const int mx = block_max<NT, int>(len, s_i);
+    const long long bl = (long long)llrint((double)sum * (double)xnnz / (double)kSampleKeys);
     if (threadIdx.x == 0) {
-      __hip_atomic_store(&part[blockIdx.x], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&part[kSelMaxGrid + blockIdx.x], (long long)mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // partials in place before the count
-      const unsigned before =
-          __hip_atomic_fetch_add(&ctl->done[kSlotSelect], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_last = before == kSampT - 1 ? 1u : 0u;
-      // the last one resets the count for the next launch (stream-ordered)
-      if (before == kSampT - 1)
-        __hip_atomic_store(&ctl->done[kSlotSelect], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ctl->bin_len = bl;
+      ctl->max_len = mx;
+      ctl->min_len = 0;
+      ctl->xnnz = xnnz;
     }
-    __syncthreads();
-    if (s_last) {
-      long long osum = 0;
-      int omx = 0;
-      for (int u = threadIdx.x; u < kSampT; u += NT) {
-        if (u == (int)blockIdx.x) continue;
-        osum += __hip_atomic_load(&part[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        omx = max(omx, (int)__hip_atomic_load(&part[kSelMaxGrid + u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      }
-      sum += block_sum<NT, long long>(osum, s_l);
-      mx = max(mx, block_max<NT, int>(omx, s_i));
-      const long long bl = (long long)llrint((double)sum * (double)xnnz / (double)kSampleKeys);
-      if (threadIdx.x == 0) {
-        ctl->bin_len = bl;
-        ctl->max_len = mx;
-        ctl->min_len = 0;
-        ctl->xnnz = xnnz;
-      }
-      SelectorArgs sl = sel;
-      if (sel.weights) sl.weights = s_w;
-      sl.mat = &s_mf;
-      select_in_wave(ctl, sl, (double)xnnz, (double)bl, (double)mx);
-      if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(bl, ctl->kernel_choice | kSelEstimated);
-    }
+    SelectorArgs sl = sel;
+    if (sel.weights) sl.weights = s_w;
+    sl.mat = &s_mf;
+    select_in_wave(ctl, sl, (double)xnnz, (double)bl, (double)mx);
+    if (sel.stash && threadIdx.x == 0) *sel.stash = sel_word(bl, ctl->kernel_choice | kSelEstimated);
+    return;
   }
-  // this workgroup's share of the support encoding: the workgroups after
-  // the sampling ones scatter (all of them when there are no others)
+  // workgroups 1..: the support encoding
   if (!enc) return;
-  const bool others = gridDim.x > (unsigned)kSampT;
-  if (others && blockIdx.x < kSampT) return;
-  const long long sb = others ? blockIdx.x - kSampT : blockIdx.x;
-  const long long sg = others ? gridDim.x - kSampT : gridDim.x;
   const int w = threadIdx.x / kWave;
+  const long long sb = blockIdx.x - 1, sg = gridDim.x - 1;
   for (long long t0 = sb * NT * KPT; t0 < xnnz; t0 += sg * NT * KPT) {
     const long long b0 = t0 + (long long)w * kWave * KPT + lane();
     int c[KPT], pc[KPT];
@@ -602,16 +572,13 @@ void launch_select_feat(hipStream_t s, const int* colptr, const int* xkey, const
                         int ncols, int xnnz, long long* part, Ctl* ctl, const SelectorArgs& sel,
                         float* enc, const int* enc_perm) {
   if (sel.estimate && sel.mat && xnnz > kSampleMin) {
-    constexpr int kPer = 256 * 16;
-    // the sampling workgroups (one key per thread), then with the encoding
-    // one round of 16 entries per thread for the others, kSelMaxGrid
-    // workgroups at most
-    const int sgrid = kSampleKeys / 256;
-    const int grid = enc ? sgrid + (int)std::max<long long>(1, std::min<long long>(kSelMaxGrid - sgrid,
-                                                                                    ((long long)xnnz + kPer - 1) / kPer))
-                         : sgrid;
-    hipLaunchKernelGGL((k_select_sampled<256, 16>), dim3(grid), dim3(256), 0, s, colptr, xkey, xval, ncols,
-                       xnnz, part, ctl, sel, enc, enc_perm);
+    constexpr int kPer = kSampleKeys * 4;
+    // the sampling workgroup, then with the encoding one round of 4 entries
+    // per thread for the others, 512 workgroups at most
+    const int grid = enc ? 1 + (int)std::max<long long>(1, std::min<long long>(511, ((long long)xnnz + kPer - 1) / kPer))
+                         : 1;
+    hipLaunchKernelGGL((k_select_sampled<kSampleKeys, 4>), dim3(grid), dim3(kSampleKeys), 0, s, colptr, xkey, xval,
+                       ncols, xnnz, ctl, sel, enc, enc_perm);
     return;
   }
   // 16 keys per thread fill at most 512 workgroups up to 2 M keys; below

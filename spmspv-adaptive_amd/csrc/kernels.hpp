@@ -77,7 +77,7 @@ __host__ __device__ inline int slot_family(int k) { return k < 4 ? 0 : (k < 6 ? 
 // bin_len is their column lengths' sum scaled by xnnz / kSampleKeys, and
 // max_len their longest column. Every launch that runs the selector uses
 // this definition, so training and dispatch see the same features.
-constexpr int kSampleMin = 1 << 16, kSampleKeys = 4096;
+constexpr int kSampleMin = 1 << 16, kSampleKeys = 1024;
 // {bin_len, kernel choice} in one 64-bit word, so that a single store
 // publishes both (no ordering fence); never negative, choice 0xff = none.
 // Flag kSelEstimated in the choice byte: bin_len is the sampled estimate.
