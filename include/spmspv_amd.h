@@ -195,12 +195,19 @@ int spmspv_csc_nosort_merge_keyval(spmspv_handle_t h, int is_binlen_computed, fl
                                    int* ynnz, float* d_y, int* d_y_key, float* d_y_val);
 int spmspv_csc_sort_merge(spmspv_handle_t h, int is_binlen_computed, float alpha, int* ynnz,
                           float* d_y, int* d_y_key, float* d_y_val);
-/* CscBasedBucketSpmspv (csc_bucket_spmspv.cuh:766-1086): products counted
- * per row bin, written into their bins, each bin summed in an LDS window.
- * buckets = the least number of row bins, a power of two in [2, 256] (bins
- * hold at most 32,768 rows, so large m gets more); 0 keeps the handle's
- * current count (256, or the autotuner's pick). Other values:
- * INVALID_ARGUMENT; m above 2^26 rows: INVALID_ARGUMENT. */
+/* CscBasedBucketSpmspv (csc_bucket_spmspv.cuh:766-1086): products written
+ * into row bins (ranges of rows with about equal numbers of nonzeros, each
+ * owning as many product slots as its rows hold nonzeros, so no count pass),
+ * each bin summed per row in an LDS window (64-bit fixed point scaled to the
+ * row's largest product; FP32 error at most that of one rounding of the
+ * exact sum plus 2^-32 of the largest product). buckets = the number of
+ * equal-nonzero bins aimed at, a power of two in [2, 1024] (bins also hold
+ * at most 4,096 rows, so large m gets more); 0 keeps the handle's current
+ * count (512, or the autotuner's pick). The first call on a matrix (and on
+ * a new count) builds the layout on the host from the row pointer (one
+ * copy back and one synchronisation). Other values: INVALID_ARGUMENT; a
+ * matrix needing more than 6,144 bins (m above about 16.7 M rows):
+ * INVALID_ARGUMENT. */
 int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* d_y_key, float* d_y_val,
                       int buckets);
 /* CscBasedMyNoSortSpmspv[_keyval] / CscBasedSortMySpmspv (spmspv.h:1823-1878,
@@ -235,12 +242,12 @@ int spmspv_spmv(spmspv_handle_t h, float alpha, float* d_y);
 int spmspv_autotune(spmspv_handle_t h, int reps, int* report);
 /* As spmspv_autotune; csr5 = 0 keeps the current K5/K7 format and only
  * tunes its parameters' merge-path side. Both also pick the bucket slot's
- * number of row buckets (16..256), the two size thresholds of the
- * single-workgroup paths (the small-x single launch, up to 11,264 products;
- * the one-workgroup sort + reduce of the sort slots, up to 3,072 items) and
- * the bucket rule's bin_len range (spmspv_set_bucket_range), each timed on
- * sub-vectors of x (evenly spaced keys) whose bin_len lies just under a
- * candidate. */
+ * number of row bins (about 256, 512 or 1,024 of equal nonzeros), the two
+ * size thresholds of the single-workgroup paths (the small-x single launch,
+ * up to 11,264 products; the one-workgroup sort + reduce of the sort slots,
+ * up to 3,072 items) and the bucket rule's bin_len range
+ * (spmspv_set_bucket_range), each timed on sub-vectors of x (one key from
+ * each of k equal intervals) whose bin_len lies just under a candidate. */
 int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report);
 /* No counterpart in the reference: every tuned parameter, out[10] =
  * {team_col, team_row, row_vt, spmv_type, sigma (0 unless CSR5), row

@@ -20,12 +20,21 @@ using namespace spmspv;
 namespace {
 thread_local std::string g_err;
 
-// k entries of x evenly spaced over its xn (sorted keys stay sorted).
+// Entry i of a k-key sub-vector of x: a pseudo-random position inside the
+// i-th of k equal intervals of x, so the keys stay in order and the
+// sub-vector's columns look like those of a random x of that size (evenly
+// spaced positions of a dense x would be the columns whose low bits are
+// zero, on a power-law matrix its heaviest ones).
 __global__ void k_spread_pick(const int* __restrict__ xk, const float* __restrict__ xv, int xn, int k,
                               int* __restrict__ ok, float* __restrict__ ov) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k) return;
-  const long long j = (long long)i * xn / k;
+  const long long j0 = (long long)i * xn / k, j1 = (long long)(i + 1) * xn / k;
+  unsigned hsh = (unsigned)i * 0x9E3779B1u;
+  hsh ^= hsh >> 15;
+  hsh *= 0x85EBCA77u;
+  hsh ^= hsh >> 13;
+  const long long j = j1 > j0 ? j0 + (long long)(hsh % (unsigned long long)(j1 - j0)) : j0;
   ok[i] = xk[j];
   ov[i] = xv[j];
 }
@@ -124,13 +133,14 @@ static size_t carve(spmspv_handle_t h, char* base) {
   long long* selp = c.take<long long>(5 * kSelMaxGrid);
   long long* psamp = c.take<long long>(2 * (kSampleKeys / kPrepNT));
   BinWs bws;
-  bws.hist = c.take<unsigned>(kBinMax);
-  bws.cursor = c.take<unsigned long long>(kBinMax);
-  bws.bstart = c.take<long long>(kBinMax + 1);
-  bws.cstart = c.take<int>(kBinMax + 1);
+  bws.cursor = c.take<unsigned>((size_t)kBinMax * kBinCurStride);
+  bws.base = c.take<int>(kBinMax + 1);
+  bws.start = c.take<int>(kBinMax + 1);
+  bws.tbl = c.take<int>(kBinTbl);
   if (base) {
     h->prep_samp = psamp;
     h->bin_ws = bws;
+    h->bin_base_bits = -1;
     h->scan_words = std::max(std::max(prep_tiles, cmp_tiles), bucket_tiles);
     h->sort_words = dscan_tiles;
     h->red_words = red_tiles;
@@ -362,6 +372,44 @@ Ep next_ep(spmspv_handle_t h, unsigned k) {
   return e;
 }
 
+// The bucket slot's bins for the current matrix and bucket_bits, built on
+// the host from the row pointer (copied back, or counted from the CSC's
+// row indices for a CSC-only matrix) once per matrix and bin count.
+int ensure_bin_layout(spmspv_handle_t h) {
+  if (h->bin_base_bits == h->bucket_bits) return SPMSPV_SUCCESS;
+  const int m = h->m;
+  std::vector<int> rp(m + 1, 0);
+  if (h->csr_row) {
+    HIP_TRY(h, hipMemcpyAsync(rp.data(), h->csr_row, sizeof(int) * (m + 1), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  } else {
+    int rc;
+    if ((rc = need_csc(h))) return rc;
+    launch_fill_i32(h->stream, h->yk_tmp, m, 0);
+    launch_row_counts(h->stream, h->csc_row, h->nnz, h->yk_tmp);
+    HIP_TRY(h, hipMemcpyAsync(rp.data() + 1, h->yk_tmp, sizeof(int) * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < m; ++i) rp[i + 1] += rp[i];
+  }
+  BinLayout L;
+  const int nb = bin_layout(rp, m, h->bucket_bits, &L);
+  if (nb < 0)
+    return set_error(h, SPMSPV_INVALID_ARGUMENT,
+                     "bucket slot: more than " + std::to_string(kBinMax) + " row bins would be needed");
+  HIP_TRY(h, hipMemcpyAsync(h->bin_ws.start, L.start.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice,
+                            h->stream));
+  HIP_TRY(h, hipMemcpyAsync(h->bin_ws.base, L.base.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(h->bin_ws.tbl, L.tbl.data(), sizeof(int) * L.tbl.size(), hipMemcpyHostToDevice,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->bin_ws.max_rows = L.max_rows;
+  h->bin_ws.ntbl = (int)L.tbl.size();
+  h->bin_ws.tbl_shift = L.tbl_shift;
+  h->bin_nbins = nb;
+  h->bin_base_bits = h->bucket_bits;
+  return SPMSPV_SUCCESS;
+}
+
 int ensure_binlen_host(spmspv_handle_t h) {
   if (h->binlen_host >= 0) return SPMSPV_SUCCESS;
   HIP_TRY(h, hipMemcpyAsync(h->h_pin, &h->ctl->bin_len, sizeof(long long), hipMemcpyDeviceToHost,
@@ -536,22 +584,20 @@ int run_kernel(spmspv_handle_t h, int kid, float* y_dense, int* y_key, float* y_
       return check_launch(h, "two-layer sort kernel");
     }
     case SPMSPV_K_COL_BUCKET: {
-      if ((rc = need_csc(h)) || (rc = need_sparse_x(h))) return rc;
+      if ((rc = need_csc(h)) || (rc = need_sparse_x(h)) || (rc = ensure_bin_layout(h))) return rc;
       if (!h->prep_valid && (rc = run_prep(h, true, nullptr))) return rc;
       int* ok = y_key ? y_key : h->yk_tmp;
       float* ov = y_val ? y_val : h->yv_tmp;
       // grids from the host's bin_len when known, else from the bound nnz
-      if (!launch_bucket(s, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl, h->xnnz, m,
-                         h->bucket_bits, h->bin_ws, h->sb.keyB, h->sb.valB, h->acc, h->marks, lb_tiles_hint(h),
-                         h->binlen_host, h->nnz))
-        return set_error(h, SPMSPV_INVALID_ARGUMENT,
-                         "bucket slot: more than " + std::to_string(kBinMax << kBinRowBits) + " rows");
+      launch_bucket(s, h->ptr, h->xbeg, h->x_val, h->split, h->csc_row, h->csc_val, h->ctl, h->xnnz, h->bin_ws,
+                    h->bin_nbins, h->sb.keyB, h->sb.valB, h->acc, h->marks, lb_tiles_hint(h), h->binlen_host,
+                    h->nnz, h->gate);
       launch_compact(s, nullptr, h->marks, m, h->acc, 1, ok, ov, h->scan_state, h->ctl, next_ep(h, 1), nullptr,
-                     Gate{}, y_dense ? nullptr : count_sig());
-      h->launches += 5;
+                     h->gate, y_dense ? nullptr : count_sig(), h->bin_ws.cursor, h->bin_nbins);
+      h->launches += 3;
       if (y_dense) {
-        launch_fill_f32(s, y_dense, m, 0.f);
-        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense);
+        launch_fill_f32(s, y_dense, m, 0.f, h->gate);
+        launch_scatter(s, ok, ov, 0, &h->ctl->ynnz, m, y_dense, false, h->gate);
       }
       return check_launch(h, "bucket kernel");
     }
@@ -750,6 +796,7 @@ int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* 
   h->row_ready = false;
   h->csr5_ready = false;
   h->mat_ready = false;
+  h->bin_base_bits = -1;
   // a CSC this handle transposed from the previous CSR no longer matches
   if (h->csc_owned) h->csc_stale = true;
   h->prep_valid = false;
@@ -774,6 +821,7 @@ int spmspv_input_csc(spmspv_handle_t h, int nnz, int* row, int* col_ptr, float* 
   h->csc_owned = false;
   h->csc_stale = false;
   h->mat_ready = false;
+  h->bin_base_bits = -1;
   h->prep_valid = false;
   h->binlen_host = -1;
   return SPMSPV_SUCCESS;
@@ -852,6 +900,7 @@ int spmspv_dealloc_pre_buffer(spmspv_handle_t h) {
   h->reserved = false;
   h->row_ready = false;
   h->mat_ready = false;
+  h->bin_base_bits = -1;
   h->prep_valid = false;
   h->have_weights = false;
   h->have_bfs_weights = false;
@@ -1178,9 +1227,9 @@ int spmspv_csc_bucket(spmspv_handle_t h, float alpha, int* ynnz, int* k, float* 
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
   if (buckets != 0) {
     int bits = 1;
-    while (bits < 8 && (1 << bits) < buckets) ++bits;
-    if (buckets < 2 || buckets > 256 || (1 << bits) != buckets)
-      return set_error(h, SPMSPV_INVALID_ARGUMENT, "buckets must be 0 or a power of two in [2, 256]");
+    while (bits < 10 && (1 << bits) < buckets) ++bits;
+    if (buckets < 2 || buckets > 1024 || (1 << bits) != buckets)
+      return set_error(h, SPMSPV_INVALID_ARGUMENT, "buckets must be 0 or a power of two in [2, 1024]");
     h->bucket_bits = bits;
   }
   return spmspv_run(h, SPMSPV_K_COL_BUCKET, alpha, nullptr, k, v, ynnz);
@@ -1302,8 +1351,8 @@ int spmspv_features(spmspv_handle_t h, float out[SPMSPV_NUM_FEATURES]) {
 // guess costs nothing more; a wrong one costs the gated launches (each exits
 // at once) and the chosen slot is queued as usual. Only slots whose launches
 // need nothing from the host about x are guessed: the atomic and sort column
-// slots (the sort then sizes its grids from the device-side bin_len) and the
-// load-balanced row slots.
+// slots (the sort then sizes its grids from the device-side bin_len), the
+// bucket slot and the load-balanced row slots.
 // slots whose launches read the column scan of x (ptr / xbeg / split)
 static bool needs_scan(int kid) {
   return kid == SPMSPV_K_COL_NAIVE_SORT || kid == SPMSPV_K_COL_LB_ATOMIC ||
@@ -1314,6 +1363,8 @@ static bool speculable(spmspv_handle_t h, int kid) {
   if (kid == SPMSPV_K_COL_NAIVE_ATOMIC || kid == SPMSPV_K_COL_LB_ATOMIC ||
       kid == SPMSPV_K_COL_NAIVE_SORT || kid == SPMSPV_K_COL_LB_SORT)
     return true;
+  // the bucket slot once its bin layout exists (building it synchronises)
+  if (kid == SPMSPV_K_COL_BUCKET) return h->bin_base_bits == h->bucket_bits;
   if (kid == SPMSPV_K_ROW_LB || kid == SPMSPV_K_SPMV_LB)
     return h->csr_row && (h->spmv_type == 1 ? h->csr5_ready : h->row_ready);
   return false;
@@ -1588,7 +1639,7 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   {
     float bbest = 1e30f;
     int bpick = h->bucket_bits;
-    for (int bb = 4; bb <= 8; ++bb) {
+    for (int bb = 8; bb <= 10; ++bb) {
       h->bucket_bits = bb;
       if ((rc = time_slot(SPMSPV_K_COL_BUCKET, &t))) return rc;
       if (better(t, bbest)) {
@@ -1601,7 +1652,8 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   // Thresholds of the single-workgroup paths, timed on sub-vectors of x
   // whose bin_len lies just under each candidate: the largest candidate at
   // which the single workgroup still beats the multi-launch path is kept.
-  // A sub-vector takes k keys evenly spaced over x (written to the handle's
+  // A sub-vector takes k keys spread over x, one from each of k equal
+  // intervals at a pseudo-random place in it (written to the handle's
   // internal x buffers), so its columns are spread like those of a sparse x
   // of that size; the first k keys of a dense x would be k adjacent columns,
   // whose contiguous reads flatter the single workgroup. When x itself lives
@@ -1675,38 +1727,73 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   // lies just under nnz / 4^k (k = 5..0), the bucket slot against the
   // load-balanced column and row slots (2, 3, 5, 7), which the rule
   // replaces; the range spans the sizes where it is more than 2 % faster
-  // than all four, its ends half-way (geometrically) to the nearest sizes
-  // where it is not.
+  // than all four. Each end is then narrowed by two bisection steps
+  // (geometric) between the last size where the bucket slot loses and the
+  // first where it wins, and set half-way between the two.
   {
     h->bucket_lo = 0;
     h->bucket_hi = -1;
-    long long prev_lose = -1, lo = -1, hi = -1;
+    // wins at a sub-vector whose bin_len is just under target: 1 yes, 0 no,
+    // -1 no such sub-vector; *blout its bin_len
+    auto wins = [&](long long target, int* out, long long* blout) -> int {
+      int kk;
+      long long bl;
+      int r;
+      if ((r = prefix(target, xn, &kk, &bl))) return r;
+      *blout = bl;
+      if (bl < target / 2 || bl <= 0) {
+        *out = -1;
+        return SPMSPV_SUCCESS;
+      }
+      float tb = 0.f, best = 1e30f;
+      if ((r = time_slot(SPMSPV_K_COL_BUCKET, &tb))) return r;
+      for (int sl : {SPMSPV_K_COL_LB_ATOMIC, SPMSPV_K_COL_LB_SORT, SPMSPV_K_ROW_LB, SPMSPV_K_SPMV_LB}) {
+        float t = 0.f;
+        if ((r = time_slot(sl, &t))) return r;
+        best = std::min(best, t);
+      }
+      *out = better(tb, best) ? 1 : 0;
+      return SPMSPV_SUCCESS;
+    };
+    // the edge between a losing size a and a winning size b (either order)
+    auto edge = [&](long long a, long long b, long long* e) -> int {
+      for (int step = 0; step < 2; ++step) {
+        const long long mid = (long long)std::sqrt((double)a * (double)b);
+        int w;
+        long long bl;
+        int r;
+        if ((r = wins(mid, &w, &bl))) return r;
+        if (w < 0) break;
+        if (w) b = bl;
+        else a = bl;
+      }
+      *e = (long long)std::sqrt((double)a * (double)b);
+      return SPMSPV_SUCCESS;
+    };
+    long long prev_lose = -1, lo = -1, hi = -1, first_win = -1, last_win = -1;
     bool done = false;
     for (int k = 5; k >= 0 && !done; --k) {
       const long long c = std::max<long long>(1, (long long)h->nnz >> (2 * k));
-      int kk;
+      int w;
       long long bl;
-      if ((rc = prefix(c, xn, &kk, &bl))) return rc;
-      if (bl < c / 2 || bl <= 0) continue;  // x holds no sub-vector near this size
-      float tb = 0.f, best = 1e30f;
-      if ((rc = time_slot(SPMSPV_K_COL_BUCKET, &tb))) return rc;
-      for (int sl : {SPMSPV_K_COL_LB_ATOMIC, SPMSPV_K_COL_LB_SORT, SPMSPV_K_ROW_LB, SPMSPV_K_SPMV_LB}) {
-        float t = 0.f;
-        if ((rc = time_slot(sl, &t))) return rc;
-        best = std::min(best, t);
-      }
-      if (better(tb, best)) {
-        if (lo < 0) lo = prev_lose < 0 ? 1 : (long long)std::sqrt((double)prev_lose * (double)bl);
-        hi = bl;
+      if ((rc = wins(c, &w, &bl))) return rc;
+      if (w < 0) continue;  // x holds no sub-vector near this size
+      if (w) {
+        if (first_win < 0) {
+          first_win = bl;
+          if (prev_lose < 0) lo = 1;
+          else if ((rc = edge(prev_lose, bl, &lo))) return rc;
+        }
+        last_win = bl;
       } else {
-        if (lo >= 0) {
-          hi = (long long)std::sqrt((double)hi * (double)bl);
+        if (first_win >= 0) {
+          if ((rc = edge(bl, last_win, &hi))) return rc;
           done = true;  // the range ends here
         }
         prev_lose = bl;
       }
     }
-    if (lo >= 0) {
+    if (first_win >= 0) {
       h->bucket_lo = lo;
       h->bucket_hi = done ? hi : LLONG_MAX;  // still ahead at the largest size: no upper end
     }

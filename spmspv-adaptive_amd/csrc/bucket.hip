@@ -3,30 +3,41 @@
 // csc_bucket_spmspv.cuh:766-1086) counts the products per row bucket, scans
 // the counts, extracts the products into their buckets and reduces each
 // bucket; its reduction is compiled out (#if 0, :999-1071) and its atomic
-// variant truncates values to int (:684). Here the buckets are row bins of
-// at most 2^15 rows, so that one bin's FP32 sums fit a 128 KB LDS window:
-//   1. k_bin_count: the load-balanced gather's merge-path tiles (one at a
-//      time per workgroup, grid-stride over at most 512 workgroups) count
-//      their products per bin in LDS, then one global atomic per (workgroup,
-//      bin) adds the counts;
-//   2. k_bin_plan (one workgroup): each bin's start (exclusive scan of the
-//      counts) and write cursor, and its accumulate chunks: a bin is cut
-//      into equal chunks of about kBinChunk products, at most kBinChunksMax
-//      (so a hub bin's partial sums meet in at most that many atomics per
-//      row); the counts are cleared for the next call;
-//   3. k_bin_scatter: the same tiles, two per workgroup round, form every
-//      product (row, a*x) again and write it into its bin, at positions the
-//      round reserves per bin with one global atomic;
-//   4. k_bin_accum: each chunk is summed per row in an LDS window over its
-//      bin's rows (LDS FP32 atomics, the window sized to the bin); a bin
-//      that is a single chunk stores its row sums into the accumulator, a
-//      longer bin adds them (one FP32 atomic per row and chunk); hit rows
-//      are marked;
-//   5. the caller's compaction turns the marks into the sorted result and
-//      clears the accumulator and the marks.
-// HBM bytes per product: 4 (count) + 8 read and 8 written (scatter) + 8
-// (accumulate), against 44 and up for gather, radix passes and reduce.
+// variant truncates values to int (:684). Here the buckets are row bins,
+// ranges of consecutive rows holding about equal numbers of the matrix's
+// nonzeros and at most 2^12 rows (one bin's sums fit a 48 KB LDS window,
+// two accumulate workgroups per CU). The layout is fixed per matrix and bin count (BinLayout, built
+// on the host from the row pointer): a bin never receives more products
+// than its rows have nonzeros, so bin b owns the product slots [base[b],
+// base[b+1]) with base[b] = nonzeros in the rows before it, and no per-call
+// count is needed; equal nonzero counts keep the bins' product counts
+// level on a power-law matrix, where equal row ranges put a tenth of them
+// into the first bin. Per call:
+//   1. k_bin_scatter: the load-balanced gather's merge-path tiles (k_prep's
+//      split), kBinTPW per workgroup round, form every product (row, a*x), find
+//      its bin (a table of row blocks, then a search among the bins that
+//      start inside the row's block) and write it into
+//      the bin at positions the round reserves per bin with one global
+//      atomic on the bin's cursor (one cursor per 128-byte line);
+//   2. k_bin_accum: every workgroup reads the cursors (products per bin) and
+//      plans the chunks itself: a bin is cut into chunks of kBinChunk
+//      products (one round of registers), so a bin that an uneven x fills
+//      far beyond its share still spreads over workgroups; one chunk per
+//      workgroup. Each chunk is summed per row in an LDS window over its
+//      bin's rows, in 64-bit fixed point scaled to the row's largest
+//      product (integer LDS atomics: FP32 LDS atomic adds are about 15
+//      times slower on gfx950), or in registers when the bin has at most
+//      kFewRows rows (a hub row's products would all queue on one LDS
+//      word); a bin that is a single chunk stores its row sums into the
+//      accumulator, a longer one adds them (one FP32 atomic per row and
+//      chunk); hit rows are marked;
+//   3. the caller's compaction turns the marks into the sorted result,
+//      clears the accumulator and the marks, and clears the cursors.
+// HBM bytes per product: 8 read and 8 written (scatter) + 8 (accumulate),
+// against 44 and up for gather, radix passes and reduce.
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "handle.hpp"
 
@@ -34,10 +45,21 @@ namespace spmspv {
 
 namespace {
 
-constexpr int kBinTPW = 2;          // merge-path tiles per scatter workgroup round
-constexpr int kBinChunksMax = 16;   // accumulate chunks per bin at most
+// merge-path tiles per scatter workgroup round: each round reserves its
+// slots with one device-scope atomic per bin it fills, and those atomics
+// (about 17 G/s on the chip, one address taking them in turn) bound the
+// kernel, so a round takes as many tiles as the registers hold
+constexpr int kBinTPW = 4;
+static_assert(kBinTPW * kLbTile < 65536 && kBinMax < 65535, "bin and rank share a word");
 constexpr int kAccNT = 1024;        // accumulate workgroup
 constexpr int kAccPPT = 16;         // products per thread per sub-round
+constexpr int kAccGrid = 2048;      // accumulate workgroups at most
+constexpr int kFewRows = 16;        // bins of at most this many rows sum in registers
+
+// LDS of the accumulate kernel for a window of rows rows and nbins bins
+constexpr size_t accum_lds_bytes(int rows, int nbins) {
+  return (sizeof(unsigned long long) + sizeof(unsigned)) * (size_t)rows + sizeof(int) * (3 * (size_t)nbins + 1);
+}
 
 // The products of merge-path tile t of the load-balanced gather (k_prep's
 // split; kLbTile merged positions: x-entry markers and products). For the
@@ -119,111 +141,59 @@ __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr,
   return np;
 }
 
-template <int NT, int VT>
-__global__ __launch_bounds__(NT) void k_bin_count(const int* __restrict__ ptr, const int* __restrict__ xbeg,
-                                                  const int* __restrict__ split, const int* __restrict__ crow,
-                                                  const Ctl* __restrict__ ctl, int xnnz, int rbits, int nbins,
-                                                  unsigned* __restrict__ hist) {
-  constexpr int T = NT * VT;
-  __shared__ int s_ptr[T + 2];
-  __shared__ int s_beg[T + 2];
-  __shared__ float s_xv[T + 2];
-  __shared__ int s_seg[T];
-  __shared__ unsigned s_h[kBinMax];
-  for (int b = threadIdx.x; b < nbins; b += NT) s_h[b] = 0u;
-  const long long ntiles = ctl->n_tiles;
-  const long long W = (long long)xnnz + ctl->bin_len;
-  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    int idx[VT];
-    float xs[VT];
-    long long b0;
-    const int np = lb_tile<NT, VT>(t, ptr, xbeg, nullptr, split, xnnz, W, s_ptr, s_beg, s_xv, s_seg, idx, xs, &b0);
-    if (np > 0) {
-      int r[VT];
-#pragma unroll
-      for (int k = 0; k < VT; ++k) r[k] = crow[idx[k]];
-#pragma unroll
-      for (int k = 0; k < VT; ++k)
-        if ((int)threadIdx.x + k * NT < np) atomicAdd(&s_h[(unsigned)r[k] >> rbits], 1u);
-    }
-    __syncthreads();  // the tables are reused by the next tile
+// The bin of row r: the last b with start[b] <= r. tbl[r >> shift] is the
+// bin holding the first row of r's block, and tbl of the next block bounds
+// the search, so a row in a block that lies inside one bin needs no step.
+__device__ __forceinline__ int bin_of(const int* s_start, const int* s_tbl, int ntbl, int shift, int nbins,
+                                      unsigned r) {
+  const int blk = (int)(r >> shift);
+  int lo = s_tbl[blk];
+  int hi = blk + 1 < ntbl ? s_tbl[blk + 1] : nbins - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((unsigned)s_start[mid] <= r) lo = mid;
+    else hi = mid - 1;
   }
-  __syncthreads();
-  for (int b = threadIdx.x; b < nbins; b += NT) {
-    const unsigned c = s_h[b];
-    if (c) atomicAdd(hist + b, c);
-  }
+  return lo;
 }
 
-// One workgroup: starts, cursors and accumulate chunks of every bin from
-// the counts, which it clears.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_bin_plan(unsigned* __restrict__ hist, int nbins, BinWs ws) {
-  constexpr int BPT = kBinMax / NT;
-  __shared__ long long s_l[NT / kWave];
-  __shared__ int s_i[NT / kWave];
-  unsigned c[BPT];
-  int nch[BPT];
-  long long csum = 0;
-  int nsum = 0;
-#pragma unroll
-  for (int j = 0; j < BPT; ++j) {
-    const int b = threadIdx.x * BPT + j;
-    c[j] = b < nbins ? hist[b] : 0u;
-    nch[j] = (int)std::min<unsigned>((c[j] + kBinChunk - 1) / kBinChunk, (unsigned)kBinChunksMax);
-    csum += c[j];
-    nsum += nch[j];
-  }
-  long long ctot;
-  int ntot;
-  long long cpre = block_excl_sum<NT, long long>(csum, s_l, &ctot);
-  int npre = block_excl_sum<NT, int>(nsum, s_i, &ntot);
-#pragma unroll
-  for (int j = 0; j < BPT; ++j) {
-    const int b = threadIdx.x * BPT + j;
-    if (b < nbins) {
-      ws.bstart[b] = cpre;
-      ws.cursor[b] = (unsigned long long)cpre;
-      ws.cstart[b] = npre;
-      hist[b] = 0u;
-    }
-    cpre += c[j];
-    npre += nch[j];
-  }
-  if (threadIdx.x == 0) {
-    ws.bstart[nbins] = ctot;
-    ws.cstart[nbins] = ntot;
-  }
-}
-
+// Dynamic LDS: the bin starts (nbins + 1), the row-block table (ntbl), the
+// round's counts and bases (nbins each).
 template <int NT, int VT>
 __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr, const int* __restrict__ xbeg,
                                                     const float* __restrict__ xval, const int* __restrict__ split,
                                                     const int* __restrict__ crow, const float* __restrict__ cval,
-                                                    const Ctl* __restrict__ ctl, int xnnz, int rbits, int nbins,
-                                                    unsigned long long* __restrict__ cursor,
-                                                    unsigned* __restrict__ okey, float* __restrict__ oval) {
+                                                    const Ctl* __restrict__ ctl, int xnnz, int nbins, BinWs ws,
+                                                    unsigned* __restrict__ okey, float* __restrict__ oval, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int T = NT * VT;
   __shared__ int s_ptr[T + 2];
   __shared__ int s_beg[T + 2];
   __shared__ float s_xv[T + 2];
   __shared__ int s_seg[T];
-  __shared__ unsigned s_c[kBinMax];
-  __shared__ unsigned long long s_base[kBinMax];
+  extern __shared__ int s_bins[];
+  const int ntbl = ws.ntbl, shift = ws.tbl_shift;
+  int* s_start = s_bins;
+  int* s_tbl = s_start + nbins + 1;
+  unsigned* s_c = reinterpret_cast<unsigned*>(s_tbl + ntbl);
+  unsigned* s_base = s_c + nbins;
+  for (int b = threadIdx.x; b <= nbins; b += NT) s_start[b] = ws.start[b];
+  for (int b = threadIdx.x; b < ntbl; b += NT) s_tbl[b] = ws.tbl[b];
   const long long ntiles = ctl->n_tiles;
   const long long W = (long long)xnnz + ctl->bin_len;
-  for (long long g = blockIdx.x; g * kBinTPW < ntiles; g += gridDim.x) {
+  for (long long rd = blockIdx.x; rd * kBinTPW < ntiles; rd += gridDim.x) {
     for (int b = threadIdx.x; b < nbins; b += NT) s_c[b] = 0u;
     __syncthreads();
-    // every product of the round in registers, with its rank in its bin
+    // every product of the round in registers, with its bin and its rank
+    // there packed into one word (bin << 16 | rank; ~0u: no product)
     unsigned row[kBinTPW][VT];
     float val[kBinTPW][VT];
-    unsigned lo[kBinTPW][VT];  // ~0u: no product
+    unsigned br[kBinTPW][VT];
 #pragma unroll
     for (int u = 0; u < kBinTPW; ++u) {
-      const long long t = g * kBinTPW + u;
+      const long long t = rd * kBinTPW + u;
 #pragma unroll
-      for (int k = 0; k < VT; ++k) lo[u][k] = ~0u;
+      for (int k = 0; k < VT; ++k) br[u][k] = ~0u;
       if (t < ntiles) {
         int idx[VT];
         float xs[VT];
@@ -239,7 +209,10 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
 #pragma unroll
           for (int k = 0; k < VT; ++k) {
             val[u][k] = a[k] * xs[k];
-            if ((int)threadIdx.x + k * NT < np) lo[u][k] = atomicAdd(&s_c[row[u][k] >> rbits], 1u);
+            if ((int)threadIdx.x + k * NT < np) {
+              const int b = bin_of(s_start, s_tbl, ntbl, shift, nbins, row[u][k]);
+              br[u][k] = ((unsigned)b << 16) | atomicAdd(&s_c[b], 1u);
+            }
           }
         }
       }
@@ -247,15 +220,15 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
     }
     for (int b = threadIdx.x; b < nbins; b += NT) {
       const unsigned c = s_c[b];
-      if (c) s_base[b] = atomicAdd(cursor + b, (unsigned long long)c);
+      if (c) s_base[b] = (unsigned)ws.base[b] + atomicAdd(ws.cursor + (long long)b * kBinCurStride, c);
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kBinTPW; ++u) {
 #pragma unroll
       for (int k = 0; k < VT; ++k) {
-        if (lo[u][k] != ~0u) {
-          const unsigned long long p = s_base[row[u][k] >> rbits] + lo[u][k];
+        if (br[u][k] != ~0u) {
+          const unsigned p = s_base[br[u][k] >> 16] + (br[u][k] & 0xFFFFu);
           okey[p] = row[u][k];
           oval[p] = val[u][k];
         }
@@ -265,123 +238,287 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
   }
 }
 
+// Fixed-point row sums. A product v = m * 2^(ef - 150), with m its 24-bit
+// significand and ef = max(exponent field, 1), enters row r's sum as the
+// integer q = m * 2^(ef - E + 22) (rounded to nearest when the shift is to
+// the right), where E is the largest ef among the products of r in the
+// chunk: |q| < 2^46, so a chunk of at most 2^14 products sums exactly in 64
+// bits, in any order, and the row's value is the sum times 2^(E - 172),
+// rounded once to FP32. Each product's error is at most 2^-46 of the row's
+// largest |product|. Integer LDS atomics (max for E, add for the sum) run
+// about 15 times faster on gfx950 than FP32 LDS atomic adds, which were
+// the kernel's limit.
+__device__ __forceinline__ unsigned prod_ef(float v) {
+  const unsigned e = (__float_as_uint(v) >> 23) & 0xFFu;
+  return e ? e : 1u;
+}
+
+__device__ __forceinline__ long long prod_fixed(float v, unsigned E) {
+  const unsigned b = __float_as_uint(v);
+  const unsigned ef = (b >> 23) & 0xFFu;
+  const unsigned long long m = (b & 0x7FFFFFu) | (ef ? 0x800000u : 0u);
+  const int d = (int)E - (int)(ef ? ef : 1u);
+  unsigned long long q;
+  if (d <= 22) q = m << (22 - d);
+  else if (d - 22 <= 24) q = (m + (1ull << (d - 23))) >> (d - 22);
+  else q = 0;
+  return (b >> 31) ? -(long long)q : (long long)q;
+}
+
+__device__ __forceinline__ float fixed_value(long long sum, unsigned E) {
+  return (float)ldexp((double)sum, (int)E - 172);
+}
+
+// Dynamic LDS: the window (per row of the largest bin: the 64-bit fixed-
+// point sum, then the exponent E, 0 while the row has no product), then
+// the plan: products, first slot and first chunk of every bin.
 template <int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_bin_accum(const unsigned* __restrict__ key, const float* __restrict__ val,
-                                                  BinWs ws, int nbins, int rbits, float* acc,
-                                                  unsigned char* marks) {
+                                                  BinWs ws, int nbins, float* acc, unsigned char* marks, Gate g) {
+  if (gate_shut(g)) return;
   constexpr int SUB = NT * PPT;
+  static_assert(SUB == kBinChunk, "a chunk is one round of registers");
   constexpr int BPT = kBinMax / NT;
-  // the window: 2^rbits FP32 sums, then their hit bits
-  extern __shared__ float s_dyn[];
-  float* s_acc = s_dyn;
-  const int R = 1 << rbits;
-  const int RW = (R + 31) / 32;
-  unsigned* s_hit = reinterpret_cast<unsigned*>(s_dyn + R);
-  __shared__ int s_b;
-  for (int r = threadIdx.x; r < R; r += NT) s_acc[r] = 0.f;
-  for (int w = threadIdx.x; w < RW; w += NT) s_hit[w] = 0u;
-  const int nch = ws.cstart[nbins];
-  for (int c = blockIdx.x; c < nch; c += gridDim.x) {
-    // the bin holding chunk c: the b with cstart[b] <= c < cstart[b + 1]
+  constexpr int NW = NT / kWave;
+  extern __shared__ unsigned long long s_q[];
+  const int R = ws.max_rows;
+  unsigned* s_e = reinterpret_cast<unsigned*>(s_q + R);
+  int* s_cnt = reinterpret_cast<int*>(s_e + R);
+  int* s_base = s_cnt + nbins;
+  int* s_cst = s_base + nbins;
+  __shared__ int s_i[NW];
+  __shared__ float s_few[NW][kFewRows];  // per-wave sums of a bin of few rows
+  __shared__ unsigned s_fewhit[NW];
+  // the plan, the same in every workgroup: a bin is cut into chunks of
+  // kBinChunk products (the last one shorter)
+  {
+    unsigned c[BPT];
+    int bb[BPT];
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
       const int b = threadIdx.x * BPT + j;
-      if (b < nbins && ws.cstart[b] <= c && c < ws.cstart[b + 1]) s_b = b;
+      c[j] = b < nbins ? ws.cursor[(long long)b * kBinCurStride] : 0u;
+      bb[j] = b < nbins ? ws.base[b] : 0;
     }
-    __syncthreads();
-    const int b = s_b;
-    const int c0 = ws.cstart[b], nb = ws.cstart[b + 1] - c0;
-    const bool owned = nb == 1;
-    // chunk c - c0 of nb equal parts of the bin
-    const long long bs = ws.bstart[b], size = ws.bstart[b + 1] - bs;
-    const long long p0 = bs + size * (c - c0) / nb;
-    const long long pe = bs + size * (c - c0 + 1) / nb;
-    const unsigned rb = (unsigned)b << rbits;
-    for (long long q0 = p0; q0 < pe; q0 += SUB) {
-      unsigned kk[PPT];
-      float vv[PPT];
+    int nsum = 0;
+    int nch[BPT];
 #pragma unroll
-      for (int j = 0; j < PPT; ++j) {
-        const long long i = q0 + threadIdx.x + (long long)j * NT;
-        const long long ic = i < pe ? i : p0;
-        kk[j] = __builtin_nontemporal_load(key + ic);
-        vv[j] = __builtin_nontemporal_load(val + ic);
+    for (int j = 0; j < BPT; ++j) {
+      nch[j] = (int)((c[j] + kBinChunk - 1) / kBinChunk);
+      nsum += nch[j];
+    }
+    int ntot;
+    int npre = block_excl_sum<NT, int>(nsum, s_i, &ntot);
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      const int b = threadIdx.x * BPT + j;
+      if (b < nbins) {
+        s_cnt[b] = (int)c[j];
+        s_base[b] = bb[j];
+        s_cst[b] = npre;
       }
+      npre += nch[j];
+    }
+    if (threadIdx.x == 0) s_cst[nbins] = ntot;
+  }
+  for (int r = threadIdx.x; r < R; r += NT) {
+    s_q[r] = 0ull;
+    s_e[r] = 0u;
+  }
+  __syncthreads();
+  const int nch = s_cst[nbins];
+  for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+    // the bin holding chunk c: the last b with s_cst[b] <= c
+    int lo = 0, hi = nbins - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_cst[mid] <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    const int b = lo;
+    const int c0 = s_cst[b], nb = s_cst[b + 1] - c0;
+    const bool owned = nb == 1;
+    // chunk c - c0 of nb parts of at most kBinChunk products; rows [rs, rs + nr)
+    const long long p0 = (long long)s_base[b] + (long long)(c - c0) * kBinChunk;
+    const long long pe = std::min<long long>(p0 + kBinChunk, (long long)s_base[b] + s_cnt[b]);
+    const unsigned rs = (unsigned)ws.start[b];
+    const int nr = ws.start[b + 1] - (int)rs;
+    unsigned kk[PPT];
+    float vv[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const long long i = p0 + threadIdx.x + (long long)j * NT;
+      const long long ic = i < pe ? i : p0;
+      kk[j] = __builtin_nontemporal_load(key + ic);
+      vv[j] = __builtin_nontemporal_load(val + ic);
+    }
+    if (nr <= kFewRows) {
+      // a bin of few (hub) rows: the lanes of a wave would all add into the
+      // same LDS words, so each thread sums per row in registers and the
+      // workgroup reduces the kFewRows sums
+      float part[kFewRows];
+      unsigned hit = 0u;
+#pragma unroll
+      for (int q = 0; q < kFewRows; ++q) part[q] = 0.f;
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
-        if (q0 + threadIdx.x + (long long)j * NT < pe) {
-          const unsigned r = kk[j] - rb;
-          atomicAdd(&s_acc[r], vv[j]);
-          atomicOr(&s_hit[r >> 5], 1u << (r & 31));
+        if (p0 + threadIdx.x + (long long)j * NT < pe) {
+          const unsigned r = kk[j] - rs;
+          hit |= 1u << r;
+#pragma unroll
+          for (int q = 0; q < kFewRows; ++q) part[q] += r == (unsigned)q ? vv[j] : 0.f;
         }
       }
+#pragma unroll
+      for (int q = 0; q < kFewRows; ++q) part[q] = wave_sum(part[q]);
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, kWave);
+      const int w = threadIdx.x / kWave;
+      if (lane() == 0) {
+#pragma unroll
+        for (int q = 0; q < kFewRows; ++q) s_few[w][q] = part[q];
+        s_fewhit[w] = hit;
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < nr) {
+        const int q = threadIdx.x;
+        float v = 0.f;
+        unsigned h = 0u;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          v += s_few[k][q];
+          h |= s_fewhit[k];
+        }
+        if ((h >> q) & 1u) {
+          if (owned) acc[rs + q] = v;  // the accumulator holds 0 there
+          else atomic_add_f32(acc + rs + q, v);
+          marks[rs + q] = 1;
+        }
+      }
+      __syncthreads();  // s_few is reused by the next chunk
+      continue;
+    }
+    // every row's largest exponent first; a chunk with an Inf or NaN
+    // product sums in FP32 instead (the window's sums read as floats)
+    int special = 0;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      if (p0 + threadIdx.x + (long long)j * NT < pe) {
+        const unsigned ef = prod_ef(vv[j]);
+        special |= ef == 0xFFu;
+        atomicMax(&s_e[kk[j] - rs], ef);
+      }
+    }
+    special = __syncthreads_or(special);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      if (p0 + threadIdx.x + (long long)j * NT < pe) {
+        const unsigned r = kk[j] - rs;
+        if (special) atomicAdd(reinterpret_cast<float*>(s_q + r), vv[j]);
+        else atomicAdd(s_q + r, (unsigned long long)prod_fixed(vv[j], s_e[r]));
+      }
     }
     __syncthreads();
-    // the window's rows into the accumulator; the window is left zeroed
-    for (int w = threadIdx.x; w < RW; w += NT) {
-      unsigned bits = s_hit[w];
-      if (!bits) continue;
-      s_hit[w] = 0u;
-      while (bits) {
-        const int bit = __ffs(bits) - 1;
-        bits &= bits - 1;
-        const unsigned lr = ((unsigned)w << 5) + bit;
-        const float v = s_acc[lr];
-        s_acc[lr] = 0.f;
-        if (owned) acc[rb + lr] = v;  // the accumulator holds 0 there
-        else atomic_add_f32(acc + rb + lr, v);
-        marks[rb + lr] = 1;
+    // the window's hit rows into the accumulator, consecutive rows on
+    // consecutive threads; the window returns to zero
+    for (int r = threadIdx.x; r < nr; r += NT) {
+      const unsigned E = s_e[r];
+      if (E) {
+        const float v = special ? *reinterpret_cast<const float*>(s_q + r) : fixed_value((long long)s_q[r], E);
+        s_q[r] = 0ull;
+        s_e[r] = 0u;
+        if (owned) acc[rs + r] = v;  // the accumulator holds 0 there
+        else atomic_add_f32(acc + rs + r, v);
+        marks[rs + r] = 1;
       }
     }
     __syncthreads();
   }
 }
 
-}  // namespace
-
-// Bin geometry for m rows: at least 2^bucket_bits bins and at most 2^15 rows
-// per bin; *rbits = log2(rows per bin). Returns the number of bins, or -1
-// when m needs more than kBinMax bins.
-int bin_geometry(int m, int bucket_bits, int* rbits) {
-  int bits = 0;
-  while (bits < 31 && (1ll << bits) < (long long)m) ++bits;
-  int rb = bits > bucket_bits ? bits - bucket_bits : 0;
-  if (rb > kBinRowBits) rb = kBinRowBits;
-  const long long nb = m > 0 ? (((long long)m - 1) >> rb) + 1 : 1;
-  *rbits = rb;
-  return nb > kBinMax ? -1 : (int)nb;
+// Nonzeros per row from the CSC row indices (a CSC-only matrix), added into
+// cnt (m ints, zero before).
+__global__ void k_row_counts(const int* __restrict__ crow, long long nnz, int* __restrict__ cnt) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride)
+    atomicAdd(cnt + crow[i], 1);
 }
 
-bool launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* xval, const int* split,
-                   const int* crow, const float* cval, const Ctl* ctl, int xnnz, int m, int bucket_bits,
-                   const BinWs& ws, unsigned* bkey, float* bval, float* acc, unsigned char* marks,
-                   long long tiles_hint, long long binlen_hint, long long binlen_cap) {
-  int rbits = 0;
-  const int nbins = bin_geometry(m, bucket_bits, &rbits);
-  if (nbins < 0) return false;
-  const long long tiles = tiles_hint > 0 ? tiles_hint : ((long long)xnnz + binlen_cap) / kLbTile + 2;
-  // a few tiles per workgroup: each workgroup ends with one atomic per bin
-  // it counted, on addresses every workgroup shares
-  const int cgrid = (int)std::max<long long>(1, std::min<long long>(512, tiles));
-  const int rgrid = (int)std::max<long long>(1, std::min<long long>(2048, (tiles + kBinTPW - 1) / kBinTPW));
-  hipLaunchKernelGGL((k_bin_count<kLbNT, kLbVT>), dim3(cgrid), dim3(kLbNT), 0, s, ptr, xbeg, split, crow, ctl, xnnz,
-                     rbits, nbins, ws.hist);
-  hipLaunchKernelGGL((k_bin_plan<1024>), dim3(1), dim3(1024), 0, s, ws.hist, nbins, ws);
-  hipLaunchKernelGGL((k_bin_scatter<kLbNT, kLbVT>), dim3(rgrid), dim3(kLbNT), 0, s, ptr, xbeg, xval, split, crow,
-                     cval, ctl, xnnz, rbits, nbins, ws.cursor, bkey, bval);
-  const long long bl = binlen_hint >= 0 ? binlen_hint : binlen_cap;
-  const int agrid = (int)std::max<long long>(1, std::min<long long>(4096, bl / kBinChunk + nbins));
-  const size_t lds = sizeof(float) * ((size_t)(1 << rbits) + ((1 << rbits) + 31) / 32);
-  // a window of 2^15 rows takes 132 KB of the 160 KB of LDS
+}  // namespace
+
+int bin_layout(const std::vector<int>& row_ptr, int m, int bucket_bits, BinLayout* out) {
+  out->start.clear();
+  out->base.clear();
+  out->tbl.clear();
+  const long long nnz = m > 0 ? row_ptr[m] : 0;
+  const long long want = std::max<long long>(1, (nnz + (1ll << bucket_bits) - 1) >> bucket_bits);
+  const int cap = 1 << kBinRowBits;
+  int mr = 0;
+  int r = 0;
+  out->start.push_back(0);
+  out->base.push_back(0);
+  while (r < m) {
+    // the bin grows row by row while it stays within its share of nonzeros
+    // (a row beyond the share on its own still makes a bin) and the cap
+    int e = r + 1;
+    const long long b0 = row_ptr[r];
+    while (e < m && e - r < cap && row_ptr[e + 1] - b0 <= want) ++e;
+    mr = std::max(mr, e - r);
+    out->start.push_back(e);
+    out->base.push_back(row_ptr[e]);
+    r = e;
+    if ((int)out->start.size() - 1 > kBinMax) return -1;
+  }
+  if (m == 0) {  // one empty bin
+    out->start.push_back(0);
+    out->base.push_back(0);
+  }
+  const int nb = (int)out->start.size() - 1;
+  out->max_rows = std::max(mr, 1);
+  // the row-block table: blocks of 2^shift rows, at most kBinTbl of them
+  int shift = 0;
+  while (((long long)std::max(m, 1) + (1ll << shift) - 1) >> shift > kBinTbl) ++shift;
+  out->tbl_shift = shift;
+  const int ntbl = (int)(((long long)std::max(m, 1) + (1ll << shift) - 1) >> shift);
+  int b = 0;
+  for (int k = 0; k < ntbl; ++k) {
+    const long long first = (long long)k << shift;
+    while (b + 1 < nb && out->start[b + 1] <= first) ++b;
+    out->tbl.push_back(b);
+  }
+  return nb;
+}
+
+void launch_row_counts(hipStream_t s, const int* crow, long long nnz, int* cnt) {
+  const int grid = (int)std::max<long long>(1, std::min<long long>(4096, (nnz + 255) / 256));
+  hipLaunchKernelGGL(k_row_counts, dim3(grid), dim3(256), 0, s, crow, nnz, cnt);
+}
+
+void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* xval, const int* split,
+                   const int* crow, const float* cval, const Ctl* ctl, int xnnz, const BinWs& ws, int nbins,
+                   unsigned* bkey, float* bval, float* acc, unsigned char* marks, long long tiles_hint,
+                   long long binlen_hint, long long binlen_cap, Gate g) {
+  // dynamic LDS beyond 64 KB (many bins, wide windows) needs the attribute
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_scatter<kLbNT, kLbVT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(sizeof(int) * (3 * (size_t)kBinMax + 1 + kBinTbl)));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_accum<kAccNT, kAccPPT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(sizeof(float) * ((1 << kBinRowBits) + (1 << kBinRowBits) / 32)));
+                              (int)accum_lds_bytes(1 << kBinRowBits, kBinMax));
   });
-  hipLaunchKernelGGL((k_bin_accum<kAccNT, kAccPPT>), dim3(agrid), dim3(kAccNT), lds, s, bkey, bval, ws, nbins,
-                     rbits, acc, marks);
-  return true;
+  const long long tiles = tiles_hint > 0 ? tiles_hint : ((long long)xnnz + binlen_cap) / kLbTile + 2;
+  const int rgrid = (int)std::max<long long>(1, std::min<long long>(2048, (tiles + kBinTPW - 1) / kBinTPW));
+  const size_t slds = sizeof(int) * (3 * (size_t)nbins + 1 + ws.ntbl);
+  hipLaunchKernelGGL((k_bin_scatter<kLbNT, kLbVT>), dim3(rgrid), dim3(kLbNT), slds, s, ptr, xbeg, xval, split, crow,
+                     cval, ctl, xnnz, nbins, ws, bkey, bval, g);
+  // one chunk per workgroup where the grid allows; with bin_len unknown on
+  // the host, two per bin on average (the workgroups loop over the rest)
+  const long long extra = binlen_hint >= 0 ? std::min(binlen_hint, binlen_cap) / kBinChunk : nbins;
+  const int agrid = (int)std::max<long long>(1, std::min<long long>(kAccGrid, nbins + extra));
+  hipLaunchKernelGGL((k_bin_accum<kAccNT, kAccPPT>), dim3(agrid), dim3(kAccNT), accum_lds_bytes(ws.max_rows, nbins),
+                     s, bkey, bval, ws, nbins, acc, marks, g);
 }
 
 }  // namespace spmspv

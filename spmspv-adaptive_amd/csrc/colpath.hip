@@ -955,7 +955,9 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
                                                 int* __restrict__ ykey, float* __restrict__ yval,
                                                 unsigned long long* state, Ctl* ctl, Ep ep,
                                                 const int* __restrict__ keymap, Gate g,
-                                                long long* sig) {
+                                                long long* sig, unsigned* __restrict__ clr, int nclr) {
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < nclr; i += NT) clr[(long long)i * kBinCurStride] = 0u;
   if (gate_shut(g)) return;
   constexpr int kRows = 8, kTile = NT * kRows;
   static_assert(kTile == kCmpNT * 32, "a tile is kCmpNT bit-array words");
@@ -1078,15 +1080,15 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
 
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep, const int* keymap, Gate g, long long* sig) {
+                    Ep ep, const int* keymap, Gate g, long long* sig, unsigned* clr, int nclr) {
   constexpr int nt = kCmpNT * 4;
   const int grid = std::max(1, (int)(((long long)m + kCmpNT * 32 - 1) / (kCmpNT * 32)));
   if (marks)
     hipLaunchKernelGGL((k_compact<nt, true>), dim3(grid), dim3(nt), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep, keymap, g, sig);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap, g, sig, clr, nclr);
   else
     hipLaunchKernelGGL((k_compact<nt, false>), dim3(grid), dim3(nt), 0, s, bitmap, marks, m, acc,
-                       reset_acc, ykey, yval, state, ctl, ep, keymap, g, sig);
+                       reset_acc, ykey, yval, state, ctl, ep, keymap, g, sig, clr, nclr);
 }
 
 // ------------------------------------------------------------ utilities

@@ -69,7 +69,9 @@ struct spmspv_handle_s {
   unsigned* bitmap = nullptr;  // ceil(m/32), all-zero between calls
   unsigned char* marks = nullptr;  // one byte per row (padded to 32), all-zero between calls
   spmspv::SortBufs sb{};
-  spmspv::BinWs bin_ws{};  // bucket slot's per-bin counts, starts, cursors and chunks
+  spmspv::BinWs bin_ws{};  // bucket slot's per-bin cursors, bases and done counter
+  int bin_base_bits = -1;  // bucket_bits the bin layout was built for (-1: none)
+  int bin_nbins = 0;       // bins of that layout
   int* yk_tmp = nullptr;     // m
   float* yv_tmp = nullptr;   // m
   float* x_dense_int = nullptr;  // n
@@ -105,7 +107,7 @@ struct spmspv_handle_s {
   int team_col = 16;
   int team_row = 8;
   int row_vt = 7;  // row-merge items per thread (autotuned)
-  int bucket_bits = 8;  // bucket slot: 2^bucket_bits row buckets (autotuned, or set by csc_bucket)
+  int bucket_bits = 9;  // bucket slot: about 2^bucket_bits row bins of equal nonzeros (autotuned, or set by csc_bucket)
   // bin_len range in which auto mode runs the bucket slot in place of a
   // load-balanced column or row slot (autotuned; off while hi < lo)
   long long bucket_lo = 0, bucket_hi = -1;
@@ -204,4 +206,6 @@ int wait_signal(spmspv_handle_t h, int idx, long long* out);
 Ep next_ep(spmspv_handle_t h, unsigned k);
 // Reads bin_len of the current x back to the host (one synchronisation).
 int ensure_binlen_host(spmspv_handle_t h);
+// Builds the bucket slot's bin layout for the current matrix and bucket_bits.
+int ensure_bin_layout(spmspv_handle_t h);
 }  // namespace spmspv

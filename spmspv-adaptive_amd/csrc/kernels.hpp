@@ -3,6 +3,8 @@
 // tile counts) are read by the kernels from the control block.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -138,9 +140,12 @@ void launch_tl_gather(hipStream_t s, int mode, const int* ptr, const int* xbeg, 
 // (keymap ascending keeps the keys sorted). With sig (may be host-mapped),
 // the result count is also written to *sig once every workgroup's stores
 // are visible, as launch_signal_int after this launch would.
+// clr (nullable): nclr words at a stride of kBinCurStride set to 0 (the
+// bucket slot's cursors, once its accumulate kernel has read them).
 void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m, float* acc,
                     int reset_acc, int* ykey, float* yval, unsigned long long* state, Ctl* ctl,
-                    Ep ep, const int* keymap = nullptr, Gate g = {}, long long* sig = nullptr);
+                    Ep ep, const int* keymap = nullptr, Gate g = {}, long long* sig = nullptr,
+                    unsigned* clr = nullptr, int nclr = 0);
 // bitmap of (v != 0) over a dense vector (whole words, no atomics)
 void launch_bitmap_from_dense(hipStream_t s, const float* v, int n, unsigned* bits);
 void launch_fill_f32(hipStream_t s, float* p, long long n, float v, Gate g = {});
@@ -205,25 +210,40 @@ int launch_radix_sort(hipStream_t s, const SortBufs& b, int range, const long lo
                       long long n_hint, long long n_cap, bool skip_small, Ep ep, Gate g = {},
                       int small_cap = kSmallSortCap);
 
-// Bucket slot: row bins of at most 2^kBinRowBits rows (at least
-// 2^bucket_bits bins, at most kBinMax); products counted per bin, scattered
-// into their bins (bkey/bval, bin_len capacity) and summed per bin in LDS
-// windows into acc (marks set); the caller's compaction gives the sorted
-// result and clears acc and marks. Needs k_prep's scan and splits. The
-// workspace's counts are zero between calls (the plan kernel clears them).
-// Returns false (nothing launched) when m needs more than kBinMax bins.
-constexpr int kBinRowBits = 15, kBinMax = 2048, kBinChunk = 16384;
-struct BinWs {
-  unsigned* hist;              // kBinMax counts
-  unsigned long long* cursor;  // kBinMax write positions
-  long long* bstart;           // kBinMax + 1 bin starts
-  int* cstart;                 // kBinMax + 1 first accumulate chunk per bin
+// Bucket slot: row bins, ranges of consecutive rows with about equal
+// numbers of the matrix's nonzeros (about nnz / 2^bucket_bits each) and at
+// most 2^kBinRowBits rows, at most kBinMax of them. Bin b owns the product
+// slots [base[b], base[b+1]) of bkey/bval (nnz capacity): the nonzeros of
+// its rows, an upper bound on its products. bin_layout computes the layout
+// on the host from the row pointer (returns the bin count, or -1 when more
+// than kBinMax bins would be needed). Per call the products are scattered
+// into their bins and summed per bin in LDS windows into acc (marks set);
+// the caller's compaction gives the sorted result and clears acc, marks
+// and the cursors (one per 128-byte line; its clr argument). Needs k_prep's
+// scan and splits.
+constexpr int kBinRowBits = 12, kBinMax = 6144, kBinChunk = 16384, kBinCurStride = 32, kBinTbl = 2048;
+struct BinLayout {
+  std::vector<int> start;  // first row of each bin, then m
+  std::vector<int> base;   // first product slot of each bin, then nnz
+  std::vector<int> tbl;    // bin of the first row of each block of 2^tbl_shift rows
+  int tbl_shift = 0;
+  int max_rows = 1;        // rows of the largest bin
 };
-int bin_geometry(int m, int bucket_bits, int* rbits);
-bool launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* xval, const int* split,
-                   const int* crow, const float* cval, const Ctl* ctl, int xnnz, int m, int bucket_bits,
-                   const BinWs& ws, unsigned* bkey, float* bval, float* acc, unsigned char* marks,
-                   long long tiles_hint, long long binlen_hint, long long binlen_cap);
+struct BinWs {
+  unsigned* cursor;  // kBinMax * kBinCurStride: products reserved per bin (word b * kBinCurStride)
+  int* base;         // kBinMax + 1
+  int* start;        // kBinMax + 1
+  int* tbl;          // kBinTbl
+  int ntbl = 1, tbl_shift = 0;
+  int max_rows = 1;
+};
+int bin_layout(const std::vector<int>& row_ptr, int m, int bucket_bits, BinLayout* out);
+// cnt[crow[i]] += 1 for every nonzero (nonzeros per row of a CSC matrix)
+void launch_row_counts(hipStream_t s, const int* crow, long long nnz, int* cnt);
+void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* xval, const int* split,
+                   const int* crow, const float* cval, const Ctl* ctl, int xnnz, const BinWs& ws, int nbins,
+                   unsigned* bkey, float* bval, float* acc, unsigned char* marks, long long tiles_hint,
+                   long long binlen_hint, long long binlen_cap, Gate g = {});
 
 // --- row path ------------------------------------------------------------
 // --- CSR5 for wave64 (spmv_type 1) ----------------------------------------

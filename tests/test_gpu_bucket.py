@@ -44,15 +44,16 @@ def bucket(M, key, val, buckets=0):
 
 @pytest.fixture(scope="module")
 def rmat16(gpu):
-    """R-MAT scale 16 (65,536 rows): with 256 bins of 256 rows the hub bins
-    hold many 16,384-product chunks and the tail bins one chunk each."""
+    """R-MAT scale 16 (65,536 rows, 1 M nonzeros): bins of equal nonzero
+    counts are a few rows long at the hub rows and thousands of rows long in
+    the tail."""
     rp, col, val = sa.gen_rmat_csr(16, 16, seed=31)
     n = 1 << 16
     M = sa.Matrix(n, n, rp, col, val)
     return M, oracle.csr_to_csc(n, n, rp, col, val)
 
 
-@pytest.mark.parametrize("buckets", [2, 16, 256])
+@pytest.mark.parametrize("buckets", [2, 16, 256, 1024])
 @pytest.mark.parametrize("density", [1e-3, 0.05, 1.0])
 def test_bins_and_chunks(rmat16, buckets, density):
     M, (cp, ri, cv) = rmat16
@@ -64,13 +65,15 @@ def test_bins_and_chunks(rmat16, buckets, density):
 
 
 def test_many_rows_per_bin_limit(gpu):
-    """2^20 rows: at most 2^15 rows per bin gives 32 bins at least, whatever
-    the requested count (2 here); one hub row collects 60,000 products."""
+    """2^20 rows: at most 2^13 rows per bin gives 128 bins at least, whatever
+    the requested count (2 here); one hub row of 150,000 nonzeros is a bin of
+    its own, which a dense x fills with 150,000 products: three chunks whose
+    row sums meet in FP32 atomics."""
     import scipy.sparse as sp
     rng = np.random.default_rng(12)
     m = n = 1 << 20
-    rows = np.concatenate([rng.integers(0, m, 400000), np.full(60000, 777)])
-    cols = np.concatenate([rng.integers(0, n, 400000), rng.choice(n, 60000, replace=False)])
+    rows = np.concatenate([rng.integers(0, m, 400000), np.full(150000, 777)])
+    cols = np.concatenate([rng.integers(0, n, 400000), rng.choice(n, 150000, replace=False)])
     vals = rng.uniform(-1, 1, rows.size).astype(np.float32)
     A = sp.csr_matrix((vals, (rows, cols)), shape=(m, n))
     A.sum_duplicates()
@@ -86,6 +89,25 @@ def test_many_rows_per_bin_limit(gpu):
             check(m, cp, ri, cv, key, val, yk, yv, f"2^20 rows xnnz {k} bins {b}")
 
 
+def test_zero_products_mark_rows(rmat16):
+    """x values of +0.0 and -0.0 (products of either sign of zero) still put
+    their rows into the result (structural support), both in bins of a few
+    hub rows (summed in registers) and in long bins (summed in the LDS
+    window, where a row's sum starts at -0.0)."""
+    M, (cp, ri, cv) = rmat16
+    for it, (k, mode) in enumerate([(3000, "mixed"), (65536, "mixed"), (20000, "all -0.0"), (65536, "all +0.0")]):
+        key, val = sa.gen_sparse_vector(M.n, k, 8100 + it)
+        val = val.copy()
+        if mode == "mixed":
+            val[::2] = 0.0
+            val[1::4] = -0.0
+        else:
+            val[:] = -0.0 if mode == "all -0.0" else 0.0
+        say(f"bucket slot, zero x values ({mode}), xnnz {k}")
+        yk, yv = bucket(M, key, val, 256)
+        check(M.m, cp, ri, cv, key, val, yk, yv, f"zero x values {mode} xnnz {k}")
+
+
 def test_repeat_calls_leave_counts_clear(rmat16):
     """The per-bin counts, the accumulator and the marks return to zero
     after every call: alternating sizes give the oracle's result each time."""
@@ -94,6 +116,36 @@ def test_repeat_calls_leave_counts_clear(rmat16):
         key, val = sa.gen_sparse_vector(M.n, k, 5000 + it)
         yk, yv = bucket(M, key, val)
         check(M.m, cp, ri, cv, key, val, yk, yv, f"repeat {it}")
+
+
+def test_matrix_change_moves_bins(rmat16):
+    """A new CSR with the same nnz (the rows in reverse order, so that the
+    hub rows fall into the last bins) re-derives every bin's product slots:
+    alternating the two matrices gives the oracle's result on each."""
+    M, (cp, ri, cv) = rmat16
+    m = M.m
+    rp, col, val = M.h_row_ptr, M.h_col, M.h_val
+    lens = np.diff(rp)[::-1]
+    rp2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    # entries of old row m-1-i, in order, become row i
+    starts = rp[:-1][::-1]
+    order = np.repeat(starts - rp2[:-1], lens) + np.arange(M.nnz)
+    col2, val2 = col[order], val[order]
+    ref2 = oracle.csr_to_csc(m, M.n, rp2, col2, val2)
+    d2 = (sa.DeviceArray.from_host(rp2, np.int32), sa.DeviceArray.from_host(col2, np.int32),
+          sa.DeviceArray.from_host(val2, np.float32))
+    d1 = (M.d_row_ptr, M.d_col, M.d_val)
+    try:
+        for it, (d, ref) in enumerate([(d2, ref2), (d1, (cp, ri, cv)), (d2, ref2)]):
+            M.A.InputCSR(M.nnz, *d)
+            M.A.ToCSC()
+            key, xv = sa.gen_sparse_vector(M.n, 20000, 300 + it)
+            say(f"bucket slot after a matrix change, step {it}")
+            yk, yv = bucket(M, key, xv, 256)
+            check(m, *ref, key, xv, yk, yv, f"matrix switch {it}")
+    finally:
+        M.A.InputCSR(M.nnz, *d1)
+        M.A.ToCSC()
 
 
 def test_rule_routes_auto_calls(rmat16):
@@ -132,6 +184,58 @@ def test_rule_routes_auto_calls(rmat16):
             c, used = A.run_auto(y_key=yk, y_val=yv)
             assert used != sa.K_COL_BUCKET
             check(M.m, cp, ri, cv, key, val, yk.to_host(c), yv.to_host(c), f"rule off xnnz {k}")
+    finally:
+        A.set_bucket_range(0, -1)
+        A.set_small_x_path(True)
+
+
+def test_speculated_bucket_calls(rmat16):
+    """Auto calls that the rule sends to the bucket slot are queued behind
+    the selector once their size class has chosen it: repeated calls guess
+    right, a call whose choice changed (rule switched off) guesses wrong and
+    runs the chosen slot, a dense output also works, and every result is the
+    oracle's."""
+    M, (cp, ri, cv) = rmat16
+    A = M.A
+    yk = sa.DeviceArray(M.m, np.int32)
+    yv = sa.DeviceArray(M.m, np.float32)
+    yd = sa.DeviceArray(M.m, np.float32)
+    k = 30000
+    key, val = sa.gen_sparse_vector(M.n, k, 9100)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(val, np.float32)
+    try:
+        A.set_small_x_path(False)
+        A.selector_mode(1)
+        A.set_sparse_x(dk, dv, k)
+        plain = A.select()
+        if plain not in (2, 3, 5, 7):
+            pytest.skip(f"the rule fallback picks slot {plain}, which the bucket rule leaves alone")
+        A.set_bucket_range(0, 1 << 62)
+        h0, m0 = A.speculation_stats()
+        for it in range(4):
+            A.set_sparse_x(dk, dv, k)
+            c, used = A.run_auto(y_key=yk, y_val=yv)
+            say(f"speculated bucket call {it}: ran {used}")
+            assert used == sa.K_COL_BUCKET
+            check(M.m, cp, ri, cv, key, val, yk.to_host(c), yv.to_host(c), f"speculated call {it}")
+        h1, m1 = A.speculation_stats()
+        assert h1 - h0 >= 3, (h0, h1)
+        # a dense result through the speculated bucket slot
+        A.set_sparse_x(dk, dv, k)
+        c, used = A.run_auto(y_dense=yd)
+        assert used == sa.K_COL_BUCKET
+        ek, ev, eb = oracle.spmspv_f64(M.m, cp, ri, cv, key, val)
+        dense = yd.to_host(M.m)
+        assert np.count_nonzero(np.delete(dense, ek)) == 0
+        assert np.all(np.abs(dense[ek].astype(np.float64) - ev) <= TOL * eb + 1e-30)
+        A.set_bucket_range(0, -1)
+        A.set_sparse_x(dk, dv, k)
+        c, used = A.run_auto(y_key=yk, y_val=yv)
+        assert used == plain
+        h2, m2 = A.speculation_stats()
+        assert m2 - m1 >= 1, (m1, m2)
+        check(M.m, cp, ri, cv, key, val, yk.to_host(c), yv.to_host(c), "after a wrong guess")
     finally:
         A.set_bucket_range(0, -1)
         A.set_small_x_path(True)

@@ -127,13 +127,13 @@ def test_bucket_counts(two_graphs):
     yk = sa.DeviceArray(n, np.int32)
     yv = sa.DeviceArray(n, np.float32)
     L = sa.lib()
-    for b in (2, 16, 256, 0):
+    for b in (2, 16, 256, 1024, 0):
         say(f"bucket slot with {b} buckets")
         M.A.set_sparse_x(dk, dv, key.size)
         c = ctypes.c_int(-1)
         assert L.spmspv_csc_bucket(M.A.h, 1.0, ctypes.byref(c), yk.ptr, yv.ptr, b) == 0
         check(n, cp, ri, cv, key, val, yk.to_host(c.value), yv.to_host(c.value), f"{b} buckets")
-    for bad in (1, 3, 512, -4):
+    for bad in (1, 3, 2048, -4):
         assert L.spmspv_csc_bucket(M.A.h, 1.0, None, yk.ptr, yv.ptr, bad) == -20
 
 
