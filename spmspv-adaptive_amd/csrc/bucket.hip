@@ -14,7 +14,8 @@
 // level on a power-law matrix, where equal row ranges put a tenth of them
 // into the first bin. Per call:
 //   1. k_bin_scatter: the load-balanced gather's merge-path tiles (k_prep's
-//      split), kBinTPW per workgroup round, form every product (row, a*x), find
+//      split, kBinS of its tiles at a time), up to kBinTPW per workgroup
+//      round, form every product (row, a*x), find
 //      its bin (a table of row blocks, then a search among the bins that
 //      start inside the row's block) and write it into
 //      the bin at positions the round reserves per bin with one global
@@ -45,12 +46,15 @@ namespace spmspv {
 
 namespace {
 
-// merge-path tiles per scatter workgroup round: each round reserves its
-// slots with one device-scope atomic per bin it fills, and those atomics
-// (about 17 G/s on the chip, one address taking them in turn) bound the
-// kernel, so a round takes as many tiles as the registers hold
-constexpr int kBinTPW = 4;
-static_assert(kBinTPW * kLbTile < 65536 && kBinMax < 65535, "bin and rank share a word");
+// The scatter: 1,024-thread workgroups whose merge-path tile is kBinS of
+// the prep's tiles (one latency chain for 7,168 merged positions), and up
+// to kBinTPW such tiles per round. Each round reserves its slots with one
+// device-scope atomic per bin it fills; those atomics (about 17 G/s on the
+// chip, one address taking them in turn at about 16 ns) are what large
+// rounds save.
+constexpr int kScatNT = 1024, kBinS = 4, kBinTPW = 2, kScatGrid = 256;
+static_assert(kScatNT * kLbVT == kBinS * kLbTile, "the scatter's tile is kBinS prep tiles");
+static_assert(kBinTPW * kBinS * kLbTile < 65536 && kBinMax < 65535, "bin and rank share a word");
 constexpr int kAccNT = 1024;        // accumulate workgroup
 constexpr int kAccPPT = 16;         // products per thread per sub-round
 constexpr int kAccGrid = 2048;      // accumulate workgroups at most
@@ -61,22 +65,25 @@ constexpr size_t accum_lds_bytes(int rows, int nbins) {
   return (sizeof(unsigned long long) + sizeof(unsigned)) * (size_t)rows + sizeof(int) * (3 * (size_t)nbins + 1);
 }
 
-// The products of merge-path tile t of the load-balanced gather (k_prep's
-// split; kLbTile merged positions: x-entry markers and products). For the
-// thread's k-th item q = threadIdx.x + k * NT < np: idx[k] = its CSC index,
-// xs[k] = its x value (0 without xval). Returns np; b0 gets the index of the
-// tile's first product. The LDS tables are the caller's; the function's two
-// barriers order their use, and the caller syncs before the next tile.
-template <int NT, int VT>
+// The products of merge-path tile t of the load-balanced gather, a tile of
+// S of k_prep's tiles (T = NT * VT = S * kLbTile merged positions: x-entry
+// markers and products; split[t * S] and split[min(t * S + S, ntiles)] are
+// its first and end markers). For the thread's k-th item q = threadIdx.x +
+// k * NT < np: idx[k] = its CSC index, xi[k] = its x entry. Returns np. The
+// LDS tables are the caller's; the function's two barriers order their
+// use, and the caller syncs before the next tile.
+template <int NT, int VT, int S>
 __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr, const int* __restrict__ xbeg,
-                                       const float* __restrict__ xval, const int* __restrict__ split, int xnnz,
-                                       long long W, int* s_ptr, int* s_beg, float* s_xv, int* s_seg,
-                                       int (&idx)[VT], float (&xs)[VT], long long* b0out) {
+                                       const int* __restrict__ split, long long ntiles, int xnnz, long long W,
+                                       int* s_ptr, int* s_beg, unsigned short* s_seg, int (&idx)[VT],
+                                       int (&xi)[VT]) {
   constexpr int T = NT * VT;
+  static_assert(T == S * kLbTile, "a tile is S of the prep's tiles");
+  static_assert(T + 2 < 65536, "segment indices fit 16 bits");
   const long long d0 = t * T;
   const long long d1 = min(d0 + T, W);
-  const int a0 = split[t];
-  const int a1 = split[t + 1];
+  const int a0 = split[t * S];
+  const int a1 = split[std::min<long long>(t * S + S, ntiles)];
   const long long b0 = d0 - a0;
   const long long b1 = d1 - a1;
   const int sa = a0 > 0 ? a0 - 1 : 0;
@@ -84,7 +91,6 @@ __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr,
   {
     constexpr int SEGPT = (T + 2 + NT - 1) / NT;
     int pv[SEGPT], bv[SEGPT];
-    float xv[SEGPT];
 #pragma unroll
     for (int u = 0; u < SEGPT; ++u) {
       const int k = threadIdx.x + u * NT;
@@ -92,7 +98,6 @@ __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr,
       const int ic = i < xnnz ? i : (xnnz > 0 ? xnnz - 1 : 0);
       pv[u] = ptr[i];
       bv[u] = xbeg[ic];
-      xv[u] = xval ? xval[ic] : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < SEGPT; ++u) {
@@ -100,7 +105,6 @@ __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr,
       if (k < nseg) {
         s_ptr[k] = pv[u];
         s_beg[k] = bv[u];
-        s_xv[k] = xv[u];
       }
     }
   }
@@ -121,7 +125,7 @@ __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr,
       if (i < a1 && (long long)i + s_ptr[i - sa] == D) {
         ++i;
       } else {
-        s_seg[p - b0] = i - 1 - sa;
+        s_seg[p - b0] = (unsigned short)(i - 1 - sa);
         ++p;
       }
       ++D;
@@ -135,9 +139,8 @@ __device__ __forceinline__ int lb_tile(long long t, const int* __restrict__ ptr,
     const bool ok = q0 < np;
     const int sg = ok ? s_seg[q0] : 0;
     idx[k] = ok ? s_beg[sg] + (int)(b0 + q0 - s_ptr[sg]) : 0;
-    xs[k] = ok ? s_xv[sg] : 0.f;
+    xi[k] = ok ? sa + sg : 0;
   }
-  *b0out = b0;
   return np;
 }
 
@@ -158,7 +161,9 @@ __device__ __forceinline__ int bin_of(const int* s_start, const int* s_tbl, int 
 }
 
 // Dynamic LDS: the bin starts (nbins + 1), the row-block table (ntbl), the
-// round's counts and bases (nbins each).
+// round's counts and bases (nbins each). A round is kBinTPW merge-path
+// tiles of kBinS prep tiles each when x is large, one when it is small (a
+// round's tiles run one after another).
 template <int NT, int VT>
 __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr, const int* __restrict__ xbeg,
                                                     const float* __restrict__ xval, const int* __restrict__ split,
@@ -169,8 +174,7 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
   constexpr int T = NT * VT;
   __shared__ int s_ptr[T + 2];
   __shared__ int s_beg[T + 2];
-  __shared__ float s_xv[T + 2];
-  __shared__ int s_seg[T];
+  __shared__ unsigned short s_seg[T];
   extern __shared__ int s_bins[];
   const int ntbl = ws.ntbl, shift = ws.tbl_shift;
   int* s_start = s_bins;
@@ -180,8 +184,10 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
   for (int b = threadIdx.x; b <= nbins; b += NT) s_start[b] = ws.start[b];
   for (int b = threadIdx.x; b < ntbl; b += NT) s_tbl[b] = ws.tbl[b];
   const long long ntiles = ctl->n_tiles;
+  const long long nbig = (ntiles + kBinS - 1) / kBinS;
   const long long W = (long long)xnnz + ctl->bin_len;
-  for (long long rd = blockIdx.x; rd * kBinTPW < ntiles; rd += gridDim.x) {
+  const int tpw = nbig > (long long)gridDim.x ? kBinTPW : 1;
+  for (long long rd = blockIdx.x; rd * tpw < nbig; rd += gridDim.x) {
     for (int b = threadIdx.x; b < nbins; b += NT) s_c[b] = 0u;
     __syncthreads();
     // every product of the round in registers, with its bin and its rank
@@ -191,20 +197,19 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
     unsigned br[kBinTPW][VT];
 #pragma unroll
     for (int u = 0; u < kBinTPW; ++u) {
-      const long long t = rd * kBinTPW + u;
+      const long long t = rd * tpw + u;
 #pragma unroll
       for (int k = 0; k < VT; ++k) br[u][k] = ~0u;
-      if (t < ntiles) {
-        int idx[VT];
-        float xs[VT];
-        long long b0;
-        const int np = lb_tile<NT, VT>(t, ptr, xbeg, xval, split, xnnz, W, s_ptr, s_beg, s_xv, s_seg, idx, xs, &b0);
+      if (u < tpw && t < nbig) {
+        int idx[VT], xi[VT];
+        const int np = lb_tile<NT, VT, kBinS>(t, ptr, xbeg, split, ntiles, xnnz, W, s_ptr, s_beg, s_seg, idx, xi);
         if (np > 0) {
-          float a[VT];
+          float a[VT], xs[VT];
 #pragma unroll
           for (int k = 0; k < VT; ++k) {
             row[u][k] = (unsigned)crow[idx[k]];
             a[k] = cval[idx[k]];
+            xs[k] = xval[xi[k]];
           }
 #pragma unroll
           for (int k = 0; k < VT; ++k) {
@@ -215,8 +220,8 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
             }
           }
         }
+        __syncthreads();  // the tables are reused by the next tile
       }
-      __syncthreads();  // the tables are reused by the next tile
     }
     for (int b = threadIdx.x; b < nbins; b += NT) {
       const unsigned c = s_c[b];
@@ -501,17 +506,19 @@ void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* 
   // dynamic LDS beyond 64 KB (many bins, wide windows) needs the attribute
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_scatter<kLbNT, kLbVT>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_scatter<kScatNT, kLbVT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(sizeof(int) * (3 * (size_t)kBinMax + 1 + kBinTbl)));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_accum<kAccNT, kAccPPT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)accum_lds_bytes(1 << kBinRowBits, kBinMax));
   });
+  // one workgroup per CU (its LDS allows one), one big tile per round while
+  // they are fewer than the workgroups, else kBinTPW per round
   const long long tiles = tiles_hint > 0 ? tiles_hint : ((long long)xnnz + binlen_cap) / kLbTile + 2;
-  const int rgrid = (int)std::max<long long>(1, std::min<long long>(2048, (tiles + kBinTPW - 1) / kBinTPW));
+  const int rgrid = (int)std::max<long long>(1, std::min<long long>(kScatGrid, (tiles + kBinS - 1) / kBinS));
   const size_t slds = sizeof(int) * (3 * (size_t)nbins + 1 + ws.ntbl);
-  hipLaunchKernelGGL((k_bin_scatter<kLbNT, kLbVT>), dim3(rgrid), dim3(kLbNT), slds, s, ptr, xbeg, xval, split, crow,
+  hipLaunchKernelGGL((k_bin_scatter<kScatNT, kLbVT>), dim3(rgrid), dim3(kScatNT), slds, s, ptr, xbeg, xval, split, crow,
                      cval, ctl, xnnz, nbins, ws, bkey, bval, g);
   // one chunk per workgroup where the grid allows; with bin_len unknown on
   // the host, two per bin on average (the workgroups loop over the rest)

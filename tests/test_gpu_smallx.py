@@ -140,7 +140,7 @@ def test_autotuned_thresholds_then_parity(uni):
     M.A.set_sparse_x(dk, dv, key.size)
     tuned = M.A.autotune(reps=2)
     print(tuned, flush=True)
-    assert tuned["buckets"] in (16, 32, 64, 128, 256)
+    assert tuned["buckets"] in (256, 512, 1024)
     assert 0 <= tuned["small_x_products"] <= CAP
     assert 0 < tuned["one_workgroup_sort"] <= 3072
     yk = sa.DeviceArray(n, np.int32)
