@@ -14,12 +14,12 @@
 // level on a power-law matrix, where equal row ranges put a tenth of them
 // into the first bin. Per call:
 //   1. k_bin_scatter: the load-balanced gather's merge-path tiles (k_prep's
-//      split, kBinS of its tiles at a time), up to kBinTPW per workgroup
-//      round, form every product (row, a*x), find
-//      its bin (a table of row blocks, then a search among the bins that
-//      start inside the row's block) and write it into
-//      the bin at positions the round reserves per bin with one global
-//      atomic on the bin's cursor (one cursor per 128-byte line);
+//      split, kBinS of its tiles at a time), one per workgroup round, form
+//      every product (row, a*x), find its bin (a table of row blocks, then a
+//      search among the bins that start inside the row's block) and rank it
+//      there in LDS; the round reserves each bin's slots with one global
+//      atomic on the bin's cursor (one cursor per 128-byte line), places its
+//      products bin by bin in LDS and writes them out in that order;
 //   2. k_bin_accum: every workgroup reads the cursors (products per bin) and
 //      plans the chunks itself: a bin is cut into chunks of kBinChunk
 //      products (one round of registers), so a bin that an uneven x fills
@@ -47,14 +47,13 @@ namespace spmspv {
 namespace {
 
 // The scatter: 1,024-thread workgroups whose merge-path tile is kBinS of
-// the prep's tiles (one latency chain for 7,168 merged positions), and up
-// to kBinTPW such tiles per round. Each round reserves its slots with one
-// device-scope atomic per bin it fills; those atomics (about 17 G/s on the
-// chip, one address taking them in turn at about 16 ns) are what large
-// rounds save.
-constexpr int kScatNT = 1024, kBinS = 4, kBinTPW = 2, kScatGrid = 256;
+// the prep's tiles (one latency chain for 7,168 merged positions), one
+// such tile per round. Each round reserves its slots with one device-scope
+// atomic per bin it fills (about 17 G/s on the chip, one address taking
+// them in turn at about 16 ns).
+constexpr int kScatNT = 1024, kBinS = 4, kScatGrid = 2048;
 static_assert(kScatNT * kLbVT == kBinS * kLbTile, "the scatter's tile is kBinS prep tiles");
-static_assert(kBinTPW * kBinS * kLbTile < 65536 && kBinMax < 65535, "bin and rank share a word");
+static_assert(kBinS * kLbTile < 65536 && kBinMax < 65535, "bin and rank share a word");
 constexpr int kAccNT = 1024;        // accumulate workgroup
 constexpr int kAccPPT = 16;         // products per thread per sub-round
 constexpr int kAccGrid = 2048;      // accumulate workgroups at most
@@ -161,9 +160,14 @@ __device__ __forceinline__ int bin_of(const int* s_start, const int* s_tbl, int 
 }
 
 // Dynamic LDS: the bin starts (nbins + 1), the row-block table (ntbl), the
-// round's counts and bases (nbins each). A round is kBinTPW merge-path
-// tiles of kBinS prep tiles each when x is large, one when it is small (a
-// round's tiles run one after another).
+// round's per-bin counts, turned into offsets (nbins + 1), and reserved
+// bases (nbins). A round is one merge-path tile of kBinS prep tiles: its
+// products are ranked per bin in LDS, the round reserves each bin's slots
+// with one device-scope atomic, places the products bin by bin in LDS (the
+// merge tables are free by then) and writes them out in that order, so
+// consecutive lanes store consecutive slots of a bin (the texture
+// addresser's cost follows the lines a store touches, and products stored
+// straight from the lanes touched one line each).
 template <int NT, int VT>
 __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr, const int* __restrict__ xbeg,
                                                     const float* __restrict__ xval, const int* __restrict__ split,
@@ -172,74 +176,90 @@ __global__ __launch_bounds__(NT) void k_bin_scatter(const int* __restrict__ ptr,
                                                     unsigned* __restrict__ okey, float* __restrict__ oval, Gate g) {
   if (gate_shut(g)) return;
   constexpr int T = NT * VT;
+  constexpr int BPT = (kBinMax + 1 + NT - 1) / NT;
   __shared__ int s_ptr[T + 2];
   __shared__ int s_beg[T + 2];
   __shared__ unsigned short s_seg[T];
+  __shared__ int s_w[NT / kWave];
   extern __shared__ int s_bins[];
   const int ntbl = ws.ntbl, shift = ws.tbl_shift;
   int* s_start = s_bins;
   int* s_tbl = s_start + nbins + 1;
   unsigned* s_c = reinterpret_cast<unsigned*>(s_tbl + ntbl);
-  unsigned* s_base = s_c + nbins;
+  unsigned* s_base = s_c + nbins + 1;
   for (int b = threadIdx.x; b <= nbins; b += NT) s_start[b] = ws.start[b];
   for (int b = threadIdx.x; b < ntbl; b += NT) s_tbl[b] = ws.tbl[b];
   const long long ntiles = ctl->n_tiles;
   const long long nbig = (ntiles + kBinS - 1) / kBinS;
   const long long W = (long long)xnnz + ctl->bin_len;
-  const int tpw = nbig > (long long)gridDim.x ? kBinTPW : 1;
-  for (long long rd = blockIdx.x; rd * tpw < nbig; rd += gridDim.x) {
-    for (int b = threadIdx.x; b < nbins; b += NT) s_c[b] = 0u;
-    __syncthreads();
-    // every product of the round in registers, with its bin and its rank
-    // there packed into one word (bin << 16 | rank; ~0u: no product)
-    unsigned row[kBinTPW][VT];
-    float val[kBinTPW][VT];
-    unsigned br[kBinTPW][VT];
-#pragma unroll
-    for (int u = 0; u < kBinTPW; ++u) {
-      const long long t = rd * tpw + u;
-#pragma unroll
-      for (int k = 0; k < VT; ++k) br[u][k] = ~0u;
-      if (u < tpw && t < nbig) {
-        int idx[VT], xi[VT];
-        const int np = lb_tile<NT, VT, kBinS>(t, ptr, xbeg, split, ntiles, xnnz, W, s_ptr, s_beg, s_seg, idx, xi);
-        if (np > 0) {
-          float a[VT], xs[VT];
-#pragma unroll
-          for (int k = 0; k < VT; ++k) {
-            row[u][k] = (unsigned)crow[idx[k]];
-            a[k] = cval[idx[k]];
-            xs[k] = xval[xi[k]];
-          }
-#pragma unroll
-          for (int k = 0; k < VT; ++k) {
-            val[u][k] = a[k] * xs[k];
-            if ((int)threadIdx.x + k * NT < np) {
-              const int b = bin_of(s_start, s_tbl, ntbl, shift, nbins, row[u][k]);
-              br[u][k] = ((unsigned)b << 16) | atomicAdd(&s_c[b], 1u);
-            }
-          }
-        }
-        __syncthreads();  // the tables are reused by the next tile
-      }
-    }
-    for (int b = threadIdx.x; b < nbins; b += NT) {
-      const unsigned c = s_c[b];
-      if (c) s_base[b] = (unsigned)ws.base[b] + atomicAdd(ws.cursor + (long long)b * kBinCurStride, c);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < kBinTPW; ++u) {
+  for (long long t = blockIdx.x; t < nbig; t += gridDim.x) {
+    for (int b = threadIdx.x; b <= nbins; b += NT) s_c[b] = 0u;
+    int idx[VT], xi[VT];
+    const int np = lb_tile<NT, VT, kBinS>(t, ptr, xbeg, split, ntiles, xnnz, W, s_ptr, s_beg, s_seg, idx, xi);
+    unsigned row[VT];
+    float val[VT];
+    unsigned br[VT];  // bin << 16 | rank in the bin (~0u: no product)
+    {
+      float a[VT], xs[VT];
 #pragma unroll
       for (int k = 0; k < VT; ++k) {
-        if (br[u][k] != ~0u) {
-          const unsigned p = s_base[br[u][k] >> 16] + (br[u][k] & 0xFFFFu);
-          okey[p] = row[u][k];
-          oval[p] = val[u][k];
+        const bool ok = (int)threadIdx.x + k * NT < np;
+        row[k] = ok ? (unsigned)crow[idx[k]] : 0u;
+        a[k] = ok ? cval[idx[k]] : 0.f;
+        xs[k] = ok ? xval[xi[k]] : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < VT; ++k) {
+        val[k] = a[k] * xs[k];
+        br[k] = ~0u;
+        if ((int)threadIdx.x + k * NT < np) {
+          const int b = bin_of(s_start, s_tbl, ntbl, shift, nbins, row[k]);
+          br[k] = ((unsigned)b << 16) | atomicAdd(&s_c[b], 1u);
         }
       }
     }
-    __syncthreads();  // s_c and s_base are reused by the next round
+    __syncthreads();
+    // reserve each bin's slots, and the bins' offsets in the round
+    {
+      unsigned c[BPT];
+      int sum = 0;
+#pragma unroll
+      for (int j = 0; j < BPT; ++j) {
+        const int b = threadIdx.x * BPT + j;
+        c[j] = b < nbins ? s_c[b] : 0u;
+        sum += (int)c[j];
+        if (c[j]) s_base[b] = (unsigned)ws.base[b] + atomicAdd(ws.cursor + (long long)b * kBinCurStride, c[j]);
+      }
+      int tot;
+      unsigned pre = (unsigned)block_excl_sum<NT, int>(sum, s_w, &tot);
+#pragma unroll
+      for (int j = 0; j < BPT; ++j) {
+        const int b = threadIdx.x * BPT + j;
+        if (b <= nbins) s_c[b] = pre;  // every thread read its counts before the scan's barriers
+        pre += c[j];
+      }
+    }
+    __syncthreads();
+    unsigned* s_row = reinterpret_cast<unsigned*>(s_ptr);
+    float* s_val = reinterpret_cast<float*>(s_beg);
+#pragma unroll
+    for (int k = 0; k < VT; ++k) {
+      if (br[k] != ~0u) {
+        const unsigned b = br[k] >> 16;
+        const unsigned q = s_c[b] + (br[k] & 0xFFFFu);
+        s_row[q] = row[k];
+        s_val[q] = val[k];
+        s_seg[q] = (unsigned short)b;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += NT) {
+      const unsigned b = s_seg[i];
+      const unsigned p = s_base[b] + ((unsigned)i - s_c[b]);
+      okey[p] = s_row[i];
+      oval[p] = s_val[i];
+    }
+    __syncthreads();  // the tables, counts and bases are reused by the next round
   }
 }
 
@@ -508,16 +528,15 @@ void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* 
   std::call_once(attr_once, [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_scatter<kScatNT, kLbVT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(sizeof(int) * (3 * (size_t)kBinMax + 1 + kBinTbl)));
+                              (int)(sizeof(int) * (3 * (size_t)kBinMax + 2 + kBinTbl)));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_accum<kAccNT, kAccPPT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)accum_lds_bytes(1 << kBinRowBits, kBinMax));
   });
-  // one workgroup per CU (its LDS allows one), one big tile per round while
-  // they are fewer than the workgroups, else kBinTPW per round
+  // one round per big tile, the workgroups looping when there are more
   const long long tiles = tiles_hint > 0 ? tiles_hint : ((long long)xnnz + binlen_cap) / kLbTile + 2;
   const int rgrid = (int)std::max<long long>(1, std::min<long long>(kScatGrid, (tiles + kBinS - 1) / kBinS));
-  const size_t slds = sizeof(int) * (3 * (size_t)nbins + 1 + ws.ntbl);
+  const size_t slds = sizeof(int) * (3 * (size_t)nbins + 2 + ws.ntbl);
   hipLaunchKernelGGL((k_bin_scatter<kScatNT, kLbVT>), dim3(rgrid), dim3(kScatNT), slds, s, ptr, xbeg, xval, split, crow,
                      cval, ctl, xnnz, nbins, ws, bkey, bval, g);
   // one chunk per workgroup where the grid allows; with bin_len unknown on
