@@ -73,7 +73,17 @@ def profile_slot(sa, A, n, args, yk, yv):
     if args.spmv_type >= 0:  # a K5/K7 format pinned for the profile
         A.set_spmv_type(args.spmv_type)
         A.set_sigma(args.sigma)
-        tuned = A.tuned() if tuned is not None else None
+    if args.buckets:  # the bucket slot's bin count pinned (one call sets it)
+        import ctypes
+        kx, dkx, dvx = xs[dens[0]]
+        A.set_sparse_x(dkx, dvx, kx)
+        c = ctypes.c_int(-1)
+        sa.lib().spmspv_csc_bucket(A.h, 1.0, ctypes.byref(c), yk.ptr, yv.ptr, args.buckets)
+    if args.bucket_range:  # the bucket rule pinned, so that every profiled run takes the same slots
+        lo, hi = (int(v) for v in args.bucket_range.split(","))
+        A.set_bucket_range(lo, hi)
+    if tuned is not None:
+        tuned = A.tuned()
     if args.no_small_x:
         A.set_small_x_path(False)
     bls = {}
@@ -490,6 +500,9 @@ def main():
     ap.add_argument("--spmv-type", type=int, default=-1,
                     help="profiling mode: K5/K7 format (0 merge-path, 1 CSR5), as reported under autotuned")
     ap.add_argument("--sigma", type=int, default=-1, help="profiling mode: CSR5 sigma")
+    ap.add_argument("--buckets", type=int, default=0, help="profiling mode: the bucket slot's bin count")
+    ap.add_argument("--bucket-range", default="",
+                    help="profiling mode: the bucket rule's bin_len range lo,hi (as reported under autotuned)")
     ap.add_argument("--flush-mb", type=int, default=512)
     ap.add_argument("--only-slot", default="",
                     help="profiling mode: time these slots (comma list) at the --density values only "
