@@ -125,7 +125,7 @@ int oracle_spmspv_f64(int m, const int* col_ptr, const int* row_idx,
   /* Column-driven accumulation of SpmSpvCSCHost (test_sort_spmspv.cu:342-351)
    * in FP64. bound[i] = sum |a_ij x_j| doubles as the structural mark: it
    * is set to -1 before any touch, so a touched row has bound >= 0 even when
-   * every product is 0. */
+   * every product is 0, or NaN when a product is NaN. */
   for (int i = 0; i < m; ++i) {
     y64[i] = 0.0;
     bound[i] = -1.0;
@@ -143,7 +143,8 @@ int oracle_spmspv_f64(int m, const int* col_ptr, const int* row_idx,
   }
   int idx = 0;
   for (int i = 0; i < m; ++i) {
-    if (bound[i] >= 0.0) {
+    /* touched: bound >= 0, or NaN once a NaN product came in (not < 0) */
+    if (!(bound[i] < 0.0)) {
       if (y_key) y_key[idx] = i;
       if (y_val) y_val[idx] = y64[i];
       if (y_bound) y_bound[idx] = bound[i];

@@ -108,6 +108,59 @@ def test_zero_products_mark_rows(rmat16):
         check(M.m, cp, ri, cv, key, val, yk, yv, f"zero x values {mode} xnnz {k}")
 
 
+def test_csc_only_handle_builds_bins(rmat16):
+    """A handle given only the CSC (no row pointer) counts the rows' nonzeros
+    from the CSC's row indices to lay out its bins; results match the
+    oracle for a few bin counts."""
+    M, (cp, ri, cv) = rmat16
+    m, n, nnz = M.m, M.n, M.nnz
+    d = (sa.DeviceArray.from_host(cp, np.int32), sa.DeviceArray.from_host(ri, np.int32),
+         sa.DeviceArray.from_host(cv, np.float32))
+    A = sa.SpmspvHandle(m, n, nnz)
+    try:
+        A.InputCSC(nnz, d[1], d[0], d[2])
+        A.allocPreBuffer()
+        for it, (k, b) in enumerate([(3000, 256), (40000, 1024), (65536, 2)]):
+            key, val = sa.gen_sparse_vector(n, k, 8700 + it)
+            dk = sa.DeviceArray.from_host(key, np.int32)
+            dv = sa.DeviceArray.from_host(val, np.float32)
+            A.set_sparse_x(dk, dv, k)
+            yk = sa.DeviceArray(m, np.int32)
+            yv = sa.DeviceArray(m, np.float32)
+            c = ctypes.c_int(-1)
+            say(f"bucket slot on a CSC-only handle, {b} bins, xnnz {k}")
+            assert sa.lib().spmspv_csc_bucket(A.h, 1.0, ctypes.byref(c), yk.ptr, yv.ptr, b) == 0
+            check(m, cp, ri, cv, key, val, yk.to_host(c.value), yv.to_host(c.value), f"CSC only, {b} bins")
+    finally:
+        A.Destroy()
+
+
+def test_inf_and_nan_products(rmat16):
+    """An x value of +Inf (and one of NaN) makes its column's products Inf or
+    NaN: the chunks holding them sum in FP32 instead of fixed point, and
+    every row's value is the oracle's (Inf, NaN and finite values alike);
+    bins without such products keep their fixed-point sums."""
+    M, (cp, ri, cv) = rmat16
+    n = M.n
+    key, val = sa.gen_sparse_vector(n, 20000, 8800)
+    val = val.copy()
+    full = np.nonzero(cp[key + 1] > cp[key])[0]  # x entries whose column holds nonzeros
+    val[full[5]] = np.inf
+    val[full[9000]] = np.nan
+    say("bucket slot, Inf and NaN in x")
+    yk, yv = bucket(M, key, val, 256)
+    ek, ev, eb = oracle.spmspv_f64(M.m, cp, ri, cv, key, val)
+    np.testing.assert_array_equal(yk, ek)
+    y = yv.astype(np.float64)
+    assert np.array_equal(np.isnan(y), np.isnan(ev)), "NaN rows differ"
+    fin = np.isfinite(ev)
+    assert np.array_equal(np.isinf(y) & ~np.isnan(ev), np.isinf(ev)), "Inf rows differ"
+    inf = np.isinf(ev)
+    assert np.array_equal(np.sign(y[inf]), np.sign(ev[inf]))
+    assert np.all(np.abs(y[fin] - ev[fin]) <= TOL * eb[fin] + 1e-30)
+    assert fin.sum() > 0 and (~fin).sum() > 0
+
+
 def test_repeat_calls_leave_counts_clear(rmat16):
     """The per-bin counts, the accumulator and the marks return to zero
     after every call: alternating sizes give the oracle's result each time."""

@@ -114,6 +114,23 @@ def test_spmspv_structural_support_keeps_exact_zeros():
     assert k32.size == 0
 
 
+def test_spmspv_structural_support_keeps_inf_and_nan_rows():
+    # a NaN or Inf x value still makes its column's rows part of the result:
+    # row 0 gets NaN (1*NaN), row 1 Inf (4*Inf), row 2 NaN (Inf - Inf), row 3 finite
+    dense = np.array([[1, 0, 0], [0, 4, 0], [0, 1, -1], [0, 0, 0]], np.float32)
+    dense[3, 2] = 2.0
+    C = sp.csc_matrix(dense)
+    C.sort_indices()
+    x = np.array([np.nan, np.inf, np.inf], np.float32)
+    yk, yv, yb = oracle.spmspv_f64(4, C.indptr, C.indices, C.data, np.array([0, 1, 2]), x)
+    np.testing.assert_array_equal(yk, [0, 1, 2, 3])
+    assert np.isnan(yv[0]) and yv[1] == np.inf and np.isnan(yv[2]) and yv[3] == np.inf
+    # only x = (NaN at column 0): row 0 alone, NaN
+    yk, yv, _ = oracle.spmspv_f64(4, C.indptr, C.indices, C.data, np.array([0]), x[:1])
+    np.testing.assert_array_equal(yk, [0])
+    assert np.isnan(yv[0])
+
+
 def test_spmspv_column_sum_identity():
     A = rand_csr(200, 150, 0.05, 11)
     C = A.tocsc()
