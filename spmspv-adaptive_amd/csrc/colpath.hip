@@ -964,6 +964,7 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
   __shared__ unsigned long long s_pre;
   __shared__ int s_i[NT / kWave];
   __shared__ int s_rows[kTile];
+  __shared__ float s_vals[kTile];
   const unsigned ntiles = (unsigned)(((long long)m + kTile - 1) / kTile);
   const unsigned t = blockIdx.x;
   if (t >= ntiles) {
@@ -986,12 +987,30 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
     if (BYTES) b = bytes_to_bits8(*reinterpret_cast<const uint2*>(marks + r0));
     else b = (bitmap[r0 >> 5] >> (r0 & 31)) & 0xFFu;
   }
+  // the thread's 8 sums, loaded while the scan and the look-back run (the
+  // 8 rows are consecutive: two 16-byte loads when they all exist)
+  float av[kRows];
+#pragma unroll
+  for (int u = 0; u < kRows; ++u) av[u] = 0.f;
+  const bool whole = r0 + kRows <= m && (reinterpret_cast<uintptr_t>(acc + r0) & 15u) == 0;
+  if (b && yval) {
+    if (whole) {
+      const float4 lo = *reinterpret_cast<const float4*>(acc + r0);
+      const float4 hi = *reinterpret_cast<const float4*>(acc + r0 + 4);
+      av[0] = lo.x; av[1] = lo.y; av[2] = lo.z; av[3] = lo.w;
+      av[4] = hi.x; av[5] = hi.y; av[6] = hi.z; av[7] = hi.w;
+    } else {
+#pragma unroll
+      for (int u = 0; u < kRows; ++u)
+        if ((b >> u) & 1u) av[u] = acc[r0 + u];
+    }
+  }
   const int cnt = __popc(b);
   int btot;
   const int bex = block_excl_sum<NT, int>(cnt, s_i, &btot);
   const unsigned long long pre = tile_prefix(state, t, epoch, (unsigned long long)btot, &s_pre);
-  // rows of the tile land in LDS at their local rank, then leave as
-  // contiguous, coalesced writes
+  // rows of the tile and their sums land in LDS at their local rank, then
+  // leave as contiguous, coalesced writes
   int loc = bex;
   if (b) {
     if (BYTES) {
@@ -1001,10 +1020,18 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
       // the word before any of them clears it
       bitmap[r0 >> 5] = 0u;
     }
-    while (b) {
-      const int bit = __ffs(b) - 1;
-      b &= b - 1;
-      s_rows[loc++] = r0 + bit;
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+      if ((b >> u) & 1u) {
+        s_rows[loc] = r0 + u;
+        s_vals[loc] = av[u];
+        ++loc;
+        if (reset_acc && !whole) acc[r0 + u] = 0.0f;
+      }
+    }
+    if (reset_acc && whole) {
+      *reinterpret_cast<float4*>(acc + r0) = make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4*>(acc + r0 + 4) = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
   __syncthreads();
@@ -1025,8 +1052,9 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NT;
       key[u] = keymap ? keymap[(unsigned)r[u]] : r[u];
-      v[u] = yval ? acc[(unsigned)r[u]] : 0.f;
+      v[u] = s_vals[i < btot ? i : 0];
     }
     // one wait for all loads here; otherwise every store behind a branch
     // below waits for the whole memory counter, stores included
@@ -1045,7 +1073,6 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
           if (ykey) ykey[o] = key[u];
           if (yval) yval[o] = v[u];
         }
-        if (reset_acc) acc[(unsigned)r[u]] = 0.0f;
       }
     }
   }
