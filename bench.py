@@ -375,6 +375,15 @@ def run_batched_bench(sa, args):
         A.selector_set_weights(np.load(WEIGHTS).astype(np.float32))
     else:
         A.selector_mode(1)
+    # per-matrix tuning on a dense x, as for the sweep's matrix (tile widths,
+    # CSR5 width, the bucket slot's bins and its rule's range)
+    tuned = None
+    if not args.no_autotune:
+        dk = sa.DeviceArray.from_host(np.arange(n, dtype=np.int32), np.int32)
+        dv = sa.DeviceArray.from_host(np.ones(n, np.float32), np.float32)
+        A.set_sparse_x(dk, dv, n)
+        tuned = A.autotune(reps=2)
+        del dk, dv
     deg = np.diff(rp).astype(np.int64)
     rng = np.random.default_rng(0x5EED0006)
     dens = np.exp(rng.uniform(np.log(1e-4), np.log(0.2), nvec))
@@ -456,7 +465,7 @@ def run_batched_bench(sa, args):
         if cpu_t > 5.0:
             break
     return {"graph": f"rmat{scale}", "nnz": int(M.nnz), "vectors": nvec,
-            "density": "log-uniform [1e-4, 0.2]", "total_bin_len": prods,
+            "density": "log-uniform [1e-4, 0.2]", "total_bin_len": prods, "autotuned": tuned,
             "batched_ms": round(b_ms, 3), "sequential_ms": round(s_ms, 3),
             "speedup": round(s_ms / b_ms, 3),
             "batched_gbs": round(sum(alg_bytes(keys[v].size, bls[v], int(yn[v])) for v in range(nvec))
