@@ -1729,7 +1729,7 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   // replaces; the range spans the sizes where it is more than 2 % faster
   // than all four. Each end is then narrowed by two bisection steps
   // (geometric) between the last size where the bucket slot loses and the
-  // first where it wins, and set half-way between the two.
+  // first where it wins, and set three quarters of the way to the losing one.
   {
     h->bucket_lo = 0;
     h->bucket_hi = -1;
@@ -1767,7 +1767,11 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
         if (w) b = bl;
         else a = bl;
       }
-      *e = (long long)std::sqrt((double)a * (double)b);
+      // three quarters of the way (geometrically) towards the losing size:
+      // just past the crossover the bucket slot loses by little, while the
+      // selector sees a sampled bin_len (a few tens of percent off) for a
+      // large x
+      *e = (long long)std::exp(0.75 * std::log((double)a) + 0.25 * std::log((double)b));
       return SPMSPV_SUCCESS;
     };
     long long prev_lose = -1, lo = -1, hi = -1, first_win = -1, last_win = -1;
