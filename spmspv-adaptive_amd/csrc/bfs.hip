@@ -114,12 +114,17 @@ __global__ void k_bfs_init(int* levels, int n, unsigned* visited, unsigned* fb0,
 // sig[1..8] first: the host polls one word per level instead of waiting for
 // the stream and three copies.
 __global__ void k_bfs_signal(const int* __restrict__ cnt, int nxt, const Ctl* __restrict__ ctl, long long* sig) {
-  if (threadIdx.x < 16) reinterpret_cast<float*>(sig + 1)[threadIdx.x] = ctl->feat[threadIdx.x];
+  // the feature words are acknowledged (each writer waits for its store)
+  // before the level word goes out; the host reads nothing else, so no L2
+  // write-back is needed
+  if (threadIdx.x < 16) {
+    reinterpret_cast<float*>(sig + 1)[threadIdx.x] = ctl->feat[threadIdx.x];
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     const long long w = ((long long)(cnt[2] & 0xffffff) << 40) | ((long long)(ctl->kernel_choice & 0xff) << 32) |
                         (long long)(unsigned)cnt[nxt];
-    __threadfence_system();
     __hip_atomic_store(sig, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }

@@ -58,6 +58,7 @@ constexpr int kAccNT = 1024;        // accumulate workgroup
 constexpr int kAccPPT = 16;         // products per thread per sub-round
 constexpr int kAccGrid = 2048;      // accumulate workgroups at most
 constexpr int kFewRows = 16;        // bins of at most this many rows sum in registers
+constexpr int kMaxDevices = 64;     // devices whose kernel attributes are tracked
 
 // LDS of the accumulate kernel for a window of rows rows and nbins bins
 constexpr size_t accum_lds_bytes(int rows, int nbins) {
@@ -524,8 +525,11 @@ void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* 
                    unsigned* bkey, float* bval, float* acc, unsigned char* marks, long long tiles_hint,
                    long long binlen_hint, long long binlen_cap, Gate g) {
   // dynamic LDS beyond 64 KB (many bins, wide windows) needs the attribute
-  static std::once_flag attr_once;
-  std::call_once(attr_once, [] {
+  // once per device (the attribute belongs to the device's copy of the kernels)
+  static std::once_flag attr_once[kMaxDevices];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::call_once(attr_once[dev % kMaxDevices], [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_scatter<kScatNT, kLbVT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(sizeof(int) * (3 * (size_t)kBinMax + 2 + kBinTbl)));

@@ -971,10 +971,7 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
     // no rows: an empty result
     if (ntiles == 0 && t == 0 && threadIdx.x == 0) {
       ctl->ynnz = 0;
-      if (sig) {
-        __threadfence_system();
-        *sig = 0;
-      }
+      if (sig) *sig = 0;
     }
     return;
   }
@@ -1098,7 +1095,9 @@ __global__ __launch_bounds__(NT) void k_compact(unsigned* __restrict__ bitmap,
           __hip_atomic_fetch_add(word, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + mine;
       if ((unsigned)(now >> 32) == ntiles) {
         __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence_system();
+        // every workgroup's device-coherent outputs were acknowledged before
+        // its arrival: the count needs no L2 write-back (a system fence would
+        // write back the whole L2 of this XCD)
         *sig = (long long)(unsigned)now;
       }
     }
@@ -1119,10 +1118,11 @@ void launch_compact(hipStream_t s, unsigned* bitmap, unsigned char* marks, int m
 }
 
 // ------------------------------------------------------------ utilities
+// The host reads only the word itself (device results stay in device
+// memory, read by later launches), so no L2 write-back precedes it.
 __global__ void k_signal_int(const int* src, long long* dst, Gate g) {
   if (gate_shut(g)) return;
   const long long v = *src;
-  __threadfence_system();
   *dst = v;
 }
 void launch_signal_int(hipStream_t s, const int* src, long long* dst, Gate g) {

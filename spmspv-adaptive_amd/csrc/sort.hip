@@ -118,7 +118,9 @@ __device__ __forceinline__ V block_seg_excl(bool f, V v, int* s_f, V* s_v, V* la
 // (called by all threads with the tile's head count) returns the number of
 // heads of all earlier tiles. Segment ends whose head lies in an earlier
 // tile go to carry_val/carry_idx[tile] for the fix-up kernel.
-template <int NT, int IPT, typename PrefixFn>
+// With COHERENT the outputs go out as device-coherent stores (visible to
+// every XCD and to copies once acknowledged, no L2 write-back needed).
+template <int NT, int IPT, typename PrefixFn, bool COHERENT = false>
 __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const float* s_v, int cnt,
                                                  unsigned prev_key, bool global_start,
                                                  PrefixFn get_prefix, int* ykey, float* yval,
@@ -154,14 +156,20 @@ __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const floa
     if (head[j]) {
       ++hc;
       acc = s_v[i];
-      ykey[gpre + hc - 1] = (int)s_k[i];
+      if (COHERENT)
+        __hip_atomic_store(ykey + gpre + hc - 1, (int)s_k[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        ykey[gpre + hc - 1] = (int)s_k[i];
     } else {
       acc += s_v[i];
     }
     const bool end = (i == cnt - 1) || (s_k[i + 1] != s_k[i]);
     if (end) {
       if (hc > 0) {
-        yval[gpre + hc - 1] = acc;
+        if (COHERENT)
+          __hip_atomic_store(yval + gpre + hc - 1, acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+          yval[gpre + hc - 1] = acc;
       } else {
         carry_val[tile] = acc;
         carry_idx[tile] = (int)(gpre - 1);
@@ -809,7 +817,7 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
     }
   }
   auto no_prefix = [](int) -> long long { return 0; };
-  const long long tot = tile_reduce<NT, IPT>(s_k, s_v, (int)bl, 0u, true, no_prefix, ykey, yval, nullptr,
+  const long long tot = tile_reduce<NT, IPT, decltype(no_prefix), true>(s_k, s_v, (int)bl, 0u, true, no_prefix, ykey, yval, nullptr,
                                              nullptr, 0, s_i, s_f, s_fv);
   // ---- publish: outputs complete, then the count and the done flag
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -819,7 +827,8 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
     ctl->ynnz = (int)tot;
     ctl->sort_small = 1;
     ctl->kernel_choice = choice | kSelDone;
-    __threadfence_system();
+    // the outputs are device-coherent and acknowledged (the wait above, in
+    // every thread, before the barrier): no L2 write-back before the words
     if (count_sig) *count_sig = tot;
     if (sel.stash) *sel.stash = sel_word(tot, choice | kSelDone);
   }
