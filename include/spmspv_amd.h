@@ -259,7 +259,9 @@ int spmspv_autotune_report(spmspv_handle_t h, int out[10]);
  * In spmspv_run_auto and spmspv_run_batched, a choice of slot 2, 3, 5 or 7
  * becomes the bucket slot (8) when lo <= bin_len <= hi (bin_len sampled for
  * a large x); hi < lo turns it off (the default). spmspv_autotune sets the
- * range where the bucket slot beat those four on sub-vectors of x. */
+ * range where the bucket slot beat those four on sub-vectors of x (off when
+ * the matrix needs more row bins than the slot holds); a new matrix
+ * (InputCSR / InputCSC) turns it off again. */
 int spmspv_set_bucket_range(spmspv_handle_t h, long long lo, long long hi);
 
 /* ---- learned selector (no reference counterpart in C++: the reference's

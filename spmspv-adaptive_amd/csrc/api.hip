@@ -797,6 +797,8 @@ int spmspv_input_csr(spmspv_handle_t h, int nnz, int* row_ptr, int* col, float* 
   h->csr5_ready = false;
   h->mat_ready = false;
   h->bin_base_bits = -1;
+  h->bucket_lo = 0;  // the bucket rule's range belongs to the previous matrix
+  h->bucket_hi = -1;
   // a CSC this handle transposed from the previous CSR no longer matches
   if (h->csc_owned) h->csc_stale = true;
   h->prep_valid = false;
@@ -822,6 +824,8 @@ int spmspv_input_csc(spmspv_handle_t h, int nnz, int* row, int* col_ptr, float* 
   h->csc_stale = false;
   h->mat_ready = false;
   h->bin_base_bits = -1;
+  h->bucket_lo = 0;
+  h->bucket_hi = -1;
   h->prep_valid = false;
   h->binlen_host = -1;
   return SPMSPV_SUCCESS;
@@ -1635,19 +1639,25 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
     if (type == 1 && (rc = ensure_csr5(h))) return rc;
   }
   h->small_x_path = saved_small;
-  // the bucket slot's number of row buckets
+  // the bucket slot's number of row bins, among the counts whose layout
+  // fits kBinMax bins (a matrix of very many rows may fit none: the slot and
+  // its rule then stay off)
+  bool bucket_ok = false;
   {
     float bbest = 1e30f;
     int bpick = h->bucket_bits;
     for (int bb = 8; bb <= 10; ++bb) {
       h->bucket_bits = bb;
+      if (ensure_bin_layout(h) != SPMSPV_SUCCESS) continue;
       if ((rc = time_slot(SPMSPV_K_COL_BUCKET, &t))) return rc;
+      bucket_ok = true;
       if (better(t, bbest)) {
         bbest = t;
         bpick = bb;
       }
     }
     h->bucket_bits = bpick;
+    if (!bucket_ok) h->err.clear();
   }
   // Thresholds of the single-workgroup paths, timed on sub-vectors of x
   // whose bin_len lies just under each candidate: the largest candidate at
@@ -1730,7 +1740,7 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
   // than all four. Each end is then narrowed by two bisection steps
   // (geometric) between the last size where the bucket slot loses and the
   // first where it wins, and set three quarters of the way to the losing one.
-  {
+  if (bucket_ok) {
     h->bucket_lo = 0;
     h->bucket_hi = -1;
     // wins at a sub-vector whose bin_len is just under target: 1 yes, 0 no,
@@ -1801,6 +1811,9 @@ int spmspv_autotune2(spmspv_handle_t h, int reps, int csr5, int* report) {
       h->bucket_lo = lo;
       h->bucket_hi = done ? hi : LLONG_MAX;  // still ahead at the largest size: no upper end
     }
+  } else {
+    h->bucket_lo = 0;
+    h->bucket_hi = -1;
   }
   if ((rc = spmspv_set_sparse_x(h, xk, xv, xn))) return rc;
   // captured BFS levels hold the old tiling and thresholds

@@ -294,6 +294,59 @@ def test_speculated_bucket_calls(rmat16):
         A.set_small_x_path(True)
 
 
+def test_new_matrix_turns_rule_off(rmat16):
+    """The bucket rule's range belongs to the matrix it was tuned on:
+    InputCSR turns it off."""
+    M, _ = rmat16
+    A = M.A
+    try:
+        A.set_bucket_range(1000, 10 ** 9)
+        assert A.tuned()["bucket_rule"] == [1000, 10 ** 9]
+        A.InputCSR(M.nnz, M.d_row_ptr, M.d_col, M.d_val)
+        A.ToCSC()
+        assert A.tuned()["bucket_rule"] == [0, -1]
+    finally:
+        A.set_bucket_range(0, -1)
+
+
+def test_autotune_with_too_many_rows_for_bins(gpu):
+    """2^25 rows need more than the bucket slot's 6,144 bins of at most
+    4,096 rows: the autotuner leaves the slot and its rule off and tunes the
+    rest; an explicit bucket call says INVALID_ARGUMENT; auto calls still
+    match the oracle."""
+    rng = np.random.default_rng(21)
+    m = n = 1 << 25
+    nz = 200000
+    rows = np.sort(rng.integers(0, m, nz))
+    cols = rng.integers(0, n, nz).astype(np.int32)
+    rp = np.zeros(m + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    order = np.lexsort((cols, rows))
+    col = cols[order]
+    val = rng.uniform(-1, 1, nz).astype(np.float32)
+    M = sa.Matrix(m, n, rp, col, val)
+    A = M.A
+    key = np.arange(0, n, 7, dtype=np.int32)
+    dk = sa.DeviceArray.from_host(key, np.int32)
+    dv = sa.DeviceArray.from_host(np.ones(key.size, np.float32), np.float32)
+    A.set_sparse_x(dk, dv, key.size)
+    say("autotune on 2^25 rows")
+    tuned = A.autotune(reps=1)
+    assert tuned["bucket_rule"] == [0, -1]
+    yk = sa.DeviceArray(m, np.int32)
+    yv = sa.DeviceArray(m, np.float32)
+    c = ctypes.c_int(-1)
+    assert sa.lib().spmspv_csc_bucket(A.h, 1.0, ctypes.byref(c), yk.ptr, yv.ptr, 0) == -20
+    cp, ri, cv = oracle.csr_to_csc(m, n, rp, col, val)
+    k = 5000
+    xk, xv = sa.gen_sparse_vector(n, k, 2121)
+    A.set_sparse_x(sa.DeviceArray.from_host(xk, np.int32), sa.DeviceArray.from_host(xv, np.float32), k)
+    cnt, used = A.run_auto(y_key=yk, y_val=yv)
+    assert used != sa.K_COL_BUCKET
+    check(m, cp, ri, cv, xk, xv, yk.to_host(cnt), yv.to_host(cnt), "2^25 rows, auto")
+
+
 def test_autotuned_rule_then_parity(rmat16):
     """The autotuner's range is off or ordered; auto calls across densities
     match the oracle whichever slot the rule sends them to."""
