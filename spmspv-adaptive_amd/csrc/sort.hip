@@ -21,20 +21,26 @@ namespace spmspv {
 
 // ---------------------------------------------------------------- ranking
 // Items of a tile are laid out per wave: wave w owns positions
-// [w*64*IPT, (w+1)*64*IPT); item j of lane l sits at w*64*IPT + j*64 + l.
-// On return cnt (this wave's 256 counters) holds the per-digit counts of
-// the wave's chunk and rank[j] the stable rank of item j among equal digits
-// inside the chunk.
+// [w*64*IPT, (w+1)*64*IPT); item j of lane l sits at w*64*IPT + j*64 + l
+// (IPT may be a run-time nj <= IPT: slots j >= nj hold no item). On return
+// cnt (this wave's 256 counters) holds the per-digit counts of the wave's
+// chunk and rank[j] the stable rank of item j among equal digits inside
+// the chunk.
 template <int IPT>
 __device__ __forceinline__ void wave_rank(const unsigned (&key)[IPT], const bool (&valid)[IPT],
                                           int shift, unsigned* cnt, int (&rank)[IPT],
-                                          unsigned (&dig)[IPT]) {
+                                          unsigned (&dig)[IPT], int nj = IPT) {
   for (int k = lane(); k < 256; k += kWave) cnt[k] = 0u;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const uint64_t lt = lane() == 0 ? 0ull : (~0ull >> (kWave - lane()));
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
+    if (j >= nj) {  // slots past the tile's fill (wave-uniform): no items
+      rank[j] = 0;
+      dig[j] = 0u;
+      continue;
+    }
     const unsigned d = valid[j] ? ((key[j] >> shift) & 255u) : 0u;
     uint64_t peers = __ballot(valid[j]);
 #pragma unroll
@@ -125,8 +131,9 @@ __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const floa
                                                  unsigned prev_key, bool global_start,
                                                  PrefixFn get_prefix, int* ykey, float* yval,
                                                  float* carry_val, int* carry_idx, long long tile,
-                                                 int* s_i, int* s_f, float* s_fv) {
-  const int i0 = threadIdx.x * IPT;
+                                                 int* s_i, int* s_f, float* s_fv, int ipt = IPT) {
+  // thread t reduces sorted items t*ipt .. t*ipt+ipt-1 (ipt <= IPT)
+  const int i0 = threadIdx.x * ipt;
   bool head[IPT];
   int nh = 0;
   float tail = 0.f;
@@ -134,7 +141,7 @@ __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const floa
   for (int j = 0; j < IPT; ++j) {
     const int i = i0 + j;
     head[j] = false;
-    if (i < cnt) {
+    if (j < ipt && i < cnt) {
       const unsigned pk = (i == 0) ? prev_key : s_k[i - 1];
       head[j] = (i == 0 && global_start) || s_k[i] != pk;
       if (head[j]) tail = s_v[i];
@@ -152,7 +159,7 @@ __device__ __forceinline__ long long tile_reduce(const unsigned* s_k, const floa
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = i0 + j;
-    if (i >= cnt) break;
+    if (j >= ipt || i >= cnt) break;
     if (head[j]) {
       ++hc;
       acc = s_v[i];
@@ -737,11 +744,15 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
     return;
   }
   __syncthreads();  // s_ptr / s_beg complete
+  // products per thread for this call (<= IPT): every wave gets its share,
+  // so that the ranking's per-slot round trips run on all the waves at once
+  // instead of IPT of them in a row on the first few
+  const int ie = (int)((bl + NT - 1) / NT);
   // ---- product -> x entry map, merge-path style: thread t fills products
-  // t*IPT .. t*IPT+IPT-1 from one search and a forward walk (x entries with
+  // t*ie .. t*ie+ie-1 from one search and a forward walk (x entries with
   // empty columns are stepped over)
   {
-    const int p0 = threadIdx.x * IPT;
+    const int p0 = threadIdx.x * ie;
     if (p0 < bl) {
       int lo = 0, hi = xnnz;  // s_ptr[lo] <= p0 < s_ptr[hi]
       while (hi - lo > 1) {
@@ -753,7 +764,7 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
 #pragma unroll
       for (int k = 0; k < IPT; ++k) {
         const int p = p0 + k;
-        if (p < bl) {
+        if (k < ie && p < bl) {
           while (nxt <= p) nxt = s_ptr[++e + 1];
           s_seg[p] = e;
         }
@@ -772,8 +783,8 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
   float xv[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const int p = w * kWave * IPT + j * kWave + lane();
-    valid[j] = p < bl;
+    const int p = w * kWave * ie + j * kWave + lane();
+    valid[j] = j < ie && p < bl;
     const int e = valid[j] ? s_seg[p] : 0;
     pos[j] = valid[j] ? s_beg[e] + (p - s_ptr[e]) : 0;
     xv[j] = valid[j] ? xval[e] : 0.f;
@@ -787,7 +798,7 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
   // ---- LDS radix sort (stable), as k_sort_small; the digit offsets by
   // threads 0..255 (one per digit)
   for (int pss = 0; pss < passes; ++pss) {
-    wave_rank<IPT>(key, valid, 8 * pss, s_cnt + w * 256, rank, dig);
+    wave_rank<IPT>(key, valid, 8 * pss, s_cnt + w * 256, rank, dig, ie);
     __syncthreads();
     unsigned tot = 0;
     if (threadIdx.x < 256) tot = digit_wave_offsets<NW>(s_cnt);
@@ -807,7 +818,7 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
     if (pss + 1 < passes) {
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
-        const int gi = w * kWave * IPT + j * kWave + lane();
+        const int gi = w * kWave * ie + j * kWave + lane();
         if (valid[j]) {
           key[j] = s_k[gi];
           val[j] = s_v[gi];
@@ -818,7 +829,7 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
   }
   auto no_prefix = [](int) -> long long { return 0; };
   const long long tot = tile_reduce<NT, IPT, decltype(no_prefix), true>(s_k, s_v, (int)bl, 0u, true, no_prefix, ykey, yval, nullptr,
-                                             nullptr, 0, s_i, s_f, s_fv);
+                                             nullptr, 0, s_i, s_f, s_fv, ie);
   // ---- publish: outputs complete, then the count and the done flag
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
