@@ -384,6 +384,8 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
   const long long n = *n_dev;
   if (n > TILE || n > cap) return;
   const int w = threadIdx.x / kWave;
+  // items per thread for this n (<= IPT): every wave ranks its share
+  const int ie = (int)((n + NT - 1) / NT);
   unsigned key[IPT];
   float val[IPT];
   bool valid[IPT];
@@ -391,13 +393,13 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
   unsigned dig[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const int gi = w * kWave * IPT + j * kWave + lane();
-    valid[j] = gi < n;
+    const int gi = w * kWave * ie + j * kWave + lane();
+    valid[j] = j < ie && gi < n;
     key[j] = valid[j] ? kin[gi] : 0u;
     val[j] = valid[j] && vin ? vin[gi] : 0.f;  // vin null: keys only
   }
   for (int p = 0; p < passes; ++p) {
-    wave_rank<IPT>(key, valid, 8 * p, s_cnt + w * 256, rank, dig);
+    wave_rank<IPT>(key, valid, 8 * p, s_cnt + w * 256, rank, dig, ie);
     __syncthreads();
     const unsigned tot = digit_wave_offsets<NW>(s_cnt);
     unsigned dtot;
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
     if (p + 1 < passes) {
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
-        const int gi = w * kWave * IPT + j * kWave + lane();
+        const int gi = w * kWave * ie + j * kWave + lane();
         if (valid[j]) {
           key[j] = s_k[gi];
           val[j] = s_v[gi];
@@ -435,7 +437,7 @@ __global__ __launch_bounds__(NT) void k_sort_small(const unsigned* __restrict__ 
   }
   auto no_prefix = [](int) -> long long { return 0; };
   const long long tot = tile_reduce<NT, IPT>(s_k, s_v, cnt, 0u, true, no_prefix, ykey, yval,
-                                             nullptr, nullptr, 0, s_i, s_f, s_fv);
+                                             nullptr, nullptr, 0, s_i, s_f, s_fv, ie);
   if (threadIdx.x == 0) {
     ctl->ynnz = (int)tot;
     ctl->sort_small = 1;
