@@ -654,6 +654,23 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
   float* s_v = reinterpret_cast<float*>(s_raw + CAP * 4);
   float* s_w = reinterpret_cast<float*>(s_raw);
   int* s_seg = reinterpret_cast<int*>(s_raw);
+  // the selector's weights and matrix features, loaded beside the column
+  // scan (stored to LDS only when the products do not fit, after the scan:
+  // their LDS space is the sort's)
+  constexpr int kWPT = (kNWeights + NT - 1) / NT;
+  constexpr int kMF = sizeof(MatFeat) / sizeof(double);
+  float wr[kWPT];
+  double mfr = 0.0;
+  if (sel.mode != 2 && sel.mat) {
+    if (sel.weights) {
+#pragma unroll
+      for (int i = 0; i < kWPT; ++i) {
+        const int idx = threadIdx.x + i * NT;
+        wr[i] = idx < kNWeights ? sel.weights[idx] : 0.f;
+      }
+    }
+    if (threadIdx.x < kMF) mfr = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
+  }
   // ---- column scan of x
   const int base = threadIdx.x * KPT;
   int c[KPT], beg[KPT], len[KPT];
@@ -725,17 +742,14 @@ __global__ __launch_bounds__(NT) void k_col_fused(const int* __restrict__ colptr
     }
     // the selector, as the last tile of k_prep runs it
     {
-      constexpr int kWPT = (kNWeights + NT - 1) / NT;
-      constexpr int kMF = sizeof(MatFeat) / sizeof(double);
       if (sel.weights) {
 #pragma unroll
         for (int i = 0; i < kWPT; ++i) {
           const int idx = threadIdx.x + i * NT;
-          if (idx < kNWeights) s_w[wlds_index(idx)] = sel.weights[idx];
+          if (idx < kNWeights) s_w[wlds_index(idx)] = wr[i];
         }
       }
-      if (threadIdx.x < kMF)
-        reinterpret_cast<double*>(&s_mf)[threadIdx.x] = reinterpret_cast<const double*>(sel.mat)[threadIdx.x];
+      if (threadIdx.x < kMF) reinterpret_cast<double*>(&s_mf)[threadIdx.x] = mfr;
       __syncthreads();
       SelectorArgs sl = sel;
       if (sel.weights) sl.weights = s_w;
