@@ -261,7 +261,10 @@ int spmspv_autotune_report(spmspv_handle_t h, int out[10]);
  * a large x); hi < lo turns it off (the default). spmspv_autotune sets the
  * range where the bucket slot beat those four on sub-vectors of x (off when
  * the matrix needs more row bins than the slot holds); a new matrix
- * (InputCSR / InputCSC) turns it off again. */
+ * (InputCSR / InputCSC) turns it off again. Setting a range (hi >= lo) on a
+ * matrix builds the slot's row bins (one host round trip the first time)
+ * and is refused with SPMSPV_INVALID_ARGUMENT, the rule left off, when the
+ * matrix needs more than 6,144 bins. */
 int spmspv_set_bucket_range(spmspv_handle_t h, long long lo, long long hi);
 
 /* ---- learned selector (no reference counterpart in C++: the reference's

@@ -1498,6 +1498,17 @@ int spmspv_autotune_report(spmspv_handle_t h, int out[10]) {
 
 int spmspv_set_bucket_range(spmspv_handle_t h, long long lo, long long hi) {
   if (!h) return set_error(nullptr, SPMSPV_INVALID_ARGUMENT, "null handle");
+  if (hi >= lo && (h->csr_row || h->csc_colptr)) {
+    // a rule the slot cannot serve on this matrix (too many row bins) would
+    // fail every auto call in its range: refused, the rule left off
+    int rc = ensure_arena(h);
+    if (!rc) rc = ensure_bin_layout(h);
+    if (rc) {
+      h->bucket_lo = 0;
+      h->bucket_hi = -1;
+      return rc;
+    }
+  }
   h->bucket_lo = lo;
   h->bucket_hi = hi;
   return SPMSPV_SUCCESS;

@@ -334,6 +334,9 @@ def test_autotune_with_too_many_rows_for_bins(gpu):
     say("autotune on 2^25 rows")
     tuned = A.autotune(reps=1)
     assert tuned["bucket_rule"] == [0, -1]
+    # a range set by hand is refused: the slot cannot serve this matrix
+    assert sa.lib().spmspv_set_bucket_range(A.h, 0, 1 << 62) == -20
+    assert A.tuned()["bucket_rule"] == [0, -1]
     yk = sa.DeviceArray(m, np.int32)
     yv = sa.DeviceArray(m, np.float32)
     c = ctypes.c_int(-1)
