@@ -546,7 +546,10 @@ void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* 
   // one chunk per workgroup where the grid allows; with bin_len unknown on
   // the host, two per bin on average (the workgroups loop over the rest)
   const long long extra = binlen_hint >= 0 ? std::min(binlen_hint, binlen_cap) / kBinChunk : nbins;
-  const int agrid = (int)std::max<long long>(1, std::min<long long>(kAccGrid, nbins + extra));
+  // no more workgroups than fit at once: one that loops over further chunks
+  // keeps its plan, where a fresh one would build it again
+  const int ares = resident_blocks(reinterpret_cast<const void*>(k_bin_accum<kAccNT, kAccPPT>), kAccNT);
+  const int agrid = (int)std::max<long long>(1, std::min<long long>(std::min(kAccGrid, ares), nbins + extra));
   hipLaunchKernelGGL((k_bin_accum<kAccNT, kAccPPT>), dim3(agrid), dim3(kAccNT), accum_lds_bytes(ws.max_rows, nbins),
                      s, bkey, bval, ws, nbins, acc, marks, g);
 }
