@@ -539,7 +539,11 @@ void launch_bucket(hipStream_t s, const int* ptr, const int* xbeg, const float* 
   });
   // one round per big tile, the workgroups looping when there are more
   const long long tiles = tiles_hint > 0 ? tiles_hint : ((long long)xnnz + binlen_cap) / kLbTile + 2;
-  const int rgrid = (int)std::max<long long>(1, std::min<long long>(kScatGrid, (tiles + kBinS - 1) / kBinS));
+  // no more workgroups than fit at once: each loads the bin tables once and
+  // loops over its rounds
+  const int sres = resident_blocks(reinterpret_cast<const void*>(k_bin_scatter<kScatNT, kLbVT>), kScatNT);
+  const int rgrid = (int)std::max<long long>(
+      1, std::min<long long>(std::min(kScatGrid, sres), (tiles + kBinS - 1) / kBinS));
   const size_t slds = sizeof(int) * (3 * (size_t)nbins + 2 + ws.ntbl);
   hipLaunchKernelGGL((k_bin_scatter<kScatNT, kLbVT>), dim3(rgrid), dim3(kScatNT), slds, s, ptr, xbeg, xval, split, crow,
                      cval, ctl, xnnz, nbins, ws, bkey, bval, g);
